@@ -1,0 +1,194 @@
+"""Benchmark of the OVERLAP fermion hot path on one MI355X.
+
+Workload (default): SU(3) overlap fermions on a 24^3 lattice, U(1) background with 8 flux
+quanta, 256 mode functions, Chebyshev sign of degree 384 (cut 0.05), rho = 1.2, dt = 0.02.
+A step is one full leapfrog step of all modes: two overlap half steps (each an overlap
+apply of every mode plus the axpy into the other field) and two gauge-link half steps.
+
+value = site*mode updates/s of the mixed FP32/FP64 form (inputs resident in HBM); the plain
+FP64 form and the CPU oracle are timed beside it.  Prints one JSON line on stdout.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from overlap_amd import configs as cf  # noqa: E402
+from overlap_amd import ovl  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "site·mode updates/s + %HBM roofline, mixed vs FP64 GPU at ≤10⁻¹⁰ overlap residual"
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def step_opts(c: cf.Config, prec, graph, tol=1e-10, chunk=0):
+    kind = ovl.OVERLAP if c.kind == "overlap" else ovl.WILSON
+    so = ovl.sign_opts(degree=max(c.cheb_n, 1), cut=c.cheb_cut, precision=prec, tol=tol, mode_chunk=chunk)
+    return ovl.StepOpts(kind=kind, dt=c.dt, gauge=ovl.GAUGE_LINKS, sign=so, use_graph=int(graph))
+
+
+def cpu_baseline(c: cf.Config, links, e, psi, threads: int, modes: int):
+    """The oracle (plain C FP64 restatement, OpenMP over modes) on a bounded sample:
+    `modes` mode functions, one full leapfrog step, on `threads` host threads."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import orc
+    L = orc.lat(c.nc, c.n, c.r_wilson, c.m_fermion, c.m_dw)
+    kind = orc.OVERLAP_CHEB if c.kind == "overlap" else orc.WILSON
+    o = orc.StepOpts(kind, c.dt, 1, c.nb, c.q, c.cheb_n, c.cheb_cut, 0.0, 0.0, threads)
+    sample = psi[:modes]
+    t0 = time.perf_counter()
+    orc.leapfrog(L, links, e, sample, sample, o, 1)
+    dt = time.perf_counter() - t0
+    return c.volume * modes / dt, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=4)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="su3_overlap_24", choices=sorted(cf.CONFIGS))
+    ap.add_argument("--modes", type=int, default=0, help="override the mode count (test runs only)")
+    ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--chunk", type=int, default=0, help="mode blocks per recurrence pass (0 = all)")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-modes", type=int, default=16)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        sys.exit(2)
+
+    c = cf.CONFIGS[args.workload]
+    if args.modes:
+        c = cf.Config(**{**c.__dict__, "nmodes": args.modes})
+    log("device", ovl.device_name())
+    log("workload", c.name, "n", c.n, "modes", c.nmodes)
+    t0 = time.perf_counter()
+    links, e, psi = c.generate()
+    log(f"inputs generated in {time.perf_counter() - t0:.1f}s")
+    ctx = ovl.Context(c.nc, c.n, c.nmodes, r_wilson=c.r_wilson, m_fermion=c.m_fermion, m_dw=c.m_dw)
+    ctx.set_links(links)
+    ctx.set_efield(e)
+    if c.nb:
+        ctx.set_u1_flux(c.nb, c.q)
+    ctx.set_modes(ovl.PSI, psi)
+    ctx.set_modes(ovl.PSIMID, psi)
+    del psi
+
+    # --- accuracy of the mixed form against the plain FP64 form (one overlap apply, all modes)
+    prec_mixed = ovl.PREC_MIXED
+    so64 = step_opts(c, ovl.PREC_FP64, False).sign
+    som = step_opts(c, prec_mixed, False, args.tol, args.chunk).sign
+    res = {}
+    if c.kind == "overlap":
+        ctx.apply_overlap(ovl.PSI, ovl.SCRATCH, so64)
+        ref = ctx.get_modes(ovl.SCRATCH, 0, min(16, c.nmodes))
+        ctx.apply_overlap(ovl.PSI, ovl.SCRATCH, som)
+        mix = ctx.get_modes(ovl.SCRATCH, 0, min(16, c.nmodes))
+        tm = ctx.timing()
+        res["rel_err_mixed_vs_fp64"] = float(max(np.linalg.norm(a - b) / np.linalg.norm(b) for a, b in zip(mix, ref)))
+        res["fp64_head_steps"] = int(tm.fp64_steps // max(1, 1))
+        res["fp32_tail_steps"] = int(tm.fp32_steps)
+        del ref, mix
+        log(f"mixed vs fp64 rel err {res['rel_err_mixed_vs_fp64']:.3e}  head {tm.fp64_steps} tail {tm.fp32_steps}")
+
+    # --- warmup + timed region: mixed form, per-launch HIP events on the library stream
+    so = step_opts(c, prec_mixed, False, args.tol, args.chunk)
+    for _ in range(args.warmup):
+        ctx.fermion_update(so, 1)
+    ctx.set_profiling(True)
+    t0 = time.perf_counter()
+    ctx.fermion_update(so, args.steps)
+    wall = time.perf_counter() - t0
+    ctx.set_profiling(False)
+    t = ctx.timing()
+    ms_per_step = 1e3 * wall / args.steps
+    value = c.volume * c.nmodes * args.steps / wall
+    log(f"mixed: {ms_per_step:.1f} ms/step  {value:.4e} site*mode/s  (events {t.total_ms / args.steps:.1f} ms/step)")
+
+    # dominant stencil variant of the timed region
+    var = int(np.argmax([t.var_ms[i] for i in range(3)]))
+    names = ["k_hop FP64", "k_hop FP32", "k_hop FP32->FP64"]
+    avg_ms = t.var_ms[var] / max(1, t.var_launches[var])
+    bytes_per_launch = t.var_bytes[var] / max(1, t.var_launches[var])
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    alg_step_bytes = None
+    if c.kind == "overlap":
+        S = 4 * c.nc * 2 * 4
+        Lb = 3 * (4 if c.nc == 2 else 18) * 4
+        M = c.nmodes
+        n = c.cheb_n
+        per_half = n * (4 * S + 2 * Lb / M) + (2 * S + Lb / M) + 3 * S
+        alg_step_bytes = 2 * per_half * c.volume * M
+    roofline = {"bound": "hbm", "kernel": names[var], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "avg_launch_us": round(avg_ms * 1e3, 2), "alg_bytes_per_launch": bytes_per_launch,
+                "launches": int(t.var_launches[var]),
+                "kernel_share_of_step": round(t.var_ms[var] / max(t.total_ms, 1e-9), 3)}
+    if alg_step_bytes:
+        roofline["step_alg_bytes"] = alg_step_bytes
+        roofline["step_alg_frac"] = round(alg_step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+
+    # --- graph-captured replay of the same step
+    sog = step_opts(c, prec_mixed, True, args.tol, args.chunk)
+    t0 = time.perf_counter()
+    ctx.fermion_update(sog, args.steps)
+    graph_ms = 1e3 * (time.perf_counter() - t0) / args.steps
+    log(f"mixed (hipGraph): {graph_ms:.1f} ms/step")
+
+    # --- plain FP64 form
+    fp64 = None
+    if not args.no_fp64:
+        so64s = step_opts(c, ovl.PREC_FP64, False)
+        ctx.fermion_update(so64s, 1)
+        k64 = max(1, args.steps // 2)
+        t0 = time.perf_counter()
+        ctx.fermion_update(so64s, k64)
+        w64 = time.perf_counter() - t0
+        fp64 = {"value": c.volume * c.nmodes * k64 / w64, "ms_per_step": 1e3 * w64 / k64, "steps": k64}
+        log(f"fp64: {fp64['ms_per_step']:.1f} ms/step")
+
+    # --- sign-function residual of the mixed form on the current state
+    if c.kind == "overlap":
+        res["sign_residual_mixed"] = ctx.sign_residual(ovl.PSI, som)
+        log(f"sign residual (mixed) {res['sign_residual_mixed']:.3e}")
+    ctx.close()
+
+    cpu = None
+    if not args.no_cpu:
+        links, e, psi = c.generate(nmodes=args.cpu_modes)
+        v, secs = cpu_baseline(c, links, e, psi, args.cpu_threads, args.cpu_modes)
+        cpu = {"value": v, "unit": "site*mode updates/s", "cores": args.cpu_threads, "kind": "port",
+               "sample": f"{args.cpu_modes} modes x 1 full step of {c.name} ({secs:.1f} s), "
+                         f"oracle/liboracle.so FP64 C restatement, OpenMP over modes",
+               "host": platform.processor() or platform.machine()}
+        log(f"cpu baseline {v:.4e} site*mode/s in {secs:.1f}s")
+
+    out = {"metric": METRIC, "value": value, "unit": "site*mode updates/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "fp32+fp64 (mixed)", "data": "synthetic",
+           "config": {"workload": c.name, "nc": c.nc, "lattice": list(c.n), "modes": c.nmodes,
+                      "chebyshev_degree": c.cheb_n, "cut": c.cheb_cut, "rho": c.m_dw, "flux_quanta": c.nb,
+                      "dt": c.dt, "tol": args.tol, "parallelism": "single GPU"},
+           "roofline": roofline, "cpu_baseline": cpu, "fp64_gpu": fp64,
+           "speedup_vs_fp64": (value / fp64["value"]) if fp64 else None,
+           "graph_ms_per_step": graph_ms, "accuracy": res}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,167 @@
+/*
+ * ovl.h -- C ABI of the MI355X (gfx950) implementation of the OVERLAP fermion hot path.
+ *
+ * The reference exposes this path as free C++ functions on GaugeLinks / ElectricFields /
+ * FermionField objects (no plugin or FFI layer).  Each entry point below names the reference
+ * interface it replaces.  All functions return an int status (OVL_OK = 0, negative = error);
+ * ovl_error() gives the message of the last failure on a context.
+ *
+ * Ownership: device memory is owned by the context; host pointers are borrowed for the
+ * duration of the call only.  One context per host thread; every call is ordered on the
+ * context's HIP stream and returns after the stream has drained, except where noted.
+ *
+ * Host layouts are the reference's:
+ *   site      s = x + Nx*(y + Ny*z)                               (FermionField.cpp:31-33)
+ *   links     SU(2): 4 doubles at 4*(mu+3s), U = u3 + i u_a sigma_a (SU2/GroupOperations.cpp:4-8)
+ *             SU(3): 9 complex at 9*(mu+3s), (i,j) at i+3j          (SU3/GroupOperations.cpp:28-30)
+ *   E field   e[a + (Nc^2-1)*(mu+3s)]                              (ElectricFields.cpp:36-41)
+ *   mode      AoS complex psi[(s*Nc + colour)*4 + spin]           (FermionField.cpp:8-9,537-545)
+ *   u_cc      complex ucc[((mu*V + s)*Nc + i)*Nc + j] = U_mu(s)(i,j)  (OverlapEvolutionOperator.cpp:3-4,23-51)
+ * Complex numbers are interleaved (re, im) doubles.
+ */
+#ifndef OVL_H
+#define OVL_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ovl_ctx ovl_ctx;
+
+enum {
+    OVL_OK = 0,
+    OVL_E_ARG = -1,      /* invalid argument (sizes, indices, NULL, non-finite input) */
+    OVL_E_HIP = -2,      /* HIP runtime error / no device */
+    OVL_E_STATE = -3,    /* call order (e.g. no links uploaded) */
+    OVL_E_CONVERGE = -4, /* iterative solver hit its iteration cap */
+    OVL_E_NOMEM = -5,    /* device allocation failed */
+    OVL_E_SPECTRUM = -6  /* invalid spectral interval (lmin <= 0 or lmin >= lmax) */
+};
+
+/* fields held on the device: the reference's Fermions::Psi and Fermions::PsiMid
+ * (FermionField.cpp:628), plus one scratch field of the same shape */
+enum { OVL_PSI = 0, OVL_PSIMID = 1, OVL_SCRATCH = 2 };
+
+enum { OVL_SIGN_CHEBYSHEV = 0, OVL_SIGN_ZOLOTAREV = 1 };
+enum { OVL_PREC_FP64 = 0, OVL_PREC_FP32 = 1, OVL_PREC_MIXED = 2 };
+enum { OVL_WILSON = 0, OVL_OVERLAP = 1 };
+enum { OVL_GAUGE_FROZEN = 0, OVL_GAUGE_LINKS = 1, OVL_GAUGE_FULL = 2 };
+
+typedef struct {
+    int nc;              /* 2 or 3 colours */
+    int n[3];            /* Nx, Ny, Nz (each >= 3) */
+    int n_modes;         /* mode functions held per field */
+    double r_wilson;     /* rWilson   (Simulation.cpp:194) */
+    double m_fermion;    /* mFermion  (Simulation.cpp:194) */
+    double m_dw;         /* MDWHeight (Simulation.cpp:200), the overlap rho */
+    int mode_block;      /* modes per SoA block: 16, 32 or 64; 0 = choose */
+    int reunitarize;     /* 1: project links back onto SU(Nc) after every link update */
+} ovl_params;
+
+/* sign function eps(h_W) options */
+typedef struct {
+    int method;          /* OVL_SIGN_CHEBYSHEV (default) or OVL_SIGN_ZOLOTAREV (reference algorithm) */
+    int precision;       /* OVL_PREC_FP64 / OVL_PREC_FP32 / OVL_PREC_MIXED (Chebyshev only) */
+    int degree;          /* Chebyshev degree n of P(x) ~ x^-1/2 */
+    double cut;          /* spectrum cut: approximation on |h_W| in [cut, 1] * lambda_max */
+    double tol;          /* MIXED: target relative error; selects the FP64 head length */
+    int head;            /* MIXED: FP64 head length (>= 0 overrides tol) */
+    double lmin, lmax;   /* ZOLOTAREV: interval of h_W^2; lmin = lmax = 0 -> Ritz estimate */
+    int mode_chunk;      /* mode blocks per recurrence pass (0 = all) */
+} ovl_sign_opts;
+
+typedef struct {
+    int kind;            /* OVL_WILSON or OVL_OVERLAP */
+    double dt;           /* full leapfrog step */
+    int gauge;           /* OVL_GAUGE_FROZEN / _LINKS (U <- exp(-i dt/2 E) U, E fixed) / _FULL */
+    ovl_sign_opts sign;
+    int use_graph;       /* capture each half step into a hipGraph and replay it */
+} ovl_step_opts;
+
+/* ---- context ---- */
+int ovl_create(const ovl_params *p, ovl_ctx **out);
+int ovl_destroy(ovl_ctx *ctx);
+const char *ovl_error(const ovl_ctx *ctx);
+int ovl_device_name(char *buf, int len);
+void ovl_sign_defaults(ovl_sign_opts *o);
+
+/* ---- gauge field (GaugeLinks / ElectricFields objects, LATTICE/GaugeLinks.cpp:11-89,
+ *      LATTICE/ElectricFields.cpp:9-88) ---- */
+int ovl_set_links(ovl_ctx *ctx, const double *links);
+int ovl_get_links(ovl_ctx *ctx, double *links);
+int ovl_set_efield(ovl_ctx *ctx, const double *e);
+int ovl_get_efield(ovl_ctx *ctx, double *e);
+/* U(1) background of QED::SetConstantMagneticField (QED/QED.cpp:131-153), charge q; folded into
+ * the stencil links (the reference applies it only to its initial eigenbasis) */
+int ovl_set_u1_flux(ovl_ctx *ctx, int nb, double q);
+
+/* ---- mode functions (FermionField, FermionField.cpp:12-623) ---- */
+int ovl_set_modes(ovl_ctx *ctx, int field, const double *aos, int first, int count);
+int ovl_get_modes(ovl_ctx *ctx, int field, double *aos, int first, int count);
+int ovl_copy_field(ovl_ctx *ctx, int src, int dst);
+
+/* ---- operators on device fields, all modes at once ---- */
+/* out = -i h_W in : WilsonEvoOperator::Compute (WilsonOperator.cpp:30-114), batched */
+int ovl_apply_wilson(ovl_ctx *ctx, int in, int out);
+/* out = -i h_ov in : OverlapOperator::Compute (OverlapOperator.cpp:25-64), batched */
+int ovl_apply_overlap(ovl_ctx *ctx, int in, int out, const ovl_sign_opts *o);
+/* out = eps(h_W) in */
+int ovl_apply_sign(ovl_ctx *ctx, int in, int out, const ovl_sign_opts *o);
+/* rel = max over modes ||eps(eps in) - in|| / ||in||  (OverlapSignTest.cpp:155-165) */
+int ovl_sign_residual(ovl_ctx *ctx, int in, const ovl_sign_opts *o, double *rel);
+/* Ritz extreme eigenvalue of +-h_W^2 (RitzSolver.cpp:92-297, ComputeRitzEigenvalues
+ * OverlapUpdate.cpp:137-173); start: flat-layout host vector (m = s + V(spin + 4 colour)) */
+int ovl_ritz(ovl_ctx *ctx, const double *start_flat, double sign, double delitr, int nitlz,
+             int nrestart, double *eigenvalue);
+
+/* ---- single-mode drop-ins with the reference's host arguments ---- */
+/* WilsonEvoOperator::Compute(u_cc*, wcf* phi, wcf* psi) (WilsonOperator.cpp:30) */
+int ovl_wilson_evo_compute(ovl_ctx *ctx, const double *ucc, const double *phi, double *psi);
+/* OverlapOperator::Compute(u_cc*, wcf* phi, wcf* psi, LambdaMin, LambdaMax) (OverlapOperator.cpp:25):
+ * 25-pole Zolotarev, multi-shift CG 1e-16 / 1000 iterations, as the reference */
+int ovl_overlap_operator_compute(ovl_ctx *ctx, const double *ucc, const double *phi, double *psi,
+                                 double lmin, double lmax);
+/* OverlapOperations::ZolotarevCoefficients (ZolotarevCoefficients.cpp:71-175): host arithmetic */
+double ovl_zolotarev_coefficients(double amin, double amax, double errtol, int no, double *bz, double *dz);
+
+/* ---- dynamics ---- */
+/* Dynamics::UpdateGaugeLinks (Dynamics.cpp:97-138): U_mu(x) <- exp(-i dt E_mu(x)) U_mu(x) */
+int ovl_update_links(ovl_ctx *ctx, double dt);
+/* Dynamics::UpdateElectricFields (Dynamics.cpp:145-197): E += dt * (-dH/dU) */
+int ovl_update_efield(ovl_ctx *ctx, double dt);
+/* Dynamics::Fermions::Update(U, E, Psi, PsiMid) (OverlapUpdate.cpp:175-211, WILSON/Update.cpp:48-79),
+ * nsteps times */
+int ovl_fermion_update(ovl_ctx *ctx, const ovl_step_opts *o, int nsteps);
+
+/* ---- observables (FP64, deterministic reductions) ---- */
+int ovl_energy(ovl_ctx *ctx, double *e_el, double *e_mag);              /* Bulk (BulkObservables.cpp) */
+int ovl_gauss_violation(ovl_ctx *ctx, double *vmax, double *vrms);      /* GaussLawViolation.cpp:13-61 */
+int ovl_unitarity_violation(ovl_ctx *ctx, double *vmax, double *vrms);  /* UnitarityViolation.cpp:13-51 */
+/* ChernSimonsNumber::NCsDot (ChernSimonsDerivative.cpp:31-49) = improved sum E.B / (8 pi^2) */
+int ovl_ncs_dot(ovl_ctx *ctx, double *ncsdot);
+/* Fermions::Observables::ComputeCurrents (OverlapCurrents.cpp:5-296): jv, ja [4*site + mu]
+ * (host arrays or NULL), axial charge = sum_x ja[4x]; first_mode = global index of mode 0 */
+int ovl_currents(ovl_ctx *ctx, int field, const ovl_sign_opts *o, int first_mode, double *jv,
+                 double *ja, double *axial_charge);
+/* per-mode squared norms of a field (host array of n_modes doubles) */
+int ovl_mode_norms(ovl_ctx *ctx, int field, double *norm2);
+
+/* ---- timing of the last ovl_apply_* / ovl_fermion_update call ---- */
+typedef struct {
+    double total_ms;          /* event-timed wall time of the call on the stream */
+    double hop_ms;            /* summed event time of the stencil kernels */
+    long long hop_launches;   /* number of stencil kernel launches */
+    double hop_bytes;         /* algorithmic bytes of those launches */
+    int fp64_steps, fp32_steps;
+    /* per stencil-kernel variant: [0] FP64 in/out, [1] FP32 in/out, [2] FP32 in / FP64 out */
+    double var_ms[3];
+    long long var_launches[3];
+    double var_bytes[3];
+} ovl_timing;
+int ovl_get_timing(ovl_ctx *ctx, ovl_timing *t);
+int ovl_set_profiling(ovl_ctx *ctx, int on);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

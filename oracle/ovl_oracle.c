@@ -257,7 +257,7 @@ void orc_su3_iexp(double c, const double *alpha, double *out)
     for (int i = 0; i < 9; i++) { a[i] *= I * c; nrm += creal(a[i] * conj(a[i])); }
     nrm = sqrt(nrm);
     int sq = 0;
-    while (nrm > 0.25) { nrm *= 0.5; sq++; }
+    while (nrm > 0.25 && sq < 64) { nrm *= 0.5; sq++; }
     double scale = ldexp(1.0, -sq);
     for (int i = 0; i < 9; i++) x[i] = a[i] * scale;
     m_id(3, sum);
@@ -782,8 +782,9 @@ static void half_step(const orc_lat *L, const double *ucc, const orc_step_opts *
 }
 
 /* OverlapUpdate.cpp:175-211 / WILSON/Update.cpp:48-79 Dynamics::Fermions::Update, nsteps times.
- * psi, psimid: nmodes consecutive AoS mode functions. */
-void orc_leapfrog(const orc_lat *L, double *links, const double *efield, int nmodes,
+ * psi, psimid: nmodes consecutive AoS mode functions.  update_links: 0 frozen gauge field,
+ * 1 links move with E fixed (the reference), 2 E moves too (orc_efield_update before each link half step). */
+void orc_leapfrog(const orc_lat *L, double *links, double *efield, int nmodes,
                   double *psi, double *psimid, const orc_step_opts *o, int nsteps)
 {
     int V = lat_vol(L), nc = L->nc;
@@ -801,9 +802,11 @@ void orc_leapfrog(const orc_lat *L, double *links, const double *efield, int nmo
     for (int st = 0; st < nsteps; st++) {
         orc_links_to_ucc(L, links, ph, ucc);
         half_step(L, ucc, o, lam, coef, nmodes, psimid, psi);
+        if (o->update_links == 2) orc_efield_update(L, links, efield, 0.5 * o->dt);
         if (o->update_links) orc_update_links(L, links, efield, 0.5 * o->dt);
         orc_links_to_ucc(L, links, ph, ucc);
         half_step(L, ucc, o, lam, coef, nmodes, psi, psimid);
+        if (o->update_links == 2) orc_efield_update(L, links, efield, 0.5 * o->dt);
         if (o->update_links) orc_update_links(L, links, efield, 0.5 * o->dt);
     }
     free(ucc); free(ph); free(coef);
@@ -833,6 +836,45 @@ static void plaq(const orc_lat *L, const double *links, int x, int y, int z, int
     m_mul(nc, p, c, p);
     m_dag(nc, e, e);
     m_mul(nc, p, e, p);
+}
+
+/* E-field half of the gauge leapfrog.  Dynamics.cpp:145-197 updates E from plaquette traces
+ * through macros that are absent from the reference, so the force is restated from the
+ * Hamiltonian H = 1/2 sum E^2 + 2 sum ReTr(1 - U_plaq) with U' = -i E U (Dynamics.cpp:97-138):
+ *   E^a_mu(x) += -2 dt sum_{nu != mu} [ReTr(i t^a U_{mu,nu}(x)) + ReTr(i t^a U_{mu,-nu}(x))]
+ * Pinned by Gauss-law and energy conservation (tests), not by a reference vector. */
+void orc_efield_update(const orc_lat *L, const double *links, double *efield, double dt)
+{
+    int V = lat_vol(L), nc = L->nc, ng = ngen(nc);
+    int d[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    double *f = calloc((size_t)3 * V * ng, sizeof(double));
+    for (int s = 0; s < V; s++) {
+        int x, y, z;
+        coords_of(L->n, s, &x, &y, &z);
+        for (int mu = 0; mu < 3; mu++)
+            for (int nu = 0; nu < 3; nu++) {
+                if (nu == mu) continue;
+                cplx p[9], a[9], b[9], c[9], e[9];
+                double t[8];
+                plaq(L, links, x, y, z, mu, nu, p);
+                retr_itu(nc, p, t);
+                for (int k = 0; k < ng; k++) f[ng * (mu + 3 * s) + k] += t[k];
+                /* U_mu(x) U_nu(x+mu-nu)^+ U_mu(x-nu)^+ U_nu(x-nu) */
+                lnk(L, links, x, y, z, mu, a);
+                lnk(L, links, x + d[mu][0] - d[nu][0], y + d[mu][1] - d[nu][1], z + d[mu][2] - d[nu][2], nu, b);
+                lnk(L, links, x - d[nu][0], y - d[nu][1], z - d[nu][2], mu, c);
+                lnk(L, links, x - d[nu][0], y - d[nu][1], z - d[nu][2], nu, e);
+                m_dag(nc, b, b);
+                m_dag(nc, c, c);
+                m_mul(nc, a, b, p);
+                m_mul(nc, p, c, p);
+                m_mul(nc, p, e, p);
+                retr_itu(nc, p, t);
+                for (int k = 0; k < ng; k++) f[ng * (mu + 3 * s) + k] += t[k];
+            }
+    }
+    for (int i = 0; i < 3 * V * ng; i++) efield[i] += -2.0 * dt * f[i];
+    free(f);
 }
 
 /* Energy of the Kogut-Susskind Hamiltonian in the normalisation of BulkObservables.cpp:29-31,84-86:

@@ -41,6 +41,7 @@ void orc_links_to_ucc(const orc_lat *L, const double *links, const double *phase
 void orc_su2_iexp(double c, const double *alpha, double *out4);
 void orc_su3_iexp(double c, const double *alpha, double *out18);
 void orc_update_links(const orc_lat *L, double *links, const double *efield, double dt);
+void orc_efield_update(const orc_lat *L, const double *links, double *efield, double dt);
 
 /* ---- operators (one mode, AoS) ---- */
 void orc_wilson_evo(const orc_lat *L, const double *ucc, const double *phi, double *psi);
@@ -68,13 +69,13 @@ enum { ORC_WILSON = 0, ORC_OVERLAP_CHEB = 1, ORC_OVERLAP_ZOLO = 2 };
 typedef struct {
     int kind;          /* ORC_WILSON / ORC_OVERLAP_CHEB / ORC_OVERLAP_ZOLO */
     double dt;         /* full step */
-    int update_links;  /* 1: U <- exp(-i dt/2 E) U after each half step (OverlapUpdate.cpp:111-127) */
+    int update_links;  /* 1: U <- exp(-i dt/2 E) U after each half step (OverlapUpdate.cpp:111-127); 2: E too */
     int nb; double q;  /* U(1) flux quanta and charge folded into the stencil links */
     int cheb_n; double cheb_cut;
     double zolo_lmin, zolo_lmax;
     int threads;       /* OpenMP threads over modes (0 = default) */
 } orc_step_opts;
-void orc_leapfrog(const orc_lat *L, double *links, const double *efield, int nmodes,
+void orc_leapfrog(const orc_lat *L, double *links, double *efield, int nmodes,
                   double *psi, double *psimid, const orc_step_opts *o, int nsteps);
 
 /* ---- reductions ---- */

@@ -1,0 +1,1482 @@
+// Host side of the C ABI (include/ovl.h): device state, the sign-function drivers and the
+// leapfrog.  Scalar recurrences (Clenshaw coefficients, multi-shift CG and Ritz scalars) run on
+// the host in FP64; every vector operation runs in the gfx950 kernels of ovl_kernels.hip.
+#include "../../include/ovl.h"
+#include "ovl_internal.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using ovl::Geom;
+using ovl::HopArgs;
+using ovl::P32;
+using ovl::P64;
+typedef std::complex<double> cplx;
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+struct ovl_ctx {
+    ovl_params prm{};
+    Geom g{};
+    int ncomp = 0;
+    size_t field64_bytes = 0, field32_bytes = 0;  // full field (all blocks)
+    size_t blk64_elems = 0, blk32_elems = 0;      // per mode block (double2 / float4 units)
+    hipStream_t st = nullptr;
+    std::string err;
+
+    double *links = nullptr, *efield = nullptr;  // master FP64, reference layout
+    double *ul64 = nullptr;                      // stencil links FP64 (double2)
+    float *ul32 = nullptr;                       // stencil links FP32 (float2)
+    double *phase = nullptr;                     // U(1) phases double2[3V] or null
+    int nb = 0;
+    double q = 1.0;
+    bool links_set = false, efield_set = false, stencil_dirty = true;
+
+    void *field[3] = {nullptr, nullptr, nullptr};  // Psi, PsiMid, scratch (FP64)
+
+    // Clenshaw work buffers (sized for `work_blocks` mode blocks)
+    int work_blocks = 0;
+    void *w64 = nullptr, *a64 = nullptr, *b64 = nullptr;
+    void *x32 = nullptr, *w32 = nullptr, *a32 = nullptr, *b32 = nullptr;
+
+    // reductions / small scratch
+    double *partial = nullptr;
+    size_t partial_len = 0;
+    double *dout = nullptr;  // 16 doubles
+    double *gam = nullptr;   // current coupling matrices
+    double *jv = nullptr, *ja = nullptr;
+    double *staging = nullptr;
+    size_t staging_bytes = 0;
+
+    // multi-shift CG
+    int ms_no = 0;
+    void *ms_ps = nullptr, *ms_r = nullptr, *ms_q = nullptr, *ms_w = nullptr, *ms_chi = nullptr;
+    double *ms_coef = nullptr;
+    int *ms_active = nullptr;
+
+    // Ritz vectors (one block)
+    void *rz[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+
+    // timing
+    bool profiling = false;
+    std::vector<hipEvent_t> evpool;
+    std::vector<int> evvar;  // stencil variant of each recorded event pair
+    size_t evused = 0;
+    long long var_launches[3] = {0, 0, 0};
+    double var_bytes[3] = {0, 0, 0};
+    long long hop_launches = 0;
+    double hop_bytes = 0;
+    int fp64_steps = 0, fp32_steps = 0;
+    ovl_timing last{};
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    bool capturing = false;
+};
+
+namespace {
+
+#define HIPCHK(ctx, call)                                                                                  \
+    do {                                                                                                   \
+        hipError_t e_ = (call);                                                                            \
+        if (e_ != hipSuccess) {                                                                            \
+            (ctx)->err = std::string(#call) + ": " + hipGetErrorString(e_);                                \
+            return OVL_E_HIP;                                                                              \
+        }                                                                                                  \
+    } while (0)
+
+#define RETCHK(expr)              \
+    do {                          \
+        int r_ = (expr);          \
+        if (r_ != OVL_OK) return r_; \
+    } while (0)
+
+int fail(ovl_ctx *c, int code, const std::string &msg)
+{
+    if (c) c->err = msg;
+    return code;
+}
+
+int dalloc(ovl_ctx *c, void **p, size_t bytes)
+{
+    if (*p) return OVL_OK;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(c, OVL_E_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
+    }
+    return OVL_OK;
+}
+
+void dfree(void *&p)
+{
+    if (p) hipFree(p);
+    p = nullptr;
+}
+
+double lambda_bound(const ovl_ctx *c)
+{
+    // ||h_W|| <= |A|/2 + 3: every direction's hopping block squares to 1 for unitary links
+    return std::fabs(3.0 * c->prm.r_wilson + c->prm.m_fermion - c->prm.m_dw) + 3.0;
+}
+
+double amass(const ovl_ctx *c) { return 6.0 * c->prm.r_wilson + 2.0 * c->prm.m_fermion - 2.0 * c->prm.m_dw; }
+
+int check_field(ovl_ctx *c, int f)
+{
+    if (f < 0 || f > 2) return fail(c, OVL_E_ARG, "field index must be OVL_PSI, OVL_PSIMID or OVL_SCRATCH");
+    return OVL_OK;
+}
+
+int ensure_stencil(ovl_ctx *c)
+{
+    if (!c->links_set) return fail(c, OVL_E_STATE, "no gauge links uploaded (ovl_set_links)");
+    if (c->stencil_dirty) {
+        HIPCHK(c, ovl::launch_link_setup(c->g, c->links, c->phase, c->ul64, c->ul32, c->st));
+        c->stencil_dirty = false;
+    }
+    return OVL_OK;
+}
+
+void *blk_ptr(void *base, int prec, size_t blk, const ovl_ctx *c)
+{
+    if (!base) return nullptr;
+    if (prec == P64) return (char *)base + blk * c->blk64_elems * 16;
+    return (char *)base + blk * c->blk32_elems * 16;
+}
+
+hipEvent_t next_event(ovl_ctx *c)
+{
+    if (c->evused == c->evpool.size()) {
+        hipEvent_t e;
+        hipEventCreate(&e);
+        c->evpool.push_back(e);
+    }
+    return c->evpool[c->evused++];
+}
+
+// algorithmic HBM bytes of one stencil launch: input stencil read once, pointwise operands and
+// output once, links once per launch
+double hop_alg_bytes(const ovl_ctx *c, const HopArgs &a, int pin, int pout)
+{
+    const double sites_modes = (double)a.nblk * c->g.V * c->g.mb;
+    const double Sin = c->ncomp * 2.0 * (pin == P64 ? 8 : 4);
+    const double Sout = c->ncomp * 2.0 * (pout == P64 ? 8 : 4);
+    double b = sites_modes * (Sin + Sout);
+    if (a.x) b += sites_modes * Sout;
+    if (a.b1) b += sites_modes * Sout;
+    if (a.b2) b += sites_modes * Sout;
+    if (a.tgt) b += sites_modes * Sout;
+    b += 3.0 * c->g.V * c->g.nc * c->g.nc * 2.0 * (pin == P64 ? 8 : 4);
+    return b;
+}
+
+int hop(ovl_ctx *c, const Geom &g, int pin, int pout, const HopArgs &a)
+{
+    const int var = (pin == P64) ? 0 : (pout == P32 ? 1 : 2);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (c->profiling && !c->capturing) {
+        e0 = next_event(c);
+        e1 = next_event(c);
+        c->evvar.resize(c->evused / 2);
+        c->evvar[c->evused / 2 - 1] = var;
+        hipEventRecord(e0, c->st);
+    }
+    HIPCHK(c, ovl::launch_hop(g, pin, pout, a, c->st));
+    if (e1) hipEventRecord(e1, c->st);
+    const double b = hop_alg_bytes(c, a, pin, pout);
+    c->hop_launches++;
+    c->hop_bytes += b;
+    c->var_launches[var]++;
+    c->var_bytes[var] += b;
+    return OVL_OK;
+}
+
+HopArgs hop_args(const ovl_ctx *c, int prec)
+{
+    HopArgs a{};
+    a.links = prec == P64 ? (const void *)c->ul64 : (const void *)c->ul32;
+    a.amass = amass(c);
+    a.nblk = c->g.nblk;
+    return a;
+}
+
+void begin_timing(ovl_ctx *c)
+{
+    c->evused = 0;
+    c->hop_launches = 0;
+    c->hop_bytes = 0;
+    for (int v = 0; v < 3; ++v) { c->var_launches[v] = 0; c->var_bytes[v] = 0; }
+    c->fp64_steps = c->fp32_steps = 0;
+    if (!c->capturing) hipEventRecord(c->t0, c->st);
+}
+
+int end_timing(ovl_ctx *c)
+{
+    HIPCHK(c, hipEventRecord(c->t1, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    float ms = 0;
+    hipEventElapsedTime(&ms, c->t0, c->t1);
+    c->last.total_ms = ms;
+    double hop_ms = 0, var_ms[3] = {0, 0, 0};
+    for (size_t i = 0; i + 1 < c->evused; i += 2) {
+        float t = 0;
+        hipEventElapsedTime(&t, c->evpool[i], c->evpool[i + 1]);
+        hop_ms += t;
+        var_ms[c->evvar[i / 2]] += t;
+    }
+    c->last.hop_ms = hop_ms;
+    for (int v = 0; v < 3; ++v) {
+        c->last.var_ms[v] = var_ms[v];
+        c->last.var_launches[v] = c->var_launches[v];
+        c->last.var_bytes[v] = c->var_bytes[v];
+    }
+    c->last.hop_launches = c->hop_launches;
+    c->last.hop_bytes = c->hop_bytes;
+    c->last.fp64_steps = c->fp64_steps;
+    c->last.fp32_steps = c->fp32_steps;
+    return OVL_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Chebyshev approximation of x^{-1/2} on [cut^2, 1] (FP64, host)
+// ---------------------------------------------------------------------------------------------
+std::vector<double> cheb_coeffs(int n, double cut)
+{
+    // c_k = (2/N) sum_j f(x(t_j)) cos(k theta_j) at N Chebyshev nodes, c_0 halved
+    const int N = 8 * (n + 1) + 64;
+    const double d2 = cut * cut;
+    std::vector<long double> acc(n + 1, 0.0L);
+    for (int j = 0; j < N; ++j) {
+        const double th = M_PI * (j + 0.5) / N;
+        const double t = std::cos(th);
+        const double x = 0.5 * ((1.0 - d2) * t + (1.0 + d2));
+        const long double f = 1.0L / std::sqrt((long double)x);
+        for (int k = 0; k <= n; ++k) acc[k] += f * (long double)std::cos(k * th);
+    }
+    std::vector<double> c(n + 1);
+    for (int k = 0; k <= n; ++k) c[k] = (double)(2.0L * acc[k] / N);
+    c[0] *= 0.5;
+    return c;
+}
+
+// FP64 head length of the mixed recurrence: steps k < head run in FP64.  The FP32 rounding
+// injected at step k is bounded by u32 * |b_k| and grows at most by (k+1) (|U_k| <= k+1 on
+// [-1,1]); |b_k| <= B_k = sum_{j>=k} |c_j| (j-k+1).
+int choose_head(const std::vector<double> &c, double tol)
+{
+    const int n = (int)c.size() - 1;
+    const double u32 = 5.96e-8;
+    std::vector<double> B(n + 2, 0.0);
+    for (int k = n; k >= 0; --k) {
+        double s = 0;
+        for (int j = k; j <= n; ++j) s += std::fabs(c[j]) * (j - k + 1);
+        B[k] = s;
+    }
+    double err = 0;
+    int head = n + 1;
+    for (int k = n; k >= 0; --k) {
+        err += u32 * B[k] * (k + 1);
+        if (err > tol) break;
+        head = k;
+    }
+    return head;
+}
+
+int ensure_work(ovl_ctx *c, int blocks, bool need32)
+{
+    if (blocks > c->work_blocks) {
+        dfree(c->w64); dfree(c->a64); dfree(c->b64);
+        dfree(c->x32); dfree(c->w32); dfree(c->a32); dfree(c->b32);
+        c->work_blocks = blocks;
+    }
+    const size_t b64 = (size_t)c->work_blocks * c->blk64_elems * 16;
+    const size_t b32 = (size_t)c->work_blocks * c->blk32_elems * 16;
+    RETCHK(dalloc(c, &c->w64, b64));
+    RETCHK(dalloc(c, &c->a64, b64));
+    RETCHK(dalloc(c, &c->b64, b64));
+    if (need32) {
+        RETCHK(dalloc(c, &c->x32, b32));
+        RETCHK(dalloc(c, &c->w32, b32));
+        RETCHK(dalloc(c, &c->a32, b32));
+        RETCHK(dalloc(c, &c->b32, b32));
+    }
+    return OVL_OK;
+}
+
+struct SignPlan {
+    std::vector<double> coef;
+    int n = 0, head = 0;
+    double lam = 0, alpha = 0, beta = 0;
+};
+
+int make_plan(ovl_ctx *c, const ovl_sign_opts *o, SignPlan &p)
+{
+    if (o->degree < 1 || o->degree > 100000) return fail(c, OVL_E_ARG, "Chebyshev degree must be in [1, 100000]");
+    if (!(o->cut > 0.0 && o->cut < 1.0)) return fail(c, OVL_E_SPECTRUM, "cut must be in (0, 1)");
+    p.n = o->degree;
+    p.coef = cheb_coeffs(p.n, o->cut);
+    p.lam = lambda_bound(c);
+    const double d2 = o->cut * o->cut;
+    p.alpha = 2.0 / (p.lam * p.lam * (1.0 - d2));
+    p.beta = -(1.0 + d2) / (1.0 - d2);
+    if (o->precision == OVL_PREC_FP64) p.head = p.n + 1;
+    else if (o->precision == OVL_PREC_FP32) p.head = 0;
+    else if (o->head >= 0) p.head = std::min(o->head, p.n + 1);
+    else p.head = choose_head(p.coef, o->tol > 0 ? o->tol : 1e-10);
+    return OVL_OK;
+}
+
+// Clenshaw recurrence for one chunk of mode blocks:  p = P(T) x,  T = alpha h_W^2 + beta.
+// Steps k >= head run in FP32, k < head in FP64.  Returns the buffer and precision of p.
+int clenshaw_chunk(ovl_ctx *c, const Geom &gc, const SignPlan &pl, const void *x64, void **pout, int *pprec)
+{
+    const int n = pl.n;
+    const bool any32 = pl.head <= n;
+    RETCHK(ensure_work(c, gc.nblk, any32));
+    if (any32) HIPCHK(c, ovl::launch_convert(gc, gc.nblk, x64, P64, c->x32, P32, c->st));
+    // buffer rotation: b1 = b_{k+1}, b2 = b_{k+2} (null = zero)
+    void *b1 = nullptr, *b2 = nullptr;
+    int prec_b = -1;  // precision of b1/b2
+    for (int k = n; k >= 0; --k) {
+        const int pk = (k < pl.head) ? P64 : P32;
+        if (pk == P64) c->fp64_steps++; else c->fp32_steps++;
+        void *xk = (pk == P64) ? (void *)x64 : c->x32;
+        void *A = (pk == P64) ? c->a64 : c->a32, *B = (pk == P64) ? c->b64 : c->b32;
+        void *W = (pk == P64) ? c->w64 : c->w32;
+        if (b1 && prec_b != pk) {
+            // FP32 -> FP64 transition: promote b_{k+1}, b_{k+2}
+            HIPCHK(c, ovl::launch_convert(gc, gc.nblk, b1, P32, c->a64, P64, c->st));
+            if (b2) HIPCHK(c, ovl::launch_convert(gc, gc.nblk, b2, P32, c->b64, P64, c->st));
+            b1 = c->a64;
+            b2 = b2 ? c->b64 : nullptr;
+            prec_b = pk;
+        }
+        if (!b1) {  // b_n = c_n x
+            HIPCHK(c, ovl::launch_lincomb(gc, gc.nblk, pk, A, xk, pl.coef[k], 0.0, nullptr, 0, 0, c->st));
+            b1 = A;
+            prec_b = pk;
+            continue;
+        }
+        // w = h_W b_{k+1}
+        HopArgs a = hop_args(c, pk);
+        a.nblk = gc.nblk;
+        a.in = b1;
+        a.out = W;
+        a.sh_r = 1.0;
+        RETCHK(hop(c, gc, pk, pk, a));
+        // b_k = c_k x + 2 alpha h_W w + 2 beta b_{k+1} - b_{k+2}    (k >= 1)
+        // p   = c_0 x +   alpha h_W w +   beta b_1     - b_2        (k == 0)
+        const double f = (k == 0) ? 1.0 : 2.0;
+        void *out = b2 ? b2 : ((b1 == A) ? B : A);
+        HopArgs q = hop_args(c, pk);
+        q.nblk = gc.nblk;
+        q.in = W;
+        q.out = out;
+        q.sh_r = f * pl.alpha;
+        q.x = xk;
+        q.sx_r = pl.coef[k];
+        q.b1 = b1;
+        q.s1_r = f * pl.beta;
+        q.b2 = b2;
+        q.s2_r = -1.0;
+        RETCHK(hop(c, gc, pk, pk, q));
+        b2 = b1;
+        b1 = out;
+    }
+    *pout = b1;
+    *pprec = prec_b;
+    return OVL_OK;
+}
+
+// result = sh * h_W(p) + sx * G(x) + st * tgt for every chunk; kind selects the combination
+enum FinalKind { FIN_SIGN = 0, FIN_OVERLAP = 1, FIN_OVERLAP_AXPY = 2 };
+
+int sign_driver(ovl_ctx *c, const ovl_sign_opts *o, const void *in64, void *out64, int kind, double dt)
+{
+    RETCHK(ensure_stencil(c));
+    SignPlan pl;
+    RETCHK(make_plan(c, o, pl));
+    const int chunk = (o->mode_chunk > 0) ? std::min(o->mode_chunk, c->g.nblk) : c->g.nblk;
+    const double M = c->prm.m_dw, m = c->prm.m_fermion;
+    for (int b0 = 0; b0 < c->g.nblk; b0 += chunk) {
+        Geom gc = c->g;
+        gc.nblk = std::min(chunk, c->g.nblk - b0);
+        const void *x = blk_ptr((void *)in64, P64, b0, c);
+        void *y = blk_ptr(out64, P64, b0, c);
+        void *p = nullptr;
+        int pp = P64;
+        RETCHK(clenshaw_chunk(c, gc, pl, x, &p, &pp));
+        HopArgs a = hop_args(c, pp);
+        a.nblk = gc.nblk;
+        a.in = p;
+        a.out = y;
+        if (kind == FIN_SIGN) {
+            a.sh_r = 1.0 / pl.lam;
+        } else {
+            // -i h_ov = -i M [ (1 + m/2) gamma0 + (1 - m/2) eps ]   (OverlapU.cpp:188-201)
+            const double s = (kind == FIN_OVERLAP_AXPY) ? dt : 1.0;
+            a.sh_i = -s * M * (1.0 - 0.5 * m) / pl.lam;
+            a.x = x;
+            a.x_gamma0 = 1;
+            a.sx_i = -s * M * (1.0 + 0.5 * m);
+            if (kind == FIN_OVERLAP_AXPY) {
+                a.tgt = y;
+                a.st_r = 1.0;
+            }
+        }
+        RETCHK(hop(c, gc, pp, P64, a));
+    }
+    return OVL_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Zolotarev rational approximation and multi-shift CG (reference algorithm, FP64)
+// ---------------------------------------------------------------------------------------------
+
+// CompElliptic (ZolotarevCoefficients.cpp:5-49): complete elliptic integral by Carlson's R_F
+double elliptic_k(double kappa, double tol)
+{
+    double X = 0.0, Y = 1.0 - kappa, Z = 1.0, mean = (X + Y + Z) / 3.0;
+    double dx = 2.0 - (mean + X) / mean, dy = 2.0 - (mean + Y) / mean, dz = 2.0 - (mean + Z) / mean;
+    for (double dev = tol; dev >= tol;) {
+        mean = (X + Y + Z) / 3.0;
+        dx = 2.0 - (mean + X) / mean;
+        dy = 2.0 - (mean + Y) / mean;
+        dz = 2.0 - (mean + Z) / mean;
+        dev = std::max(std::fabs(dx), std::max(std::fabs(dy), std::fabs(dz)));
+        const double sx = std::sqrt(X), sy = std::sqrt(Y), sz = std::sqrt(Z);
+        const double lam = sx * (sy + sz) + sy * sz;
+        X = 0.25 * (X + lam);
+        Y = 0.25 * (Y + lam);
+        Z = 0.25 * (Z + lam);
+    }
+    const double e2 = dx * dy - dz * dz, e3 = dx * dy * dz;
+    return (1.0 + (e2 / 24.0 - 0.1 - e3 / 44.0) * e2 + e3 / 14.0) / std::sqrt(mean);
+}
+
+}  // namespace
+
+extern "C" double ovl_zolotarev_coefficients(double amin, double amax, double errtol, int no, double *bz, double *dz)
+{
+    // ZolotarevCoefficients.cpp:71-175.  The single-precision square roots of the reference
+    // (:97, :106, :148) and its fmin start value 1.0 (:136) are kept so that the coefficients
+    // are the reference's bit for bit.
+    if (no < 1 || !(amin > 0) || !(amax > amin) || !bz || !dz) return NAN;
+    const double kappa = std::sqrt(1.0 - amin / amax);
+    const double K = elliptic_k(kappa, errtol);
+    const double step = 0.5 * K / no;
+    std::vector<double> c(2 * no), al(4000), ga(4000);
+    for (int l = 0; l < 2 * no - 1; ++l) {
+        const double u = (l + 1) * step;
+        // descending Landen / AGM for the Jacobi amplitude, then tan^2
+        al[0] = 1.0;
+        double b = (double)sqrtf((float)(1.0 - kappa));
+        ga[0] = std::sqrt(kappa);
+        int i = 1;
+        double p2 = 1.0;
+        while (std::fabs(ga[i - 1] / al[i - 1]) >= errtol) {
+            const double bprev = b;
+            al[i] = 0.5 * (al[i - 1] + bprev);
+            b = (double)sqrtf((float)(al[i - 1] * bprev));
+            ga[i] = 0.5 * (al[i - 1] - bprev);
+            ++i;
+            p2 *= 2.0;
+        }
+        --i;
+        double phi = p2 * al[i] * u;
+        for (; i >= 1; --i) phi = 0.5 * (phi + std::asin(ga[i] * std::sin(phi) / al[i]));
+        c[l] = std::tan(phi) * std::tan(phi);
+    }
+    for (int l = 0; l < no; ++l) {
+        dz[l] = c[2 * l] * amin;
+        // coeffb (ZolotarevCoefficients.cpp:51-69)
+        double prod = 1.0;
+        int j = 0;
+        for (int i = 0; i < no - 1; ++i) {
+            prod *= (c[2 * i + 1] - c[2 * l]);
+            if (i == l) ++j;
+            prod /= (c[2 * j] - c[2 * l]);
+            ++j;
+        }
+        bz[l] = prod;
+    }
+    const int np = 5000;
+    const double dxl = (std::log(amax) - std::log(amin)) / np;
+    double y = std::log(amin) - dxl, fmin = std::pow(1.0, 10.0), fmax = 0.0;
+    for (int i = 0; i < np; ++i) {
+        y += dxl;
+        const double x = std::exp(y);
+        double z = 0.0;
+        for (int l = 0; l < no; ++l) z += bz[l] / (x + dz[l]);
+        z *= (double)sqrtf((float)x);
+        fmax = std::max(fmax, z);
+        fmin = std::min(fmin, z);
+    }
+    const double anorm = 2.0 / (fmin + fmax);
+    for (int k = 0; k < no; ++k) bz[k] *= anorm;
+    return (fmax - fmin) / (fmax + fmin);
+}
+
+namespace {
+
+int mode_dot(ovl_ctx *c, const Geom &g, const void *a, const void *b, std::vector<cplx> &out)
+{
+    HIPCHK(c, ovl::launch_mode_dot(g, a, b, c->partial, c->dout + 16, c->st));
+    const int nm = g.nblk * g.mb;
+    out.resize(nm);
+    HIPCHK(c, hipMemcpyAsync(out.data(), c->dout + 16, sizeof(cplx) * nm, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return OVL_OK;
+}
+
+// (h_W^2 + d0) v  ->  q   via two stencil launches, FP64
+int apply_h2(ovl_ctx *c, const Geom &g, const void *v, void *w, void *q, double d0)
+{
+    HopArgs a = hop_args(c, P64);
+    a.nblk = g.nblk;
+    a.in = v;
+    a.out = w;
+    a.sh_r = 1.0;
+    RETCHK(hop(c, g, P64, P64, a));
+    HopArgs b = hop_args(c, P64);
+    b.nblk = g.nblk;
+    b.in = w;
+    b.out = q;
+    b.sh_r = 1.0;
+    if (d0 != 0.0) {
+        b.x = v;
+        b.sx_r = d0;
+    }
+    return hop(c, g, P64, P64, b);
+}
+
+// Zolotrr (OverlapU.cpp:5-149) for all modes of `in`: chi = sum_k bz_k (h^2 + dz_k)^{-1} in.
+// Per-mode scalars live on the host; the chi accumulation is folded into the shift update.
+int mscg(ovl_ctx *c, const Geom &g, const void *in, void *chi, const double *bz, const double *dz, int no,
+         double cgerr, int nitcgm, int *max_iter, int *nonconv)
+{
+    const size_t fb = (size_t)g.nblk * c->blk64_elems * 16;      // stride of one shift vector for g
+    const size_t fmax = (size_t)c->g.nblk * c->blk64_elems * 16;  // allocation: the whole context
+    if (c->ms_no < no) {
+        dfree(c->ms_ps);
+        c->ms_no = no;
+    }
+    RETCHK(dalloc(c, &c->ms_ps, fmax * no));
+    RETCHK(dalloc(c, &c->ms_r, fmax));
+    RETCHK(dalloc(c, &c->ms_q, fmax));
+    RETCHK(dalloc(c, &c->ms_w, fmax));
+    const int nm = g.nblk * g.mb;
+    RETCHK(dalloc(c, (void **)&c->ms_coef, sizeof(double) * (size_t)c->g.nblk * c->g.mb * 3 * 32));
+    RETCHK(dalloc(c, (void **)&c->ms_active, sizeof(int) * (size_t)c->g.nblk * c->g.mb));
+    HIPCHK(c, ovl::launch_lincomb(g, g.nblk, P64, c->ms_r, in, 1, 0, nullptr, 0, 0, c->st));
+    for (int k = 0; k < no; ++k)
+        HIPCHK(c, ovl::launch_lincomb(g, g.nblk, P64, (char *)c->ms_ps + k * fb, in, 1, 0, nullptr, 0, 0, c->st));
+    HIPCHK(c, ovl::launch_lincomb(g, g.nblk, P64, chi, in, 0, 0, nullptr, 0, 0, c->st));
+    std::vector<cplx> d;
+    RETCHK(mode_dot(c, g, c->ms_r, c->ms_r, d));
+    std::vector<double> rn(nm), bn(nm), betam(nm, 1.0), a0(nm, 0.0);
+    std::vector<double> z((size_t)nm * no, 1.0), gc((size_t)nm * no, 1.0), be((size_t)nm * no, 0.0), al((size_t)nm * no, 0.0);
+    std::vector<int> act(nm), it(nm, 0);
+    std::vector<double> coefA(nm), coef((size_t)nm * 3 * no, 0.0);
+    for (int i = 0; i < nm; ++i) {
+        rn[i] = d[i].real();
+        bn[i] = std::sqrt(rn[i]);
+        act[i] = rn[i] > cgerr * bn[i];
+    }
+    *max_iter = 0;
+    *nonconv = 0;
+    while (std::any_of(act.begin(), act.end(), [](int v) { return v != 0; })) {
+        HIPCHK(c, hipMemcpyAsync(c->ms_active, act.data(), sizeof(int) * nm, hipMemcpyHostToDevice, c->st));
+        RETCHK(apply_h2(c, g, c->ms_ps, c->ms_w, c->ms_q, dz[0]));
+        std::vector<cplx> pq;
+        RETCHK(mode_dot(c, g, c->ms_ps, c->ms_q, pq));
+        for (int i = 0; i < nm; ++i) coefA[i] = act[i] ? rn[i] / pq[i].real() : 0.0;
+        HIPCHK(c, hipMemcpyAsync(c->ms_coef, coefA.data(), sizeof(double) * nm, hipMemcpyHostToDevice, c->st));
+        HIPCHK(c, ovl::launch_axpy_modes(g, c->ms_r, c->ms_q, c->ms_coef, 1, -1.0, c->ms_active, c->st));
+        std::vector<cplx> rr;
+        RETCHK(mode_dot(c, g, c->ms_r, c->ms_r, rr));
+        std::vector<int> nact(nm, 0);
+        for (int i = 0; i < nm; ++i) {
+            if (!act[i]) continue;
+            double *zi = &z[(size_t)i * no], *bi = &be[(size_t)i * no], *ai = &al[(size_t)i * no], *gi = &gc[(size_t)i * no];
+            bi[0] = coefA[i];
+            for (int k = 1; k < no; ++k) {
+                const double dr = bi[0] * a0[i] * (1.0 - zi[k]) + betam[i] * (1.0 + (dz[k] - dz[0]) * bi[0]);
+                zi[k] = betam[i] / dr;
+                bi[k] = bi[0] * zi[k];
+            }
+            a0[i] = rr[i].real() / rn[i];
+            rn[i] = rr[i].real();
+            for (int k = 0; k < no; ++k) {
+                ai[k] = a0[i] * zi[k] * bi[k] / bi[0];
+                gi[k] = gi[k] * zi[k];
+                coef[((size_t)i * no + k) * 3 + 0] = bz[k] * bi[k];
+                coef[((size_t)i * no + k) * 3 + 1] = ai[k];
+                coef[((size_t)i * no + k) * 3 + 2] = gi[k];
+            }
+            betam[i] = bi[0];
+            it[i]++;
+            nact[i] = (rn[i] > cgerr * bn[i]) && it[i] < nitcgm;
+            if (it[i] >= nitcgm && rn[i] > cgerr * bn[i]) (*nonconv)++;
+            *max_iter = std::max(*max_iter, it[i]);
+        }
+        HIPCHK(c, hipMemcpyAsync(c->ms_coef, coef.data(), sizeof(double) * coef.size(), hipMemcpyHostToDevice, c->st));
+        HIPCHK(c, ovl::launch_mscg_update(g, no, (double *)c->ms_ps, (double *)chi, (const double *)c->ms_r, c->ms_coef,
+                                          c->ms_active, c->st));
+        HIPCHK(c, hipStreamSynchronize(c->st));
+        act = nact;
+    }
+    return OVL_OK;
+}
+
+int ritz_bounds(ovl_ctx *c, double *lmin, double *lmax);
+
+// -i h_ov in -> out (or out += dt (-i h_ov in)) with the Zolotarev sign, all modes of g
+int zolo_overlap(ovl_ctx *c, const Geom &g, const void *in, void *out, double lmin, double lmax, int axpy, double dt)
+{
+    if (lmin == 0.0 && lmax == 0.0) RETCHK(ritz_bounds(c, &lmin, &lmax));
+    if (!(lmin > 0.0) || !(lmax > lmin) || !std::isfinite(lmax))
+        return fail(c, OVL_E_SPECTRUM, "Zolotarev interval needs 0 < lmin < lmax");
+    RETCHK(ensure_stencil(c));
+    const int no = 25;
+    double bz[25], dz[25];
+    ovl_zolotarev_coefficients(lmin, lmax, 1e-12, no, bz, dz);
+    const size_t fb = (size_t)g.nblk * c->blk64_elems * 16;
+    RETCHK(dalloc(c, &c->ms_chi, (size_t)c->g.nblk * c->blk64_elems * 16));
+    int mx = 0, nc = 0;
+    RETCHK(mscg(c, g, in, c->ms_chi, bz, dz, no, 1e-16, 1000, &mx, &nc));
+    (void)fb;
+    // psi = -i h_W chi, combined as OverlapU.cpp:188-201
+    const double M = c->prm.m_dw, m = c->prm.m_fermion, s = axpy ? dt : 1.0;
+    HopArgs a = hop_args(c, P64);
+    a.nblk = g.nblk;
+    a.in = c->ms_chi;
+    a.out = out;
+    a.sh_i = -s * M * (1.0 - 0.5 * m);
+    a.x = in;
+    a.x_gamma0 = 1;
+    a.sx_i = -s * M * (1.0 + 0.5 * m);
+    if (axpy) {
+        a.tgt = out;
+        a.st_r = 1.0;
+    }
+    RETCHK(hop(c, g, P64, P64, a));
+    if (nc) return fail(c, OVL_E_CONVERGE, "multi-shift CG reached 1000 iterations for " + std::to_string(nc) + " mode(s)");
+    return OVL_OK;
+}
+
+}  // namespace
+
+// =============================================================================================
+// C ABI
+// =============================================================================================
+extern "C" {
+
+void ovl_sign_defaults(ovl_sign_opts *o)
+{
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->method = OVL_SIGN_CHEBYSHEV;
+    o->precision = OVL_PREC_FP64;
+    o->degree = 256;
+    o->cut = 0.05;
+    o->tol = 1e-10;
+    o->head = -1;
+}
+
+int ovl_device_name(char *buf, int len)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n < 1) return OVL_E_HIP;
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, 0) != hipSuccess) return OVL_E_HIP;
+    if (buf && len > 0) std::snprintf(buf, len, "%s (%s, %d CUs)", p.name, p.gcnArchName, p.multiProcessorCount);
+    return OVL_OK;
+}
+
+int ovl_create(const ovl_params *p, ovl_ctx **out)
+{
+    if (!p || !out) return OVL_E_ARG;
+    *out = nullptr;
+    if (p->nc != 2 && p->nc != 3) return OVL_E_ARG;
+    if (p->n[0] < 3 || p->n[1] < 3 || p->n[2] < 3) return OVL_E_ARG;
+    if (p->n_modes < 1) return OVL_E_ARG;
+    if (p->mode_block != 0 && p->mode_block != 16 && p->mode_block != 32 && p->mode_block != 64) return OVL_E_ARG;
+    const long V = (long)p->n[0] * p->n[1] * p->n[2];
+    if (V > (1L << 26)) return OVL_E_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return OVL_E_HIP;
+    ovl_ctx *c = new ovl_ctx();
+    c->prm = *p;
+    int mb = p->mode_block;
+    if (mb == 0) mb = p->n_modes >= 64 ? 64 : (p->n_modes > 16 ? 32 : 16);
+    c->g.nc = p->nc;
+    c->g.n0 = p->n[0];
+    c->g.n1 = p->n[1];
+    c->g.n2 = p->n[2];
+    c->g.V = (int)V;
+    c->g.mb = mb;
+    c->g.nblk = (p->n_modes + mb - 1) / mb;
+    c->ncomp = 4 * p->nc;
+    c->blk64_elems = (size_t)V * c->ncomp * mb;
+    c->blk32_elems = (size_t)V * (c->ncomp / 2) * mb;
+    c->field64_bytes = c->blk64_elems * 16 * c->g.nblk;
+    c->field32_bytes = c->blk32_elems * 16 * c->g.nblk;
+    int rc = OVL_OK;
+    auto bail = [&](int r) {
+        ovl_destroy(c);
+        return r;
+    };
+    if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) return bail(OVL_E_HIP);
+    hipEventCreate(&c->t0);
+    hipEventCreate(&c->t1);
+    const int ng = p->nc * p->nc - 1;
+    const size_t lsz = (p->nc == 2 ? 4 : 18) * 3 * V * sizeof(double);
+    if ((rc = dalloc(c, (void **)&c->links, lsz)) || (rc = dalloc(c, (void **)&c->efield, sizeof(double) * ng * 3 * V)) ||
+        (rc = dalloc(c, (void **)&c->ul64, 16 * 3 * V * p->nc * p->nc)) ||
+        (rc = dalloc(c, (void **)&c->ul32, 8 * 3 * V * p->nc * p->nc)))
+        return bail(rc);
+    for (int f = 0; f < 3; ++f)
+        if ((rc = dalloc(c, &c->field[f], c->field64_bytes))) return bail(rc);
+    for (int f = 0; f < 3; ++f) hipMemsetAsync(c->field[f], 0, c->field64_bytes, c->st);
+    hipMemsetAsync(c->efield, 0, sizeof(double) * ng * 3 * V, c->st);
+    c->partial_len = std::max(std::max(ovl::mode_dot_partial_len(c->g), ovl::reduce_partial_len(c->g)),
+                              ovl::currents_partial_len(c->g));
+    if ((rc = dalloc(c, (void **)&c->partial, sizeof(double) * c->partial_len)) ||
+        (rc = dalloc(c, (void **)&c->dout, sizeof(double) * (16 + 2 * (size_t)c->g.nblk * mb))))
+        return bail(rc);
+    if (hipStreamSynchronize(c->st) != hipSuccess) return bail(OVL_E_HIP);
+    *out = c;
+    return OVL_OK;
+}
+
+int ovl_destroy(ovl_ctx *c)
+{
+    if (!c) return OVL_OK;
+    if (c->st) hipStreamSynchronize(c->st);
+    void *ps[] = {c->links, c->efield, c->ul64, c->ul32, c->phase, c->field[0], c->field[1], c->field[2], c->w64, c->a64,
+                  c->b64, c->x32, c->w32, c->a32, c->b32, c->partial, c->dout, c->gam, c->jv, c->ja, c->staging,
+                  c->ms_ps, c->ms_r, c->ms_q, c->ms_w, c->ms_chi, c->ms_coef, c->ms_active, c->rz[0], c->rz[1],
+                  c->rz[2], c->rz[3], c->rz[4], c->rz[5]};
+    for (void *p : ps)
+        if (p) hipFree(p);
+    for (auto e : c->evpool) hipEventDestroy(e);
+    if (c->t0) hipEventDestroy(c->t0);
+    if (c->t1) hipEventDestroy(c->t1);
+    if (c->st) hipStreamDestroy(c->st);
+    delete c;
+    return OVL_OK;
+}
+
+const char *ovl_error(const ovl_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+static int stage(ovl_ctx *c, size_t bytes)
+{
+    if (c->staging_bytes >= bytes) return OVL_OK;
+    if (c->staging) hipFree(c->staging);
+    c->staging = nullptr;
+    c->staging_bytes = 0;
+    RETCHK(dalloc(c, (void **)&c->staging, bytes));
+    c->staging_bytes = bytes;
+    return OVL_OK;
+}
+
+static bool all_finite(const double *p, size_t n)
+{
+    for (size_t i = 0; i < n; ++i)
+        if (!std::isfinite(p[i])) return false;
+    return true;
+}
+
+int ovl_set_links(ovl_ctx *c, const double *links)
+{
+    if (!c || !links) return OVL_E_ARG;
+    const size_t n = (size_t)(c->g.nc == 2 ? 4 : 18) * 3 * c->g.V;
+    if (!all_finite(links, n)) return fail(c, OVL_E_ARG, "non-finite gauge link");
+    HIPCHK(c, hipMemcpyAsync(c->links, links, n * sizeof(double), hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    c->links_set = true;
+    c->stencil_dirty = true;
+    return OVL_OK;
+}
+
+int ovl_get_links(ovl_ctx *c, double *links)
+{
+    if (!c || !links) return OVL_E_ARG;
+    const size_t n = (size_t)(c->g.nc == 2 ? 4 : 18) * 3 * c->g.V;
+    HIPCHK(c, hipMemcpyAsync(links, c->links, n * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return OVL_OK;
+}
+
+int ovl_set_efield(ovl_ctx *c, const double *e)
+{
+    if (!c || !e) return OVL_E_ARG;
+    const size_t n = (size_t)(c->g.nc * c->g.nc - 1) * 3 * c->g.V;
+    if (!all_finite(e, n)) return fail(c, OVL_E_ARG, "non-finite electric field");
+    HIPCHK(c, hipMemcpyAsync(c->efield, e, n * sizeof(double), hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    c->efield_set = true;
+    return OVL_OK;
+}
+
+int ovl_get_efield(ovl_ctx *c, double *e)
+{
+    if (!c || !e) return OVL_E_ARG;
+    const size_t n = (size_t)(c->g.nc * c->g.nc - 1) * 3 * c->g.V;
+    HIPCHK(c, hipMemcpyAsync(e, c->efield, n * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return OVL_OK;
+}
+
+int ovl_set_u1_flux(ovl_ctx *c, int nb, double q)
+{
+    if (!c || !std::isfinite(q)) return OVL_E_ARG;
+    c->nb = nb;
+    c->q = q;
+    if (nb == 0) {
+        if (c->phase) hipFree(c->phase);
+        c->phase = nullptr;
+        c->stencil_dirty = true;
+        return OVL_OK;
+    }
+    const int n0 = c->g.n0, n1 = c->g.n1, V = c->g.V;
+    std::vector<double> ph(6 * (size_t)V);
+    for (int s = 0; s < V; ++s) {
+        const int z = s / (n0 * n1), y = (s - z * n0 * n1) / n0, x = s - n0 * (y + n1 * z);
+        (void)z;
+        // QED.cpp:136-143: U_y = exp(-2 pi i Nb x/(Nx Ny)), U_x(Nx-1,y) = exp(2 pi i Nb y Nx/(Nx Ny))
+        const double ay = -2.0 * M_PI * (double)(nb * x) / (double)(n0 * n1);
+        const double ax = (x == n0 - 1) ? 2.0 * M_PI * (double)(nb * y * n0) / (double)(n0 * n1) : 0.0;
+        const cplx px = std::exp(cplx(0, q * ax)), py = std::exp(cplx(0, q * ay));
+        ph[2 * (0 * (size_t)V + s)] = px.real();
+        ph[2 * (0 * (size_t)V + s) + 1] = px.imag();
+        ph[2 * (1 * (size_t)V + s)] = py.real();
+        ph[2 * (1 * (size_t)V + s) + 1] = py.imag();
+        ph[2 * (2 * (size_t)V + s)] = 1.0;
+        ph[2 * (2 * (size_t)V + s) + 1] = 0.0;
+    }
+    RETCHK(dalloc(c, (void **)&c->phase, sizeof(double) * ph.size()));
+    HIPCHK(c, hipMemcpyAsync(c->phase, ph.data(), sizeof(double) * ph.size(), hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    c->stencil_dirty = true;
+    return OVL_OK;
+}
+
+int ovl_set_modes(ovl_ctx *c, int field, const double *aos, int first, int count)
+{
+    if (!c || !aos) return OVL_E_ARG;
+    RETCHK(check_field(c, field));
+    if (first < 0 || count < 1 || first + count > c->prm.n_modes) return fail(c, OVL_E_ARG, "mode range out of bounds");
+    const size_t n = (size_t)count * c->g.V * c->ncomp * 2;
+    if (!all_finite(aos, n)) return fail(c, OVL_E_ARG, "non-finite spinor component");
+    RETCHK(stage(c, n * sizeof(double)));
+    HIPCHK(c, hipMemcpyAsync(c->staging, aos, n * sizeof(double), hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, ovl::launch_aos_to_soa(c->g, c->staging, first, count, c->field[field], c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return OVL_OK;
+}
+
+int ovl_get_modes(ovl_ctx *c, int field, double *aos, int first, int count)
+{
+    if (!c || !aos) return OVL_E_ARG;
+    RETCHK(check_field(c, field));
+    if (first < 0 || count < 1 || first + count > c->prm.n_modes) return fail(c, OVL_E_ARG, "mode range out of bounds");
+    const size_t n = (size_t)count * c->g.V * c->ncomp * 2;
+    RETCHK(stage(c, n * sizeof(double)));
+    HIPCHK(c, ovl::launch_soa_to_aos(c->g, c->field[field], first, count, c->staging, c->st));
+    HIPCHK(c, hipMemcpyAsync(aos, c->staging, n * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return OVL_OK;
+}
+
+int ovl_copy_field(ovl_ctx *c, int src, int dst)
+{
+    if (!c) return OVL_E_ARG;
+    RETCHK(check_field(c, src));
+    RETCHK(check_field(c, dst));
+    if (src == dst) return OVL_OK;
+    HIPCHK(c, ovl::launch_lincomb(c->g, c->g.nblk, P64, c->field[dst], c->field[src], 1, 0, nullptr, 0, 0, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return OVL_OK;
+}
+
+int ovl_apply_wilson(ovl_ctx *c, int in, int out)
+{
+    if (!c) return OVL_E_ARG;
+    RETCHK(check_field(c, in));
+    RETCHK(check_field(c, out));
+    if (in == out) return fail(c, OVL_E_ARG, "in and out must be different fields");
+    RETCHK(ensure_stencil(c));
+    begin_timing(c);
+    HopArgs a = hop_args(c, P64);
+    a.in = c->field[in];
+    a.out = c->field[out];
+    a.sh_i = -1.0;  // -i h_W
+    RETCHK(hop(c, c->g, P64, P64, a));
+    return end_timing(c);
+}
+
+static int check_sign_opts(ovl_ctx *c, const ovl_sign_opts *o)
+{
+    if (!o) return fail(c, OVL_E_ARG, "null sign options");
+    if (o->method != OVL_SIGN_CHEBYSHEV && o->method != OVL_SIGN_ZOLOTAREV) return fail(c, OVL_E_ARG, "unknown sign method");
+    if (o->precision < OVL_PREC_FP64 || o->precision > OVL_PREC_MIXED) return fail(c, OVL_E_ARG, "unknown precision");
+    if (o->method == OVL_SIGN_ZOLOTAREV && o->precision != OVL_PREC_FP64)
+        return fail(c, OVL_E_ARG, "the Zolotarev sign runs in FP64 only");
+    return OVL_OK;
+}
+
+int ovl_apply_overlap(ovl_ctx *c, int in, int out, const ovl_sign_opts *o)
+{
+    if (!c) return OVL_E_ARG;
+    RETCHK(check_field(c, in));
+    RETCHK(check_field(c, out));
+    RETCHK(check_sign_opts(c, o));
+    if (in == out) return fail(c, OVL_E_ARG, "in and out must be different fields");
+    begin_timing(c);
+    if (o->method == OVL_SIGN_ZOLOTAREV) {
+        int r = zolo_overlap(c, c->g, c->field[in], c->field[out], o->lmin, o->lmax, 0, 0.0);
+        end_timing(c);
+        return r;
+    }
+    RETCHK(sign_driver(c, o, c->field[in], c->field[out], FIN_OVERLAP, 0.0));
+    return end_timing(c);
+}
+
+int ovl_apply_sign(ovl_ctx *c, int in, int out, const ovl_sign_opts *o)
+{
+    if (!c) return OVL_E_ARG;
+    RETCHK(check_field(c, in));
+    RETCHK(check_field(c, out));
+    RETCHK(check_sign_opts(c, o));
+    if (o->method != OVL_SIGN_CHEBYSHEV) return fail(c, OVL_E_ARG, "ovl_apply_sign supports the Chebyshev sign");
+    if (in == out) return fail(c, OVL_E_ARG, "in and out must be different fields");
+    begin_timing(c);
+    RETCHK(sign_driver(c, o, c->field[in], c->field[out], FIN_SIGN, 0.0));
+    return end_timing(c);
+}
+
+int ovl_mode_norms(ovl_ctx *c, int field, double *norm2)
+{
+    if (!c || !norm2) return OVL_E_ARG;
+    RETCHK(check_field(c, field));
+    std::vector<cplx> d;
+    RETCHK(mode_dot(c, c->g, c->field[field], c->field[field], d));
+    for (int i = 0; i < c->prm.n_modes; ++i) norm2[i] = d[i].real();
+    return OVL_OK;
+}
+
+int ovl_sign_residual(ovl_ctx *c, int in, const ovl_sign_opts *o, double *rel)
+{
+    // ||eps(eps x) - x|| / ||x||  (OverlapSignTest.cpp:155-165), max over modes
+    if (!c || !rel) return OVL_E_ARG;
+    RETCHK(check_field(c, in));
+    RETCHK(check_sign_opts(c, o));
+    const int f1 = (in == OVL_SCRATCH) ? OVL_PSI : OVL_SCRATCH;
+    (void)f1;
+    void *t1 = nullptr, *t2 = nullptr;
+    RETCHK(dalloc(c, &t1, c->field64_bytes));
+    if (dalloc(c, &t2, c->field64_bytes) != OVL_OK) {
+        hipFree(t1);
+        return OVL_E_NOMEM;
+    }
+    int r = sign_driver(c, o, c->field[in], t1, FIN_SIGN, 0.0);
+    if (r == OVL_OK) r = sign_driver(c, o, t1, t2, FIN_SIGN, 0.0);
+    std::vector<cplx> dx, dd;
+    if (r == OVL_OK) {
+        ovl::launch_lincomb(c->g, c->g.nblk, P64, t2, t2, 1, 0, c->field[in], -1, 0, c->st);
+        r = mode_dot(c, c->g, t2, t2, dd);
+        if (r == OVL_OK) r = mode_dot(c, c->g, c->field[in], c->field[in], dx);
+    }
+    hipStreamSynchronize(c->st);
+    hipFree(t1);
+    hipFree(t2);
+    if (r != OVL_OK) return r;
+    double worst = 0;
+    for (int i = 0; i < c->prm.n_modes; ++i)
+        if (dx[i].real() > 0) worst = std::max(worst, std::sqrt(dd[i].real() / dx[i].real()));
+    *rel = worst;
+    return OVL_OK;
+}
+
+// ---- single-mode drop-ins on the first mode block of the scratch field ----
+static int dropin_setup(ovl_ctx *c, const double *ucc, const double *phi)
+{
+    if (!ucc || !phi) return fail(c, OVL_E_ARG, "null argument");
+    const size_t nu = (size_t)3 * c->g.V * c->g.nc * c->g.nc * 2, np = (size_t)c->g.V * c->ncomp * 2;
+    if (!all_finite(ucc, nu) || !all_finite(phi, np)) return fail(c, OVL_E_ARG, "non-finite input");
+    RETCHK(stage(c, nu * sizeof(double)));
+    HIPCHK(c, hipMemcpyAsync(c->staging, ucc, nu * sizeof(double), hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, ovl::launch_ucc_to_links(c->g, c->staging, c->ul64, c->ul32, c->st));
+    c->stencil_dirty = true;  // the next batched call re-derives the stencil from the master links
+    Geom g1 = c->g;
+    g1.nblk = 1;
+    HIPCHK(c, hipMemsetAsync(c->field[OVL_SCRATCH], 0, c->blk64_elems * 16, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    RETCHK(stage(c, np * sizeof(double)));
+    HIPCHK(c, hipMemcpyAsync(c->staging, phi, np * sizeof(double), hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, ovl::launch_aos_to_soa(g1, c->staging, 0, 1, c->field[OVL_SCRATCH], c->st));
+    return OVL_OK;
+}
+
+static int dropin_fetch(ovl_ctx *c, const void *field, double *psi)
+{
+    Geom g1 = c->g;
+    g1.nblk = 1;
+    const size_t np = (size_t)c->g.V * c->ncomp * 2;
+    HIPCHK(c, ovl::launch_soa_to_aos(g1, field, 0, 1, c->staging, c->st));
+    HIPCHK(c, hipMemcpyAsync(psi, c->staging, np * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return OVL_OK;
+}
+
+int ovl_wilson_evo_compute(ovl_ctx *c, const double *ucc, const double *phi, double *psi)
+{
+    if (!c || !psi) return OVL_E_ARG;
+    RETCHK(dropin_setup(c, ucc, phi));
+    RETCHK(ensure_work(c, 1, false));
+    Geom g1 = c->g;
+    g1.nblk = 1;
+    HopArgs a = hop_args(c, P64);
+    a.nblk = 1;
+    a.in = c->field[OVL_SCRATCH];
+    a.out = c->w64;
+    a.sh_i = -1.0;
+    RETCHK(hop(c, g1, P64, P64, a));
+    return dropin_fetch(c, c->w64, psi);
+}
+
+int ovl_overlap_operator_compute(ovl_ctx *c, const double *ucc, const double *phi, double *psi, double lmin,
+                                 double lmax)
+{
+    if (!c || !psi) return OVL_E_ARG;
+    if (!(lmin > 0.0) || !(lmax > lmin)) return fail(c, OVL_E_SPECTRUM, "Zolotarev interval needs 0 < lmin < lmax");
+    RETCHK(dropin_setup(c, ucc, phi));
+    RETCHK(ensure_work(c, 1, false));
+    Geom g1 = c->g;
+    g1.nblk = 1;
+    // zolo_overlap's ensure_stencil() must not overwrite the uploaded u_cc links
+    const bool saved_set = c->links_set;
+    c->links_set = true;
+    c->stencil_dirty = false;
+    int r = zolo_overlap(c, g1, c->field[OVL_SCRATCH], c->w64, lmin, lmax, 0, 0.0);
+    c->links_set = saved_set;
+    c->stencil_dirty = true;
+    if (r != OVL_OK && r != OVL_E_CONVERGE) return r;
+    RETCHK(dropin_fetch(c, c->w64, psi));
+    return r;
+}
+
+// ---- Ritz (RitzSolver.cpp:92-297), neigen = 1, on mode 0 of one block ----
+int ovl_ritz(ovl_ctx *c, const double *start_flat, double sign, double delitr, int nitlz, int nrestart,
+             double *eigenvalue)
+{
+    if (!c || !start_flat || !eigenvalue) return OVL_E_ARG;
+    RETCHK(ensure_stencil(c));
+    Geom g = c->g;
+    g.nblk = 1;
+    const size_t fb = c->blk64_elems * 16;
+    for (int i = 0; i < 6; ++i) RETCHK(dalloc(c, &c->rz[i], fb));
+    void *xr = c->rz[0], *yr = c->rz[1], *zr = c->rz[2], *gr = c->rz[3], *pr = c->rz[4], *w = c->rz[5];
+    // flat (m = s + V(spin + 4 colour)) -> AoS -> SoA mode 0
+    const int V = c->g.V, nc = c->g.nc;
+    std::vector<double> aos((size_t)V * c->ncomp * 2);
+    for (int s = 0; s < V; ++s)
+        for (int k = 0; k < 4; ++k)
+            for (int j = 0; j < nc; ++j) {
+                const size_t m = s + (size_t)V * (k + 4 * j);
+                aos[2 * ((s * nc + j) * 4 + k)] = start_flat[2 * m];
+                aos[2 * ((s * nc + j) * 4 + k) + 1] = start_flat[2 * m + 1];
+            }
+    RETCHK(stage(c, aos.size() * sizeof(double)));
+    for (void *v : {xr, yr, zr, gr, pr, w}) HIPCHK(c, hipMemsetAsync(v, 0, fb, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->staging, aos.data(), aos.size() * sizeof(double), hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, ovl::launch_aos_to_soa(g, c->staging, 0, 1, xr, c->st));
+    auto dot = [&](const void *a, const void *b, cplx &res) -> int {
+        std::vector<cplx> d;
+        RETCHK(mode_dot(c, g, a, b, d));
+        res = d[0];
+        return OVL_OK;
+    };
+    auto lin = [&](void *out, const void *a, cplx ca, const void *b, cplx cb) -> int {
+        HIPCHK(c, ovl::launch_lincomb(g, 1, P64, out, a, ca.real(), ca.imag(), b, cb.real(), cb.imag(), c->st));
+        return OVL_OK;
+    };
+    auto uu = [&](const void *v, void *out) -> int {  // out = sign * h^2 v
+        RETCHK(apply_h2(c, g, v, w, out, 0.0));
+        return lin(out, out, sign, nullptr, 0.0);
+    };
+    cplx t;
+    RETCHK(dot(xr, xr, t));
+    double xnorm = 1.0 / std::sqrt(t.real());
+    RETCHK(lin(xr, xr, xnorm, nullptr, 0.0));
+    int ittot = 0;
+    bool done = false;
+    cplx xynorm = 0, oxynorm = 0, xpnorm = 0, amb = 0;
+    double gnorm = 0, gnormim = 0, pnorm = 0;
+    while (!done) {
+        RETCHK(uu(xr, yr));
+        RETCHK(dot(xr, yr, xynorm));
+        RETCHK(lin(gr, yr, 1.0, xr, -xynorm));
+        RETCHK(lin(pr, gr, 1.0, nullptr, 0.0));
+        RETCHK(dot(gr, gr, t));
+        gnormim = t.real();
+        pnorm = std::sqrt(gnormim);
+        RETCHK(lin(pr, pr, 1.0 / pnorm, nullptr, 0.0));
+        for (int itt = 1; itt <= nrestart; ++itt) {
+            ittot++;
+            RETCHK(uu(pr, zr));
+            amb = xynorm;
+            RETCHK(dot(pr, zr, t));
+            amb -= t.real();
+            RETCHK(dot(xr, zr, t));
+            const double ccc = t.real();
+            const double cox = 2.0 * ccc / amb.real();
+            const double sig1 = -std::copysign(1.0, amb.real());
+            const double sig2 = std::copysign(1.0, cox) * sig1;
+            const double cox2 = cox * cox;
+            double ct, stt;
+            if (cox2 <= 1.0e-3) {
+                // series of RitzSolver.cpp:199-200
+                const double sqp = 1.0 - 0.125 * cox2 + 0.0859375 * cox2 * cox2 - 0.0673828125 * cox2 * cox2 * cox2 +
+                                   0.0562439 * cox2 * cox2 * cox2 * cox2 + 0.0487 * cox2 * cox2 * cox2 * cox2 * cox2;
+                const double sqm = 0.5 * sig1 * sig2 * cox *
+                                   (1.0 - 0.375 * cox2 + 0.2421875 * cox2 * cox2 - 0.182617875 * cox2 * cox2 * cox2 +
+                                    0.1482849 * cox2 * cox2 * cox2 * cox2 - 0.12575 * cox2 * cox2 * cox2 * cox2 * cox2 +
+                                    0.1097 * cox2 * cox2 * cox2 * cox2 * cox2 * cox2);
+                if (sig1 < 0.0) { ct = sqm; stt = sig2 * sqp; }
+                else { ct = sqp; stt = sig2 * sqm; }
+            } else {
+                const double sox = sig1 / std::sqrt(1.0 + cox2);
+                ct = std::sqrt((1.0 + sox) * 0.5);
+                stt = std::sqrt((1.0 - sox) * 0.5) * sig2;
+            }
+            RETCHK(lin(xr, xr, ct, pr, stt));
+            RETCHK(lin(yr, yr, ct, zr, stt));
+            RETCHK(dot(xr, xr, t));
+            xnorm = t.real();
+            oxynorm = xynorm;
+            RETCHK(dot(xr, yr, xynorm));
+            xnorm = 1.0 / std::sqrt(xnorm);
+            xynorm *= xnorm;
+            RETCHK(lin(xr, xr, xnorm, nullptr, 0.0));
+            RETCHK(lin(gr, yr, 1.0, xr, -xynorm));
+            RETCHK(dot(gr, gr, t));
+            gnorm = t.real();
+            RETCHK(dot(xr, pr, xpnorm));
+            if (xynorm.real() > oxynorm.real()) { done = true; break; }
+            if (gnorm < delitr * sign * xynorm.real()) { done = true; break; }
+            const double beta = gnorm / gnormim * pnorm;
+            // pr = gr + beta (pr - xr xpnorm)
+            RETCHK(lin(pr, pr, beta, xr, -beta * xpnorm));
+            RETCHK(lin(pr, pr, 1.0, gr, 1.0));
+            RETCHK(dot(pr, pr, t));
+            pnorm = std::sqrt(t.real());
+            RETCHK(lin(pr, pr, 1.0 / pnorm, nullptr, 0.0));
+            gnormim = gnorm;
+            if (ittot >= nitlz) { done = true; break; }
+        }
+    }
+    *eigenvalue = xynorm.real();
+    return OVL_OK;
+}
+
+}  // extern "C"
+
+namespace {
+// MaxEValue = -1.05 Ritz(-1), MinEValue = 0.95 Ritz(+1) with a normalised uniform start vector,
+// 100 restarts, 2000 iterations, 1e-16 (OverlapUpdate.cpp:130-173); the start vector comes from a
+// fixed-seed 64-bit Mersenne twister instead of the reference's GSL stream
+int ritz_bounds(ovl_ctx *c, double *lmin, double *lmax)
+{
+    const size_t n = (size_t)c->g.V * c->ncomp;
+    std::vector<double> v(2 * n);
+    unsigned long long st = 0x9E3779B97F4A7C15ull;
+    double nrm = 0;
+    for (size_t i = 0; i < 2 * n; ++i) {
+        st ^= st << 13; st ^= st >> 7; st ^= st << 17;  // xorshift64
+        v[i] = (double)(st >> 11) * (1.0 / 9007199254740992.0);
+        nrm += v[i] * v[i];
+    }
+    nrm = std::sqrt(nrm);
+    for (auto &x : v) x /= nrm;
+    double a = 0, b = 0;
+    RETCHK(ovl_ritz(c, v.data(), -1.0, 1e-16, 2000, 100, &a));
+    RETCHK(ovl_ritz(c, v.data(), 1.0, 1e-16, 2000, 100, &b));
+    *lmax = -1.05 * a;
+    *lmin = 0.95 * b;
+    return OVL_OK;
+}
+}  // namespace
+
+extern "C" {
+
+// ---- dynamics ----
+int ovl_update_links(ovl_ctx *c, double dt)
+{
+    if (!c || !std::isfinite(dt)) return OVL_E_ARG;
+    if (!c->links_set) return fail(c, OVL_E_STATE, "no gauge links uploaded");
+    HIPCHK(c, ovl::launch_link_update(c->g, c->links, c->efield, dt, c->prm.reunitarize, c->st));
+    c->stencil_dirty = true;
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return OVL_OK;
+}
+
+int ovl_update_efield(ovl_ctx *c, double dt)
+{
+    if (!c || !std::isfinite(dt)) return OVL_E_ARG;
+    if (!c->links_set) return fail(c, OVL_E_STATE, "no gauge links uploaded");
+    HIPCHK(c, ovl::launch_efield_update(c->g, c->links, c->efield, dt, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return OVL_OK;
+}
+
+static int gauge_half(ovl_ctx *c, int mode, double h)
+{
+    if (mode == OVL_GAUGE_FROZEN) return OVL_OK;
+    if (mode == OVL_GAUGE_FULL) HIPCHK(c, ovl::launch_efield_update(c->g, c->links, c->efield, h, c->st));
+    HIPCHK(c, ovl::launch_link_update(c->g, c->links, c->efield, h, c->prm.reunitarize, c->st));
+    HIPCHK(c, ovl::launch_link_setup(c->g, c->links, c->phase, c->ul64, c->ul32, c->st));
+    return OVL_OK;
+}
+
+// target += dt * (-i h src)   (OverlapUpdate.cpp:85-124, WILSON/Update.cpp:6-45)
+static int fermion_half(ovl_ctx *c, const ovl_step_opts *o, int src, int tgt)
+{
+    if (o->kind == OVL_WILSON) {
+        HopArgs a = hop_args(c, P64);
+        a.in = c->field[src];
+        a.out = c->field[tgt];
+        a.tgt = c->field[tgt];
+        a.st_r = 1.0;
+        a.sh_i = -o->dt;
+        return hop(c, c->g, P64, P64, a);
+    }
+    if (o->sign.method == OVL_SIGN_ZOLOTAREV)
+        return zolo_overlap(c, c->g, c->field[src], c->field[tgt], o->sign.lmin, o->sign.lmax, 1, o->dt);
+    return sign_driver(c, &o->sign, c->field[src], c->field[tgt], FIN_OVERLAP_AXPY, o->dt);
+}
+
+static int one_step(ovl_ctx *c, const ovl_step_opts *o)
+{
+    RETCHK(fermion_half(c, o, OVL_PSIMID, OVL_PSI));
+    RETCHK(gauge_half(c, o->gauge, 0.5 * o->dt));
+    RETCHK(fermion_half(c, o, OVL_PSI, OVL_PSIMID));
+    RETCHK(gauge_half(c, o->gauge, 0.5 * o->dt));
+    return OVL_OK;
+}
+
+int ovl_fermion_update(ovl_ctx *c, const ovl_step_opts *o, int nsteps)
+{
+    if (!c || !o || nsteps < 0 || !std::isfinite(o->dt)) return OVL_E_ARG;
+    if (o->kind != OVL_WILSON && o->kind != OVL_OVERLAP) return fail(c, OVL_E_ARG, "unknown fermion kind");
+    if (o->gauge < OVL_GAUGE_FROZEN || o->gauge > OVL_GAUGE_FULL) return fail(c, OVL_E_ARG, "unknown gauge mode");
+    if (o->kind == OVL_OVERLAP) RETCHK(check_sign_opts(c, &o->sign));
+    if (o->kind == OVL_OVERLAP && o->sign.method == OVL_SIGN_CHEBYSHEV) {
+        SignPlan pl;
+        RETCHK(make_plan(c, &o->sign, pl));
+    }
+    RETCHK(ensure_stencil(c));
+    begin_timing(c);
+    const bool graph = o->use_graph && !(o->kind == OVL_OVERLAP && o->sign.method == OVL_SIGN_ZOLOTAREV) && nsteps > 0;
+    if (!graph) {
+        for (int s = 0; s < nsteps; ++s) RETCHK(one_step(c, o));
+        return end_timing(c);
+    }
+    // capture one full step (both half steps and both gauge half steps) and replay it
+    RETCHK(ensure_work(c, (o->sign.mode_chunk > 0) ? std::min(o->sign.mode_chunk, c->g.nblk) : c->g.nblk,
+                       o->kind == OVL_OVERLAP && o->sign.precision != OVL_PREC_FP64));
+    hipGraph_t graph_h = nullptr;
+    hipGraphExec_t exec = nullptr;
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    HIPCHK(c, hipEventRecord(c->t0, c->st));
+    c->capturing = true;
+    hipError_t e = hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+        c->capturing = false;
+        return fail(c, OVL_E_HIP, std::string("hipStreamBeginCapture: ") + hipGetErrorString(e));
+    }
+    const long long l0 = c->hop_launches;
+    const double b0 = c->hop_bytes;
+    long long vl0[3];
+    double vb0[3];
+    for (int v = 0; v < 3; ++v) { vl0[v] = c->var_launches[v]; vb0[v] = c->var_bytes[v]; }
+    const int s64 = c->fp64_steps, s32 = c->fp32_steps;
+    int r = one_step(c, o);
+    hipError_t ee = hipStreamEndCapture(c->st, &graph_h);
+    c->capturing = false;
+    if (r != OVL_OK) {
+        if (graph_h) hipGraphDestroy(graph_h);
+        return r;
+    }
+    if (ee != hipSuccess) return fail(c, OVL_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ee));
+    const long long per_l = c->hop_launches - l0;
+    const double per_b = c->hop_bytes - b0;
+    const int p64 = c->fp64_steps - s64, p32 = c->fp32_steps - s32;
+    e = hipGraphInstantiate(&exec, graph_h, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        hipGraphDestroy(graph_h);
+        return fail(c, OVL_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+    for (int s = 0; s < nsteps && e == hipSuccess; ++s) e = hipGraphLaunch(exec, c->st);
+    hipGraphExecDestroy(exec);
+    hipGraphDestroy(graph_h);
+    if (e != hipSuccess) return fail(c, OVL_E_HIP, std::string("hipGraphLaunch: ") + hipGetErrorString(e));
+    c->hop_launches = l0 + per_l * nsteps;
+    c->hop_bytes = b0 + per_b * nsteps;
+    for (int v = 0; v < 3; ++v) {
+        c->var_launches[v] = vl0[v] + (c->var_launches[v] - vl0[v]) * nsteps;
+        c->var_bytes[v] = vb0[v] + (c->var_bytes[v] - vb0[v]) * nsteps;
+    }
+    c->fp64_steps = s64 + p64 * nsteps;
+    c->fp32_steps = s32 + p32 * nsteps;
+    return end_timing(c);
+}
+
+// ---- observables ----
+static int fetch_out(ovl_ctx *c, double *h, int n)
+{
+    HIPCHK(c, hipMemcpyAsync(h, c->dout, sizeof(double) * n, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return OVL_OK;
+}
+
+int ovl_energy(ovl_ctx *c, double *e_el, double *e_mag)
+{
+    if (!c || !e_el || !e_mag) return OVL_E_ARG;
+    if (!c->links_set) return fail(c, OVL_E_STATE, "no gauge links uploaded");
+    HIPCHK(c, ovl::launch_energy(c->g, c->links, c->efield, c->partial, c->dout, c->st));
+    double h[2];
+    RETCHK(fetch_out(c, h, 2));
+    *e_el = h[0];
+    *e_mag = h[1];
+    return OVL_OK;
+}
+
+int ovl_gauss_violation(ovl_ctx *c, double *vmax, double *vrms)
+{
+    if (!c || !vmax || !vrms) return OVL_E_ARG;
+    if (!c->links_set) return fail(c, OVL_E_STATE, "no gauge links uploaded");
+    HIPCHK(c, ovl::launch_gauss(c->g, c->links, c->efield, c->partial, c->dout, c->st));
+    double h[2];
+    RETCHK(fetch_out(c, h, 2));
+    *vmax = h[0];
+    *vrms = std::sqrt(h[1] / ((c->g.nc * c->g.nc - 1) * (double)c->g.V));
+    return OVL_OK;
+}
+
+int ovl_unitarity_violation(ovl_ctx *c, double *vmax, double *vrms)
+{
+    if (!c || !vmax || !vrms) return OVL_E_ARG;
+    if (!c->links_set) return fail(c, OVL_E_STATE, "no gauge links uploaded");
+    HIPCHK(c, ovl::launch_unitarity(c->g, c->links, c->partial, c->dout, c->st));
+    double h[2];
+    RETCHK(fetch_out(c, h, 2));
+    *vmax = h[0];
+    *vrms = std::sqrt(h[1] / (3.0 * c->g.V));
+    return OVL_OK;
+}
+
+int ovl_ncs_dot(ovl_ctx *c, double *ncsdot)
+{
+    if (!c || !ncsdot) return OVL_E_ARG;
+    if (!c->links_set) return fail(c, OVL_E_STATE, "no gauge links uploaded");
+    HIPCHK(c, ovl::launch_edotb(c->g, c->links, c->efield, c->partial, c->dout, c->st));
+    double h[1];
+    RETCHK(fetch_out(c, h, 1));
+    *ncsdot = 0.125 * h[0] / (M_PI * M_PI);  // ChernSimonsDerivative.cpp:47
+    return OVL_OK;
+}
+
+static void build_gammas(std::vector<double> &out)
+{
+    // OverlapCurrents.cpp:14-97 coupling matrices from DiracAlgebra.cpp:5-9
+    const cplx I(0, 1);
+    cplx g0[4][4] = {{1, 0, 0, 0}, {0, 1, 0, 0}, {0, 0, -1, 0}, {0, 0, 0, -1}};
+    cplx gx[4][4] = {{0, 0, 0, 1}, {0, 0, 1, 0}, {0, -1, 0, 0}, {-1, 0, 0, 0}};
+    cplx gy[4][4] = {{0, 0, 0, -I}, {0, 0, I, 0}, {0, I, 0, 0}, {-I, 0, 0, 0}};
+    cplx gz[4][4] = {{0, 0, 1, 0}, {0, 0, 0, -1}, {-1, 0, 0, 0}, {0, 1, 0, 0}};
+    cplx g5[4][4] = {{0, 0, 1, 0}, {0, 0, 0, 1}, {1, 0, 0, 0}, {0, 1, 0, 0}};
+    cplx(*gm[4])[4] = {g0, gx, gy, gz};
+    out.assign(4 * 4 * 16 * 2, 0.0);
+    for (int mu = 0; mu < 4; ++mu)
+        for (int a = 0; a < 4; ++a)
+            for (int b = 0; b < 4; ++b) {
+                cplx v = 0, ax = 0, v2 = 0, a2 = 0;
+                for (int g = 0; g < 4; ++g) {
+                    v += g0[a][g] * gm[mu][g][b];
+                    for (int d = 0; d < 4; ++d) {
+                        ax += g0[a][g] * gm[mu][g][d] * g5[d][b];
+                        v2 += cplx(0, -0.5) * g0[a][g] * gm[mu][g][d] * g0[d][b];
+                        for (int e = 0; e < 4; ++e) a2 += cplx(0, -0.5) * g0[a][g] * gm[mu][g][d] * g5[d][e] * g0[e][b];
+                    }
+                }
+                const cplx vals[4] = {v, ax, v2, a2};
+                for (int k = 0; k < 4; ++k) {
+                    const size_t idx = ((k * 4 + mu) * 4 + a) * 4 + b;
+                    out[2 * idx] = vals[k].real();
+                    out[2 * idx + 1] = vals[k].imag();
+                }
+            }
+}
+
+int ovl_currents(ovl_ctx *c, int field, const ovl_sign_opts *o, int first_mode, double *jv, double *ja,
+                 double *axial_charge)
+{
+    if (!c) return OVL_E_ARG;
+    RETCHK(check_field(c, field));
+    RETCHK(check_sign_opts(c, o));
+    if (field == OVL_SCRATCH) return fail(c, OVL_E_ARG, "currents use the scratch field for -i h_ov psi");
+    if (o->method == OVL_SIGN_ZOLOTAREV)
+        RETCHK(zolo_overlap(c, c->g, c->field[field], c->field[OVL_SCRATCH], o->lmin, o->lmax, 0, 0.0));
+    else
+        RETCHK(sign_driver(c, o, c->field[field], c->field[OVL_SCRATCH], FIN_OVERLAP, 0.0));
+    if (!c->gam) {
+        std::vector<double> g;
+        build_gammas(g);
+        RETCHK(dalloc(c, (void **)&c->gam, sizeof(double) * g.size()));
+        HIPCHK(c, hipMemcpyAsync(c->gam, g.data(), sizeof(double) * g.size(), hipMemcpyHostToDevice, c->st));
+    }
+    RETCHK(dalloc(c, (void **)&c->jv, sizeof(double) * 4 * c->g.V));
+    RETCHK(dalloc(c, (void **)&c->ja, sizeof(double) * 4 * c->g.V));
+    HIPCHK(c, ovl::launch_currents(c->g, c->field[field], c->field[OVL_SCRATCH], c->gam, first_mode, c->partial, c->jv,
+                                   c->ja, c->st));
+    std::vector<double> hv(4 * (size_t)c->g.V), ha(4 * (size_t)c->g.V);
+    HIPCHK(c, hipMemcpyAsync(hv.data(), c->jv, sizeof(double) * hv.size(), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipMemcpyAsync(ha.data(), c->ja, sizeof(double) * ha.size(), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    if (jv) std::memcpy(jv, hv.data(), sizeof(double) * hv.size());
+    if (ja) std::memcpy(ja, ha.data(), sizeof(double) * ha.size());
+    if (axial_charge) {
+        double s = 0;
+        for (int i = 0; i < c->g.V; ++i) s += ha[4 * i];
+        *axial_charge = s;
+    }
+    return OVL_OK;
+}
+
+int ovl_get_timing(ovl_ctx *c, ovl_timing *t)
+{
+    if (!c || !t) return OVL_E_ARG;
+    *t = c->last;
+    return OVL_OK;
+}
+
+int ovl_set_profiling(ovl_ctx *c, int on)
+{
+    if (!c) return OVL_E_ARG;
+    c->profiling = on != 0;
+    return OVL_OK;
+}
+
+}  // extern "C"

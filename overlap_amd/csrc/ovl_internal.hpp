@@ -1,0 +1,83 @@
+// Internal declarations shared by the host driver and the gfx950 kernels.
+//
+// Device layout of a mode-function field ("mode-block SoA"):
+//   modes are grouped in blocks of MB (16, 32 or 64); inside a block the mode index is the
+//   fastest-running index so a wavefront reads MB consecutive modes of one site per load.
+//   FP64: double2 per complex, element (blk, site, comp, m) at ((blk*V + site)*NCOMP + comp)*MB + m
+//   FP32: float4 per two complex,      (blk, site, comp/2, m) at ((blk*V + site)*NCOMP/2 + comp/2)*MB + m
+//   comp = spin*Nc + colour, NCOMP = 4*Nc.
+// Stencil links: [mu][site][Nc][Nc] complex, row-major, FP64 (double2) and FP32 (float2) copies,
+// each already multiplied by its U(1) phase.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace ovl {
+
+struct Geom {
+    int nc, n0, n1, n2, V;
+    int mb;      // modes per block
+    int nblk;    // mode blocks held per field
+};
+
+// pointwise epilogue of the stencil kernel:
+//   out = sh * h_W(in) + sx * G(x) + s1 * b1 + s2 * b2 + st * tgt      (G = 1 or gamma0)
+// the five operands are block-relative pointers (already offset to the first block launched);
+// a null pointer drops its term.
+struct HopArgs {
+    const void *in;
+    void *out;
+    const void *links;
+    const void *x;
+    const void *b1;
+    const void *b2;
+    const void *tgt;
+    double sh_r, sh_i, sx_r, sx_i, s1_r, s1_i, s2_r, s2_i, st_r, st_i;
+    double amass;     // A = 6r + 2m - 2M of h_W = (1/2) gamma0 [A - hopping]
+    int x_gamma0;     // 1: G = gamma0
+    int nblk;         // blocks covered by this launch
+};
+
+enum Prec { P64 = 0, P32 = 1 };
+
+// launchers (ovl_kernels.hip); all enqueue on `st` and return hipError_t
+hipError_t launch_hop(const Geom &g, int pin, int pout, const HopArgs &a, hipStream_t st);
+hipError_t launch_convert(const Geom &g, int nblk, const void *src, int psrc, void *dst, int pdst, hipStream_t st);
+// out = ca * a + cb * b (same precision p), b may be null
+hipError_t launch_lincomb(const Geom &g, int nblk, int p, void *out, const void *a, double ca_r, double ca_i,
+                          const void *b, double cb_r, double cb_i, hipStream_t st);
+hipError_t launch_aos_to_soa(const Geom &g, const double *aos, int first, int count, void *field, hipStream_t st);
+hipError_t launch_soa_to_aos(const Geom &g, const void *field, int first, int count, double *aos, hipStream_t st);
+hipError_t launch_link_setup(const Geom &g, const double *links, const double *phase, double *ul64, float *ul32,
+                             hipStream_t st);
+hipError_t launch_ucc_to_links(const Geom &g, const double *ucc, double *ul64, float *ul32, hipStream_t st);
+hipError_t launch_link_update(const Geom &g, double *links, const double *efield, double dt, int reunit,
+                              hipStream_t st);
+hipError_t launch_efield_update(const Geom &g, const double *links, double *efield, double dt, hipStream_t st);
+// per-mode inner products <a,b> = sum conj(a) b over one field (FP64 fields), nmodes results
+// written as (re, im) pairs to `out` (device); `partial` scratch of partial_len doubles
+hipError_t launch_mode_dot(const Geom &g, const void *a, const void *b, double *partial, double *out,
+                           hipStream_t st);
+size_t mode_dot_partial_len(const Geom &g);
+// global observables, written to device `out`
+hipError_t launch_energy(const Geom &g, const double *links, const double *efield, double *partial, double *out,
+                         hipStream_t st);
+hipError_t launch_gauss(const Geom &g, const double *links, const double *efield, double *partial, double *out,
+                        hipStream_t st);
+hipError_t launch_unitarity(const Geom &g, const double *links, double *partial, double *out, hipStream_t st);
+hipError_t launch_edotb(const Geom &g, const double *links, const double *efield, double *partial, double *out,
+                        hipStream_t st);
+size_t reduce_partial_len(const Geom &g);
+// per-site currents summed over modes: jv, ja [4*site + mu] (device); psi, hpsi FP64 fields
+hipError_t launch_currents(const Geom &g, const void *psi, const void *hpsi, const double *gam, int first_mode,
+                           double *partial, double *jv, double *ja, hipStream_t st);
+size_t currents_partial_len(const Geom &g);
+// multi-shift CG helpers (FP64 fields)
+hipError_t launch_mscg_update(const Geom &g, int nshift, double *ps, double *chi, const double *r,
+                              const double *coef, const int *active, hipStream_t st);
+hipError_t launch_axpy_modes(const Geom &g, void *y, const void *x, const double *coef, int coef_stride,
+                             double sign, const int *active, hipStream_t st);
+
+}  // namespace ovl

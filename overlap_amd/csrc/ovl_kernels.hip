@@ -1,0 +1,1236 @@
+// gfx950 kernels of the OVERLAP fermion hot path.
+//
+// All spinor kernels are HBM-bandwidth bound: one lane owns one (site, mode) pair, a wavefront
+// covers 64 consecutive (mode, site) lanes of one mode block, so every 16-byte load of a spinor
+// component is one contiguous MB*16-byte segment.  Links are shared by all modes of a site and
+// read once per wavefront (MB = 64) or once per MB-lane group.
+#include "ovl_internal.hpp"
+
+#include <cmath>
+
+namespace ovl {
+
+namespace {
+
+constexpr int TPB = 256;  // threads per block for the streaming kernels
+
+template <typename T> struct Vec;
+template <> struct Vec<double> { using F = double2; using L = double2; static constexpr int PER = 1; };
+template <> struct Vec<float> { using F = float4; using L = float2; static constexpr int PER = 2; };
+
+// ---------------------------------------------------------------------------------------------
+// spinor load / store for one lane
+// ---------------------------------------------------------------------------------------------
+template <int NC, int MB, typename T>
+__device__ __forceinline__ void load_sp(const void *__restrict__ base, int V, int blk, int s, int m, T *re, T *im)
+{
+    constexpr int NCOMP = 4 * NC;
+    if constexpr (sizeof(T) == 8) {
+        const double2 *p = reinterpret_cast<const double2 *>(base) + ((size_t)(blk * V + s) * NCOMP) * MB + m;
+#pragma unroll
+        for (int c = 0; c < NCOMP; ++c) {
+            double2 t = p[(size_t)c * MB];
+            re[c] = t.x;
+            im[c] = t.y;
+        }
+    } else {
+        const float4 *p = reinterpret_cast<const float4 *>(base) + ((size_t)(blk * V + s) * (NCOMP / 2)) * MB + m;
+#pragma unroll
+        for (int c = 0; c < NCOMP / 2; ++c) {
+            float4 t = p[(size_t)c * MB];
+            re[2 * c] = t.x;
+            im[2 * c] = t.y;
+            re[2 * c + 1] = t.z;
+            im[2 * c + 1] = t.w;
+        }
+    }
+}
+
+template <int NC, int MB, typename T>
+__device__ __forceinline__ void store_sp(void *__restrict__ base, int V, int blk, int s, int m, const T *re,
+                                         const T *im)
+{
+    constexpr int NCOMP = 4 * NC;
+    if constexpr (sizeof(T) == 8) {
+        double2 *p = reinterpret_cast<double2 *>(base) + ((size_t)(blk * V + s) * NCOMP) * MB + m;
+#pragma unroll
+        for (int c = 0; c < NCOMP; ++c) p[(size_t)c * MB] = make_double2(re[c], im[c]);
+    } else {
+        float4 *p = reinterpret_cast<float4 *>(base) + ((size_t)(blk * V + s) * (NCOMP / 2)) * MB + m;
+#pragma unroll
+        for (int c = 0; c < NCOMP / 2; ++c)
+            p[(size_t)c * MB] = make_float4(re[2 * c], im[2 * c], re[2 * c + 1], im[2 * c + 1]);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// one direction of the Wilson hopping term (spin-projected):
+//   forward  (sg=+1): h = u + i sigma_k l,  v = U h,      X_u -= v, X_l += i sigma_k v
+//   backward (sg=-1): g = u - i sigma_k l,  v = U^+ g,    X_u -= v, X_l -= i sigma_k v
+// which is -(1 +- i gamma_k) U phi  in the Dirac basis of DiracAlgebra.cpp:5-9.
+// ---------------------------------------------------------------------------------------------
+template <int NC, int K, int SG, typename T, typename LT>
+__device__ __forceinline__ void hop_dir(const T *pr, const T *pi, const LT *__restrict__ U, T *xr, T *xi)
+{
+    // spinor component index: spin*NC + colour; u = spins 0,1, l = spins 2,3
+    T hr[2][NC], hi[2][NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        const T u0r = pr[0 * NC + c], u0i = pi[0 * NC + c];
+        const T u1r = pr[1 * NC + c], u1i = pi[1 * NC + c];
+        const T l0r = pr[2 * NC + c], l0i = pi[2 * NC + c];
+        const T l1r = pr[3 * NC + c], l1i = pi[3 * NC + c];
+        if constexpr (K == 0) {  // i sigma_x l = (i l1, i l0)
+            hr[0][c] = u0r - SG * l1i; hi[0][c] = u0i + SG * l1r;
+            hr[1][c] = u1r - SG * l0i; hi[1][c] = u1i + SG * l0r;
+        } else if constexpr (K == 1) {  // i sigma_y l = (l1, -l0)
+            hr[0][c] = u0r + SG * l1r; hi[0][c] = u0i + SG * l1i;
+            hr[1][c] = u1r - SG * l0r; hi[1][c] = u1i - SG * l0i;
+        } else {  // i sigma_z l = (i l0, -i l1)
+            hr[0][c] = u0r - SG * l0i; hi[0][c] = u0i + SG * l0r;
+            hr[1][c] = u1r + SG * l1i; hi[1][c] = u1i - SG * l1r;
+        }
+    }
+    T vr[2][NC], vi[2][NC];
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int i = 0; i < NC; ++i) {
+            T ar = 0, ai = 0;
+#pragma unroll
+            for (int j = 0; j < NC; ++j) {
+                if constexpr (SG > 0) {
+                    const LT u = U[i * NC + j];
+                    const T ur = (T)u.x, ui = (T)u.y;
+                    ar += ur * hr[s][j] - ui * hi[s][j];
+                    ai += ur * hi[s][j] + ui * hr[s][j];
+                } else {
+                    const LT u = U[j * NC + i];  // (U^+)_{ij} = conj(U_{ji})
+                    const T ur = (T)u.x, ui = (T)u.y;
+                    ar += ur * hr[s][j] + ui * hi[s][j];
+                    ai += ur * hi[s][j] - ui * hr[s][j];
+                }
+            }
+            vr[s][i] = ar;
+            vi[s][i] = ai;
+        }
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        xr[0 * NC + c] -= vr[0][c]; xi[0 * NC + c] -= vi[0][c];
+        xr[1 * NC + c] -= vr[1][c]; xi[1 * NC + c] -= vi[1][c];
+        // X_l += SG * i sigma_k v
+        if constexpr (K == 0) {  // (i v1, i v0)
+            xr[2 * NC + c] -= SG * vi[1][c]; xi[2 * NC + c] += SG * vr[1][c];
+            xr[3 * NC + c] -= SG * vi[0][c]; xi[3 * NC + c] += SG * vr[0][c];
+        } else if constexpr (K == 1) {  // (v1, -v0)
+            xr[2 * NC + c] += SG * vr[1][c]; xi[2 * NC + c] += SG * vi[1][c];
+            xr[3 * NC + c] -= SG * vr[0][c]; xi[3 * NC + c] -= SG * vi[0][c];
+        } else {  // (i v0, -i v1)
+            xr[2 * NC + c] -= SG * vi[0][c]; xi[2 * NC + c] += SG * vr[0][c];
+            xr[3 * NC + c] += SG * vi[1][c]; xi[3 * NC + c] -= SG * vr[1][c];
+        }
+    }
+}
+
+// stencil kernel: out = sh*h_W(in) + sx*G(x) + s1*b1 + s2*b2 + st*tgt,
+// h_W = (1/2) gamma0 [A in - sum_k (1+i gamma_k) U in(x+k) + (1-i gamma_k) U^+ in(x-k)]
+// (WilsonOperator.cpp:30-114 computes -i h_W of one mode; here all modes of `nblk` blocks)
+template <int NC, typename TI, typename TO, int MB>
+__global__ __launch_bounds__(TPB) void k_hop(HopArgs a, int n0, int n1, int n2)
+{
+    constexpr int NCOMP = 4 * NC;
+    using LT = typename Vec<TI>::L;
+    const int V = n0 * n1 * n2;
+    const long gl = (long)blockIdx.x * TPB + threadIdx.x;
+    const long total = (long)a.nblk * V * MB;
+    if (gl >= total) return;
+    const int m = (int)(gl % MB);
+    const long sl = gl / MB;
+    const int s = (int)(sl % V);
+    const int blk = (int)(sl / V);
+    const int z = s / (n0 * n1);
+    const int rem = s - z * n0 * n1;
+    const int y = rem / n0;
+    const int x = rem - y * n0;
+
+    const LT *__restrict__ L = reinterpret_cast<const LT *>(a.links);
+    constexpr int NN = NC * NC;
+
+    TI xr[NCOMP], xi[NCOMP];
+    TI pr[NCOMP], pi[NCOMP];
+    load_sp<NC, MB, TI>(a.in, V, blk, s, m, pr, pi);
+    const TI am = (TI)a.amass;
+#pragma unroll
+    for (int c = 0; c < NCOMP; ++c) { xr[c] = am * pr[c]; xi[c] = am * pi[c]; }
+
+    {  // x direction
+        const int sp = s + ((x + 1 == n0) ? 1 - n0 : 1);
+        const int sm = s + ((x == 0) ? n0 - 1 : -1);
+        load_sp<NC, MB, TI>(a.in, V, blk, sp, m, pr, pi);
+        hop_dir<NC, 0, 1>(pr, pi, L + ((size_t)0 * V + s) * NN, xr, xi);
+        load_sp<NC, MB, TI>(a.in, V, blk, sm, m, pr, pi);
+        hop_dir<NC, 0, -1>(pr, pi, L + ((size_t)0 * V + sm) * NN, xr, xi);
+    }
+    {  // y direction
+        const int sp = s + ((y + 1 == n1) ? (1 - n1) * n0 : n0);
+        const int sm = s + ((y == 0) ? (n1 - 1) * n0 : -n0);
+        load_sp<NC, MB, TI>(a.in, V, blk, sp, m, pr, pi);
+        hop_dir<NC, 1, 1>(pr, pi, L + ((size_t)1 * V + s) * NN, xr, xi);
+        load_sp<NC, MB, TI>(a.in, V, blk, sm, m, pr, pi);
+        hop_dir<NC, 1, -1>(pr, pi, L + ((size_t)1 * V + sm) * NN, xr, xi);
+    }
+    {  // z direction
+        const int sp = s + ((z + 1 == n2) ? (1 - n2) * n0 * n1 : n0 * n1);
+        const int sm = s + ((z == 0) ? (n2 - 1) * n0 * n1 : -n0 * n1);
+        load_sp<NC, MB, TI>(a.in, V, blk, sp, m, pr, pi);
+        hop_dir<NC, 2, 1>(pr, pi, L + ((size_t)2 * V + s) * NN, xr, xi);
+        load_sp<NC, MB, TI>(a.in, V, blk, sm, m, pr, pi);
+        hop_dir<NC, 2, -1>(pr, pi, L + ((size_t)2 * V + sm) * NN, xr, xi);
+    }
+
+    // h_W = (1/2) gamma0 X, then the epilogue in the output precision
+    TO orr[NCOMP], oi[NCOMP];
+    const TO shr = (TO)a.sh_r, shi = (TO)a.sh_i;
+#pragma unroll
+    for (int c = 0; c < NCOMP; ++c) {
+        const TO g = (c < 2 * NC) ? (TO)0.5 : (TO)-0.5;
+        const TO hr = g * (TO)xr[c], hi = g * (TO)xi[c];
+        orr[c] = shr * hr - shi * hi;
+        oi[c] = shr * hi + shi * hr;
+    }
+    TO tr[NCOMP], ti[NCOMP];
+    if (a.x) {
+        load_sp<NC, MB, TO>(a.x, V, blk, s, m, tr, ti);
+        const TO cr = (TO)a.sx_r, ci = (TO)a.sx_i;
+#pragma unroll
+        for (int c = 0; c < NCOMP; ++c) {
+            const TO g = (a.x_gamma0 && c >= 2 * NC) ? (TO)-1 : (TO)1;
+            const TO vr = g * tr[c], vi = g * ti[c];
+            orr[c] += cr * vr - ci * vi;
+            oi[c] += cr * vi + ci * vr;
+        }
+    }
+    if (a.b1) {
+        load_sp<NC, MB, TO>(a.b1, V, blk, s, m, tr, ti);
+        const TO cr = (TO)a.s1_r, ci = (TO)a.s1_i;
+#pragma unroll
+        for (int c = 0; c < NCOMP; ++c) { orr[c] += cr * tr[c] - ci * ti[c]; oi[c] += cr * ti[c] + ci * tr[c]; }
+    }
+    if (a.b2) {
+        load_sp<NC, MB, TO>(a.b2, V, blk, s, m, tr, ti);
+        const TO cr = (TO)a.s2_r, ci = (TO)a.s2_i;
+#pragma unroll
+        for (int c = 0; c < NCOMP; ++c) { orr[c] += cr * tr[c] - ci * ti[c]; oi[c] += cr * ti[c] + ci * tr[c]; }
+    }
+    if (a.tgt) {
+        load_sp<NC, MB, TO>(a.tgt, V, blk, s, m, tr, ti);
+        const TO cr = (TO)a.st_r, ci = (TO)a.st_i;
+#pragma unroll
+        for (int c = 0; c < NCOMP; ++c) { orr[c] += cr * tr[c] - ci * ti[c]; oi[c] += cr * ti[c] + ci * tr[c]; }
+    }
+    store_sp<NC, MB, TO>(a.out, V, blk, s, m, orr, oi);
+}
+
+// ---------------------------------------------------------------------------------------------
+// pointwise kernels
+// ---------------------------------------------------------------------------------------------
+// precision conversion between the FP64 (double2) and FP32 (float4) layouts
+__global__ __launch_bounds__(TPB) void k_d2f(const double2 *__restrict__ src, float4 *__restrict__ dst, long npairs,
+                                             int mb)
+{
+    long i = (long)blockIdx.x * TPB + threadIdx.x;
+    if (i >= npairs) return;
+    const long m = i % mb, q = i / mb;  // q = pair index (blk,site,comp/2)
+    const double2 a = src[(2 * q) * mb + m], b = src[(2 * q + 1) * mb + m];
+    dst[i] = make_float4((float)a.x, (float)a.y, (float)b.x, (float)b.y);
+}
+
+__global__ __launch_bounds__(TPB) void k_f2d(const float4 *__restrict__ src, double2 *__restrict__ dst, long npairs,
+                                             int mb)
+{
+    long i = (long)blockIdx.x * TPB + threadIdx.x;
+    if (i >= npairs) return;
+    const long m = i % mb, q = i / mb;
+    const float4 t = src[i];
+    dst[(2 * q) * mb + m] = make_double2(t.x, t.y);
+    dst[(2 * q + 1) * mb + m] = make_double2(t.z, t.w);
+}
+
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_lincomb(T *__restrict__ out, const T *__restrict__ a, double car, double cai,
+                                                 const T *__restrict__ b, double cbr, double cbi, long ncplx)
+{
+    long i = (long)blockIdx.x * TPB + threadIdx.x;
+    if (i >= ncplx) return;
+    const T ar = a[2 * i], ai = a[2 * i + 1];
+    T r = (T)car * ar - (T)cai * ai, im = (T)car * ai + (T)cai * ar;
+    if (b) {
+        const T br = b[2 * i], bi = b[2 * i + 1];
+        r += (T)cbr * br - (T)cbi * bi;
+        im += (T)cbr * bi + (T)cbi * br;
+    }
+    out[2 * i] = r;
+    out[2 * i + 1] = im;
+}
+
+// host AoS (count modes starting at `first`) <-> device SoA FP64
+__global__ __launch_bounds__(TPB) void k_aos_to_soa(const double2 *__restrict__ aos, double2 *__restrict__ f, int first,
+                                                    int count, int V, int ncomp, int mb)
+{
+    long i = (long)blockIdx.x * TPB + threadIdx.x;
+    long tot = (long)count * V * ncomp;
+    if (i >= tot) return;
+    const int c = (int)(i % ncomp);
+    const long q = i / ncomp;
+    const int s = (int)(q % V);
+    const int k = (int)(q / V);
+    const int mode = first + k, blk = mode / mb, m = mode % mb;
+    f[((size_t)(blk * V + s) * ncomp + c) * mb + m] = aos[i];
+}
+
+__global__ __launch_bounds__(TPB) void k_soa_to_aos(const double2 *__restrict__ f, double2 *__restrict__ aos, int first,
+                                                    int count, int V, int ncomp, int mb)
+{
+    long i = (long)blockIdx.x * TPB + threadIdx.x;
+    long tot = (long)count * V * ncomp;
+    if (i >= tot) return;
+    const int c = (int)(i % ncomp);
+    const long q = i / ncomp;
+    const int s = (int)(q % V);
+    const int k = (int)(q / V);
+    const int mode = first + k, blk = mode / mb, m = mode % mb;
+    aos[i] = f[((size_t)(blk * V + s) * ncomp + c) * mb + m];
+}
+
+// ---------------------------------------------------------------------------------------------
+// gauge-link algebra (FP64)
+// ---------------------------------------------------------------------------------------------
+struct cd { double r, i; };
+__device__ __forceinline__ cd cmul(cd a, cd b) { return {a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r}; }
+__device__ __forceinline__ cd cmulc(cd a, cd b) { return {a.r * b.r + a.i * b.i, a.i * b.r - a.r * b.i}; }  // a*conj(b)
+__device__ __forceinline__ cd cadd(cd a, cd b) { return {a.r + b.r, a.i + b.i}; }
+
+template <int NC> struct Mat { cd e[NC * NC]; };
+
+template <int NC> __device__ __forceinline__ Mat<NC> mmul(const Mat<NC> &a, const Mat<NC> &b)
+{
+    Mat<NC> c;
+#pragma unroll
+    for (int i = 0; i < NC; ++i)
+#pragma unroll
+        for (int j = 0; j < NC; ++j) {
+            cd s = {0, 0};
+#pragma unroll
+            for (int k = 0; k < NC; ++k) s = cadd(s, cmul(a.e[i * NC + k], b.e[k * NC + j]));
+            c.e[i * NC + j] = s;
+        }
+    return c;
+}
+
+template <int NC> __device__ __forceinline__ Mat<NC> mdag(const Mat<NC> &a)
+{
+    Mat<NC> c;
+#pragma unroll
+    for (int i = 0; i < NC; ++i)
+#pragma unroll
+        for (int j = 0; j < NC; ++j) c.e[i * NC + j] = {a.e[j * NC + i].r, -a.e[j * NC + i].i};
+    return c;
+}
+
+template <int NC> __device__ __forceinline__ Mat<NC> mid()
+{
+    Mat<NC> c;
+#pragma unroll
+    for (int i = 0; i < NC * NC; ++i) c.e[i] = {0, 0};
+#pragma unroll
+    for (int i = 0; i < NC; ++i) c.e[i * NC + i] = {1, 0};
+    return c;
+}
+
+// master link (reference format) -> row-major matrix
+template <int NC> __device__ __forceinline__ Mat<NC> get_link(const double *__restrict__ links, int s, int mu)
+{
+    Mat<NC> m;
+    if constexpr (NC == 2) {
+        const double *u = links + 4 * (mu + 3 * s);  // GetMatrix, SU2/GroupOperations.cpp:213-225
+        m.e[0] = {u[3], u[2]};
+        m.e[1] = {u[1], u[0]};
+        m.e[2] = {-u[1], u[0]};
+        m.e[3] = {u[3], -u[2]};
+    } else {
+        const double *u = links + 18 * (mu + 3 * s);
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) m.e[i * 3 + j] = {u[2 * (i + 3 * j)], u[2 * (i + 3 * j) + 1]};
+    }
+    return m;
+}
+
+// generators t^a as (row, col, value) entries; value = (re, im) times 1/2 (SU(3): lambda_8/(2 sqrt 3))
+template <int NC> __device__ __forceinline__ Mat<NC> alg_to_mat(const double *alpha)
+{
+    Mat<NC> m;
+    if constexpr (NC == 2) {  // alpha_a sigma_a / 2
+        m.e[0] = {0.5 * alpha[2], 0};
+        m.e[3] = {-0.5 * alpha[2], 0};
+        m.e[1] = {0.5 * alpha[0], -0.5 * alpha[1]};
+        m.e[2] = {0.5 * alpha[0], 0.5 * alpha[1]};
+    } else {  // alpha_a lambda_a / 2
+        const double r3 = 0.5 / sqrt(3.0);
+        m.e[0] = {0.5 * alpha[2] + r3 * alpha[7], 0};
+        m.e[4] = {-0.5 * alpha[2] + r3 * alpha[7], 0};
+        m.e[8] = {-2.0 * r3 * alpha[7], 0};
+        m.e[1] = {0.5 * alpha[0], -0.5 * alpha[1]};
+        m.e[3] = {0.5 * alpha[0], 0.5 * alpha[1]};
+        m.e[2] = {0.5 * alpha[3], -0.5 * alpha[4]};
+        m.e[6] = {0.5 * alpha[3], 0.5 * alpha[4]};
+        m.e[5] = {0.5 * alpha[5], -0.5 * alpha[6]};
+        m.e[7] = {0.5 * alpha[5], 0.5 * alpha[6]};
+    }
+    return m;
+}
+
+// ReTr(i t^a V) for all generators
+template <int NC> __device__ __forceinline__ void retr_it(const Mat<NC> &v, double *out)
+{
+    // tr(t^a V) = sum_ij t_ij V_ji; ReTr(i X) = -Im tr(X)
+    if constexpr (NC == 2) {
+        // t0: (0,1)=(1,0)=1/2 -> tr = (V10 + V01)/2
+        out[0] = -0.5 * (v.e[2].i + v.e[1].i);
+        // t1: (0,1)=-i/2, (1,0)=i/2 -> tr = (-i V10 + i V01)/2 ; Im = (V01.r - V10.r)/2
+        out[1] = -0.5 * (v.e[1].r - v.e[2].r);
+        out[2] = -0.5 * (v.e[0].i - v.e[3].i);
+    } else {
+        out[0] = -0.5 * (v.e[3].i + v.e[1].i);
+        out[1] = -0.5 * (v.e[1].r - v.e[3].r);
+        out[2] = -0.5 * (v.e[0].i - v.e[4].i);
+        out[3] = -0.5 * (v.e[6].i + v.e[2].i);
+        out[4] = -0.5 * (v.e[2].r - v.e[6].r);
+        out[5] = -0.5 * (v.e[7].i + v.e[5].i);
+        out[6] = -0.5 * (v.e[5].r - v.e[7].r);
+        const double r3 = 0.5 / sqrt(3.0);
+        out[7] = -r3 * (v.e[0].i + v.e[4].i - 2.0 * v.e[8].i);
+    }
+}
+
+// exp(i c alpha^a t^a): scaling and squaring of the Taylor series (FP64)
+template <int NC> __device__ Mat<NC> iexp(double c, const double *alpha)
+{
+    Mat<NC> a = alg_to_mat<NC>(alpha);
+    double nrm = 0;
+#pragma unroll
+    for (int i = 0; i < NC * NC; ++i) {
+        a.e[i] = {-c * a.e[i].i, c * a.e[i].r};  // i c A
+        nrm += a.e[i].r * a.e[i].r + a.e[i].i * a.e[i].i;
+    }
+    nrm = sqrt(nrm);
+    int sq = 0;
+    while (nrm > 0.25 && sq < 64) { nrm *= 0.5; ++sq; }
+    const double sc = ldexp(1.0, -sq);
+#pragma unroll
+    for (int i = 0; i < NC * NC; ++i) { a.e[i].r *= sc; a.e[i].i *= sc; }
+    // Horner: I + a(1 + a/2(1 + a/3(...)))
+    Mat<NC> r = mid<NC>();
+    for (int k = 20; k >= 1; --k) {
+        Mat<NC> t = mmul<NC>(a, r);
+        const double f = 1.0 / k;
+        r = mid<NC>();
+#pragma unroll
+        for (int i = 0; i < NC * NC; ++i) { r.e[i].r += f * t.e[i].r; r.e[i].i += f * t.e[i].i; }
+    }
+    for (int k = 0; k < sq; ++k) r = mmul<NC>(r, r);
+    return r;
+}
+
+__device__ __forceinline__ void store_link3(double *links, int s, int mu, const Mat<3> &m)
+{
+    double *u = links + 18 * (mu + 3 * s);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            u[2 * (i + 3 * j)] = m.e[i * 3 + j].r;
+            u[2 * (i + 3 * j) + 1] = m.e[i * 3 + j].i;
+        }
+}
+
+// project onto SU(3): Gram-Schmidt of rows 0,1, row 2 = conj(row0 x row1)
+__device__ Mat<3> reunit3(Mat<3> m)
+{
+    double n0 = 0;
+    for (int j = 0; j < 3; ++j) n0 += m.e[j].r * m.e[j].r + m.e[j].i * m.e[j].i;
+    n0 = 1.0 / sqrt(n0);
+    for (int j = 0; j < 3; ++j) { m.e[j].r *= n0; m.e[j].i *= n0; }
+    cd d = {0, 0};
+    for (int j = 0; j < 3; ++j) d = cadd(d, cmulc(m.e[3 + j], m.e[j]));  // <r0, r1> = sum r1 conj(r0)
+    for (int j = 0; j < 3; ++j) {
+        cd t = cmul(d, m.e[j]);
+        m.e[3 + j].r -= t.r;
+        m.e[3 + j].i -= t.i;
+    }
+    double n1 = 0;
+    for (int j = 0; j < 3; ++j) n1 += m.e[3 + j].r * m.e[3 + j].r + m.e[3 + j].i * m.e[3 + j].i;
+    n1 = 1.0 / sqrt(n1);
+    for (int j = 0; j < 3; ++j) { m.e[3 + j].r *= n1; m.e[3 + j].i *= n1; }
+    // r2 = conj(r0 x r1)
+    for (int j = 0; j < 3; ++j) {
+        const int a = (j + 1) % 3, b = (j + 2) % 3;
+        cd c = cadd(cmul(m.e[a], m.e[3 + b]), {-cmul(m.e[b], m.e[3 + a]).r, -cmul(m.e[b], m.e[3 + a]).i});
+        m.e[6 + j] = {c.r, -c.i};
+    }
+    return m;
+}
+
+// stencil links from master links and U(1) phases (ConvertGaugeLinks,
+// OverlapEvolutionOperator.cpp:23-51, with the QED phase folded in)
+template <int NC>
+__global__ __launch_bounds__(TPB) void k_link_setup(const double *__restrict__ links, const double2 *__restrict__ phase,
+                                                    double2 *__restrict__ u64, float2 *__restrict__ u32, int V)
+{
+    const int i = blockIdx.x * TPB + threadIdx.x;
+    if (i >= 3 * V) return;
+    const int mu = i / V, s = i - mu * V;
+    Mat<NC> m = get_link<NC>(links, s, mu);
+    cd ph = {1, 0};
+    if (phase) ph = {phase[i].x, phase[i].y};
+#pragma unroll
+    for (int k = 0; k < NC * NC; ++k) {
+        cd v = cmul(ph, m.e[k]);
+        u64[(size_t)i * NC * NC + k] = make_double2(v.r, v.i);
+        u32[(size_t)i * NC * NC + k] = make_float2((float)v.r, (float)v.i);
+    }
+}
+
+// stencil links from a host-layout u_cc array (already in [mu*V+s][i][j] order)
+__global__ __launch_bounds__(TPB) void k_ucc_setup(const double2 *__restrict__ ucc, double2 *__restrict__ u64,
+                                                   float2 *__restrict__ u32, long n)
+{
+    const long i = (long)blockIdx.x * TPB + threadIdx.x;
+    if (i >= n) return;
+    const double2 v = ucc[i];
+    u64[i] = v;
+    u32[i] = make_float2((float)v.x, (float)v.y);
+}
+
+// Dynamics.cpp:97-138: U <- exp(i cU E) U with cU = -dt
+template <int NC>
+__global__ __launch_bounds__(TPB) void k_link_update(double *__restrict__ links, const double *__restrict__ e,
+                                                     double dt, int V, int reunit)
+{
+    const int i = blockIdx.x * TPB + threadIdx.x;
+    if (i >= 3 * V) return;
+    const int s = i / 3, mu = i - 3 * s;  // link (s, mu) at index mu + 3 s
+    constexpr int NG = NC * NC - 1;
+    double al[NG];
+#pragma unroll
+    for (int a = 0; a < NG; ++a) al[a] = e[NG * (mu + 3 * s) + a];
+    const double c = -dt;
+    if constexpr (NC == 2) {
+        // MatrixIExp (SU2/AlgebraOperations.cpp:37-55) and UU (SU2/GroupOperations.cpp:103-110)
+        const double d = sqrt((c * al[0]) * (c * al[0]) + (c * al[1]) * (c * al[1]) + (c * al[2]) * (c * al[2]));
+        double up[4];
+        if (d != 0) {
+            const double mm = c * sin(0.5 * d) / d;
+            up[0] = mm * al[0]; up[1] = mm * al[1]; up[2] = mm * al[2]; up[3] = cos(0.5 * d);
+        } else {
+            up[0] = up[1] = up[2] = 0; up[3] = 1;
+        }
+        double *u = links + 4 * (mu + 3 * s);
+        const double o0 = u[0], o1 = u[1], o2 = u[2], o3 = u[3];
+        double n0 = up[3] * o0 + up[0] * o3 - (up[1] * o2 - up[2] * o1);
+        double n1 = up[3] * o1 + up[1] * o3 + (up[0] * o2 - up[2] * o0);
+        double n2 = up[3] * o2 + up[2] * o3 - (up[0] * o1 - up[1] * o0);
+        double n3 = up[3] * o3 - up[0] * o0 - up[1] * o1 - up[2] * o2;
+        if (reunit) {  // MaxTraceProjection (SU2/GroupOperations.cpp:261-270)
+            const double nn = sqrt(n0 * n0 + n1 * n1 + n2 * n2 + n3 * n3);
+            n0 /= nn; n1 /= nn; n2 /= nn; n3 /= nn;
+        }
+        u[0] = n0; u[1] = n1; u[2] = n2; u[3] = n3;
+    } else {
+        Mat<3> ex = iexp<3>(c, al);
+        Mat<3> u = get_link<3>(links, s, mu);
+        Mat<3> r = mmul<3>(ex, u);
+        if (reunit) r = reunit3(r);
+        store_link3(links, s, mu, r);
+    }
+}
+
+__device__ __forceinline__ int nbr(int s, int dmu, int step, int n0, int n1, int n2)
+{
+    int z = s / (n0 * n1), rem = s - z * n0 * n1, y = rem / n0, x = rem - y * n0;
+    if (dmu == 0) { x = (x + step) % n0; if (x < 0) x += n0; }
+    else if (dmu == 1) { y = (y + step) % n1; if (y < 0) y += n1; }
+    else { z = (z + step) % n2; if (z < 0) z += n2; }
+    return x + n0 * (y + n1 * z);
+}
+
+// E_mu(x) += dt * (-2) sum_{nu != mu} [ReTr(i t^a U_{mu nu}(x)) + ReTr(i t^a U_{mu,-nu}(x))]
+// (Hamilton's equation of H = 1/2 E^2 + 2 sum ReTr(1 - U_plaq) for U' = -i E U; the
+// reference's plaquette macros used by Dynamics.cpp:145-197 are absent)
+template <int NC>
+__global__ __launch_bounds__(TPB) void k_efield_update(const double *__restrict__ links, double *__restrict__ e,
+                                                       double dt, int n0, int n1, int n2)
+{
+    const int V = n0 * n1 * n2;
+    const int i = blockIdx.x * TPB + threadIdx.x;
+    if (i >= 3 * V) return;
+    const int s = i / 3, mu = i - 3 * s;
+    constexpr int NG = NC * NC - 1;
+    double f[NG];
+#pragma unroll
+    for (int a = 0; a < NG; ++a) f[a] = 0;
+    const Mat<NC> umu = get_link<NC>(links, s, mu);
+    const int sp = nbr(s, mu, 1, n0, n1, n2);
+    for (int nu = 0; nu < 3; ++nu) {
+        if (nu == mu) continue;
+        // forward: U_mu(x) U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+
+        const int snu = nbr(s, nu, 1, n0, n1, n2);
+        Mat<NC> p = mmul<NC>(umu, get_link<NC>(links, sp, nu));
+        p = mmul<NC>(p, mdag<NC>(get_link<NC>(links, snu, mu)));
+        p = mmul<NC>(p, mdag<NC>(get_link<NC>(links, s, nu)));
+        double t[NG];
+        retr_it<NC>(p, t);
+#pragma unroll
+        for (int a = 0; a < NG; ++a) f[a] += t[a];
+        // backward: U_mu(x) U_nu(x+mu-nu)^+ U_mu(x-nu)^+ U_nu(x-nu)
+        const int smn = nbr(sp, nu, -1, n0, n1, n2);
+        const int sm = nbr(s, nu, -1, n0, n1, n2);
+        Mat<NC> q = mmul<NC>(umu, mdag<NC>(get_link<NC>(links, smn, nu)));
+        q = mmul<NC>(q, mdag<NC>(get_link<NC>(links, sm, mu)));
+        q = mmul<NC>(q, get_link<NC>(links, sm, nu));
+        retr_it<NC>(q, t);
+#pragma unroll
+        for (int a = 0; a < NG; ++a) f[a] += t[a];
+    }
+#pragma unroll
+    for (int a = 0; a < NG; ++a) e[NG * (mu + 3 * s) + a] += -2.0 * dt * f[a];
+}
+
+// ---------------------------------------------------------------------------------------------
+// deterministic reductions: fixed grid, per-block partials, one fixed-order final pass
+// ---------------------------------------------------------------------------------------------
+constexpr int RED_BLOCKS = 512;
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// block reduction of NV values (sum or max per value); result valid in thread 0
+template <int NV> __device__ void block_reduce(double *v, const int *is_max)
+{
+    __shared__ double sh[TPB / 64][NV];
+    const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] = is_max[k] ? wave_max(v[k]) : wave_sum(v[k]);
+    if (l == 0)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) sh[w][k] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            double r = sh[0][k];
+            for (int j = 1; j < TPB / 64; ++j) r = is_max[k] ? fmax(r, sh[j][k]) : r + sh[j][k];
+            v[k] = r;
+        }
+    }
+}
+
+template <int NV>
+__global__ __launch_bounds__(TPB) void k_finish(const double *__restrict__ partial, int nblocks, double *out, int m0,
+                                                int m1, int m2, int m3)
+{
+    const int is_max[4] = {m0, m1, m2, m3};
+    double v[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        double r = is_max[k] ? 0.0 : 0.0;
+        for (int b = threadIdx.x; b < nblocks; b += TPB) {
+            const double p = partial[(size_t)b * NV + k];
+            r = is_max[k] ? fmax(r, p) : r + p;
+        }
+        v[k] = r;
+    }
+    block_reduce<NV>(v, is_max);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) out[k] = v[k];
+}
+
+template <int NC> __device__ double plaq_retr_1mu(const double *links, int s, int mu, int nu, int n0, int n1, int n2)
+{
+    const int sp = nbr(s, mu, 1, n0, n1, n2), snu = nbr(s, nu, 1, n0, n1, n2);
+    Mat<NC> p = mmul<NC>(get_link<NC>(links, s, mu), get_link<NC>(links, sp, nu));
+    p = mmul<NC>(p, mdag<NC>(get_link<NC>(links, snu, mu)));
+    p = mmul<NC>(p, mdag<NC>(get_link<NC>(links, s, nu)));
+    double tr = 0;
+#pragma unroll
+    for (int i = 0; i < NC; ++i) tr += 1.0 - p.e[i * NC + i].r;
+    return tr;
+}
+
+// energy: (1/2) sum E^2 and (1/2) sum_planes 4 ReTr(1 - U_plaq)   (BulkObservables.cpp:29-31,84-86)
+template <int NC>
+__global__ __launch_bounds__(TPB) void k_energy(const double *__restrict__ links, const double *__restrict__ e,
+                                                double *partial, int n0, int n1, int n2)
+{
+    const int V = n0 * n1 * n2;
+    constexpr int NG = NC * NC - 1;
+    double v[2] = {0, 0};
+    for (int s = blockIdx.x * TPB + threadIdx.x; s < V; s += RED_BLOCKS * TPB) {
+        for (int k = 0; k < 3 * NG; ++k) { const double q = e[(size_t)3 * NG * s + k]; v[0] += q * q; }
+        v[1] += 4.0 * (plaq_retr_1mu<NC>(links, s, 1, 2, n0, n1, n2) + plaq_retr_1mu<NC>(links, s, 2, 0, n0, n1, n2) +
+                       plaq_retr_1mu<NC>(links, s, 0, 1, n0, n1, n2));
+    }
+    v[0] *= 0.5;
+    v[1] *= 0.5;
+    const int ism[2] = {0, 0};
+    block_reduce<2>(v, ism);
+    if (threadIdx.x == 0) { partial[2 * blockIdx.x] = v[0]; partial[2 * blockIdx.x + 1] = v[1]; }
+}
+
+// Gauss law G(x) = sum_mu E_mu(x) - U_mu(x-mu)^+ E_mu(x-mu) U_mu(x-mu): max |G^a| and sum G^2
+template <int NC>
+__global__ __launch_bounds__(TPB) void k_gauss(const double *__restrict__ links, const double *__restrict__ e,
+                                               double *partial, int n0, int n1, int n2)
+{
+    const int V = n0 * n1 * n2;
+    constexpr int NG = NC * NC - 1;
+    double v[2] = {0, 0};
+    for (int s = blockIdx.x * TPB + threadIdx.x; s < V; s += RED_BLOCKS * TPB) {
+        Mat<NC> G;
+#pragma unroll
+        for (int k = 0; k < NC * NC; ++k) G.e[k] = {0, 0};
+        for (int mu = 0; mu < 3; ++mu) {
+            const int sm = nbr(s, mu, -1, n0, n1, n2);
+            Mat<NC> e0 = alg_to_mat<NC>(e + NG * (mu + 3 * s));
+            Mat<NC> e1 = alg_to_mat<NC>(e + NG * (mu + 3 * sm));
+            Mat<NC> u = get_link<NC>(links, sm, mu);
+            Mat<NC> t = mmul<NC>(mmul<NC>(mdag<NC>(u), e1), u);
+#pragma unroll
+            for (int k = 0; k < NC * NC; ++k) G.e[k] = {G.e[k].r + e0.e[k].r - t.e[k].r, G.e[k].i + e0.e[k].i - t.e[k].i};
+        }
+        // components G^a = 2 tr(t^a G) = 2 ReTr(i t^a (-i G)) ; use ReTr(i t^a X) with X = -iG
+        Mat<NC> mg;
+#pragma unroll
+        for (int k = 0; k < NC * NC; ++k) mg.e[k] = {G.e[k].i, -G.e[k].r};
+        double comp[NG];
+        retr_it<NC>(mg, comp);
+#pragma unroll
+        for (int a = 0; a < NG; ++a) {
+            const double g = fabs(2.0 * comp[a]);
+            v[0] = fmax(v[0], g);
+            v[1] += g * g;
+        }
+    }
+    const int ism[2] = {1, 0};
+    block_reduce<2>(v, ism);
+    if (threadIdx.x == 0) { partial[2 * blockIdx.x] = v[0]; partial[2 * blockIdx.x + 1] = v[1]; }
+}
+
+template <int NC>
+__global__ __launch_bounds__(TPB) void k_unitarity(const double *__restrict__ links, double *partial, int V)
+{
+    double v[2] = {0, 0};
+    for (int i = blockIdx.x * TPB + threadIdx.x; i < 3 * V; i += RED_BLOCKS * TPB) {
+        const int s = i / 3, mu = i - 3 * s;
+        double nrm;
+        if constexpr (NC == 2) {  // UnitarityNorm (SU2/GroupOperations.cpp:172-184)
+            const double *u = links + 4 * (mu + 3 * s);
+            const double c3 = u[3] * u[3] + u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+            nrm = fabs(c3 - 1.0);
+        } else {
+            Mat<3> u = get_link<3>(links, s, mu);
+            Mat<3> p = mmul<3>(u, mdag<3>(u));
+            nrm = 0;
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) {
+                    const double re = p.e[a * 3 + b].r - (a == b ? 1.0 : 0.0), im = p.e[a * 3 + b].i;
+                    nrm += re * re + im * im;
+                }
+            nrm = sqrt(nrm);
+        }
+        v[0] = fmax(v[0], nrm);
+        v[1] += nrm * nrm;
+    }
+    const int ism[2] = {1, 0};
+    block_reduce<2>(v, ism);
+    if (threadIdx.x == 0) { partial[2 * blockIdx.x] = v[0]; partial[2 * blockIdx.x + 1] = v[1]; }
+}
+
+// improved sum_x E.B (ChernSimonsOperators.cpp:228-439)
+template <int NC> __device__ __forceinline__ Mat<NC> L_(const double *links, int s, int mu) { return get_link<NC>(links, s, mu); }
+
+template <int NC> __device__ __forceinline__ Mat<NC> transport(const Mat<NC> &u, const Mat<NC> &q)
+{
+    return mmul<NC>(mmul<NC>(u, q), mdag<NC>(u));
+}
+
+template <int NC>
+__device__ __forceinline__ void acc(Mat<NC> &a, const Mat<NC> &b, double w)
+{
+#pragma unroll
+    for (int k = 0; k < NC * NC; ++k) { a.e[k].r += w * b.e[k].r; a.e[k].i += w * b.e[k].i; }
+}
+
+template <int NC>
+__device__ double edotb_site(const double *links, const double *e, int s, int n0, int n1, int n2)
+{
+    constexpr int NG = NC * NC - 1;
+    double total = 0;
+    for (int mu = 0; mu < 3; ++mu) {
+        const int a = (mu + 1) % 3, b = (mu + 2) % 3;
+        // site at (i along a, j along b)
+        auto at = [&](int i, int j) {
+            int t = s;
+            if (i) t = nbr(t, a, i, n0, n1, n2);
+            if (j) t = nbr(t, b, j, n0, n1, n2);
+            return t;
+        };
+        auto atm = [&](int k) { return nbr(s, mu, k, n0, n1, n2); };
+        // E_mu transported average
+        Mat<NC> ea;
+#pragma unroll
+        for (int k = 0; k < NC * NC; ++k) ea.e[k] = {0, 0};
+        acc<NC>(ea, alg_to_mat<NC>(e + NG * (mu + 3 * s)), 7.0 / 12.0);
+        acc<NC>(ea, transport<NC>(L_<NC>(links, s, mu), alg_to_mat<NC>(e + NG * (mu + 3 * atm(1)))), -1.0 / 12.0);
+        const Mat<NC> um = mdag<NC>(L_<NC>(links, atm(-1), mu));
+        acc<NC>(ea, transport<NC>(um, alg_to_mat<NC>(e + NG * (mu + 3 * atm(-1)))), 7.0 / 12.0);
+        const Mat<NC> um2 = mmul<NC>(um, mdag<NC>(L_<NC>(links, atm(-2), mu)));
+        acc<NC>(ea, transport<NC>(um2, alg_to_mat<NC>(e + NG * (mu + 3 * atm(-2)))), -1.0 / 12.0);
+        // E^a = 2 tr(t^a E) = 2 ReTr(i t^a (-i E))
+        Mat<NC> mie;
+#pragma unroll
+        for (int k = 0; k < NC * NC; ++k) mie.e[k] = {ea.e[k].i, -ea.e[k].r};
+        double Ea[NG];
+        retr_it<NC>(mie, Ea);
+
+        // B from the (a, b) plane
+        const Mat<NC> uaM = mdag<NC>(L_<NC>(links, at(-1, 0), a));
+        const Mat<NC> ubM = mdag<NC>(L_<NC>(links, at(0, -1), b));
+        const Mat<NC> ua2M = mmul<NC>(uaM, mdag<NC>(L_<NC>(links, at(-2, 0), a)));
+        const Mat<NC> ub2M = mmul<NC>(ubM, mdag<NC>(L_<NC>(links, at(0, -2), b)));
+        Mat<NC> ub;
+#pragma unroll
+        for (int k = 0; k < NC * NC; ++k) ub.e[k] = {0, 0};
+        auto box = [&](int i, int j, int dag12) {  // UUDD from (i,j) or DDUU
+            (void)dag12;
+            return mmul<NC>(mmul<NC>(L_<NC>(links, at(i, j), a), L_<NC>(links, at(i + 1, j), b)),
+                            mmul<NC>(mdag<NC>(L_<NC>(links, at(i, j + 1), a)), mdag<NC>(L_<NC>(links, at(i, j), b))));
+        };
+        acc<NC>(ub, box(0, 0, 0), 5.0 / 12.0);
+        acc<NC>(ub, transport<NC>(uaM, box(-1, 0, 0)), 5.0 / 12.0);
+        {  // lower left: U_a(-1,0)^+ U_b(-1,-1)^+ U_a(-1,-1) U_b(0,-1)
+            Mat<NC> p = mmul<NC>(mdag<NC>(L_<NC>(links, at(-1, 0), a)), mdag<NC>(L_<NC>(links, at(-1, -1), b)));
+            p = mmul<NC>(p, mmul<NC>(L_<NC>(links, at(-1, -1), a), L_<NC>(links, at(0, -1), b)));
+            acc<NC>(ub, p, 5.0 / 12.0);
+        }
+        acc<NC>(ub, transport<NC>(ubM, box(0, -1, 0)), 5.0 / 12.0);
+        // rectangles: six links (offset i, offset j, direction b?, dagger?)
+        struct R6 { signed char o[6][2]; unsigned char d[6]; unsigned char g[6]; unsigned char t; };
+        const R6 R[8] = {
+            {{{0, 0}, {1, 0}, {2, 0}, {1, 1}, {0, 1}, {0, 0}}, {0, 0, 1, 0, 0, 1}, {0, 0, 0, 1, 1, 1}, 0},
+            {{{0, 0}, {1, 0}, {1, 1}, {0, 2}, {0, 1}, {0, 0}}, {0, 1, 1, 0, 1, 1}, {0, 0, 0, 1, 1, 1}, 0},
+            {{{-1, 0}, {0, 0}, {0, 1}, {-1, 2}, {-1, 1}, {-1, 0}}, {0, 1, 1, 0, 1, 1}, {0, 0, 0, 1, 1, 1}, 1},
+            {{{-2, 0}, {-1, 0}, {0, 0}, {-1, 1}, {-2, 1}, {-2, 0}}, {0, 0, 1, 0, 0, 1}, {0, 0, 0, 1, 1, 1}, 2},
+            {{{-1, 0}, {-1, -1}, {-1, -2}, {-1, -2}, {0, -2}, {0, -1}}, {0, 1, 1, 0, 1, 1}, {1, 1, 1, 0, 0, 0}, 0},
+            {{{-1, 0}, {-2, 0}, {-2, -1}, {-2, -1}, {-1, -1}, {0, -1}}, {0, 0, 1, 0, 0, 1}, {1, 1, 1, 0, 0, 0}, 0},
+            {{{0, -1}, {1, -1}, {2, -1}, {1, 0}, {0, 0}, {0, -1}}, {0, 0, 1, 0, 0, 1}, {0, 0, 0, 1, 1, 1}, 3},
+            {{{0, -2}, {1, -2}, {1, -1}, {0, 0}, {0, -1}, {0, -2}}, {0, 1, 1, 0, 1, 1}, {0, 0, 0, 1, 1, 1}, 4},
+        };
+        for (int r = 0; r < 8; ++r) {
+            Mat<NC> p = mid<NC>();
+            for (int k = 0; k < 6; ++k) {
+                Mat<NC> l = L_<NC>(links, at(R[r].o[k][0], R[r].o[k][1]), R[r].d[k] ? b : a);
+                p = mmul<NC>(p, R[r].g[k] ? mdag<NC>(l) : l);
+            }
+            if (R[r].t == 1) p = transport<NC>(uaM, p);
+            if (R[r].t == 2) p = transport<NC>(ua2M, p);
+            if (R[r].t == 3) p = transport<NC>(ubM, p);
+            if (R[r].t == 4) p = transport<NC>(ub2M, p);
+            acc<NC>(ub, p, -1.0 / 24.0);
+        }
+        double Bb[NG];
+        retr_it<NC>(ub, Bb);
+#pragma unroll
+        for (int c = 0; c < NG; ++c) total += 2.0 * Ea[c] * 2.0 * Bb[c];
+    }
+    return total;
+}
+
+template <int NC>
+__global__ __launch_bounds__(TPB) void k_edotb(const double *__restrict__ links, const double *__restrict__ e,
+                                               double *partial, int n0, int n1, int n2)
+{
+    const int V = n0 * n1 * n2;
+    double v[1] = {0};
+    for (int s = blockIdx.x * TPB + threadIdx.x; s < V; s += RED_BLOCKS * TPB)
+        v[0] += edotb_site<NC>(links, e, s, n0, n1, n2);
+    const int ism[1] = {0};
+    block_reduce<1>(v, ism);
+    if (threadIdx.x == 0) partial[blockIdx.x] = v[0];
+}
+
+// per-mode inner products: partial[chunk][mode] then a fixed-order sum over chunks
+constexpr int DOT_CHUNKS = 128;
+
+template <int NC, int MB>
+__global__ __launch_bounds__(TPB) void k_mode_dot(const double2 *__restrict__ a, const double2 *__restrict__ b,
+                                                  double2 *__restrict__ partial, int V, int nblk)
+{
+    // grid: (DOT_CHUNKS, nblk); each thread: one mode (m) and a strided subset of (site, comp)
+    constexpr int NCOMP = 4 * NC;
+    constexpr int LPM = TPB / MB;  // lanes per mode inside the block
+    const int m = threadIdx.x % MB, lane = threadIdx.x / MB;
+    const int blk = blockIdx.y;
+    const long nel = (long)V * NCOMP;
+    double sr = 0, si = 0;
+    for (long q = (long)blockIdx.x * LPM + lane; q < nel; q += (long)DOT_CHUNKS * LPM) {
+        const size_t idx = ((size_t)blk * nel + q) * MB + m;
+        const double2 x = a[idx], y = b[idx];
+        sr += x.x * y.x + x.y * y.y;
+        si += x.x * y.y - x.y * y.x;
+    }
+    __shared__ double2 sh[TPB];
+    sh[threadIdx.x] = make_double2(sr, si);
+    __syncthreads();
+    if (lane == 0) {
+        double rr = 0, ii = 0;
+        for (int l = 0; l < LPM; ++l) { rr += sh[l * MB + m].x; ii += sh[l * MB + m].y; }
+        partial[((size_t)blockIdx.x) * nblk * MB + (size_t)blk * MB + m] = make_double2(rr, ii);
+    }
+}
+
+__global__ __launch_bounds__(TPB) void k_mode_dot_finish(const double2 *__restrict__ partial, double2 *__restrict__ out,
+                                                         int nmodes_pad)
+{
+    const int i = blockIdx.x * TPB + threadIdx.x;
+    if (i >= nmodes_pad) return;
+    double r = 0, im = 0;
+    for (int c = 0; c < DOT_CHUNKS; ++c) { r += partial[(size_t)c * nmodes_pad + i].x; im += partial[(size_t)c * nmodes_pad + i].y; }
+    out[i] = make_double2(r, im);
+}
+
+// currents (OverlapCurrents.cpp:137-228), per (site, mode) bilinears summed over the modes of a block.
+// gam: [kind][mu][a][b] complex, kind 0 = g0 g^mu, 1 = g0 g^mu g5, 2 = -(i/2) g0 g^mu g0,
+// 3 = -(i/2) g0 g^mu g5 g0 (built on the host from the Dirac matrices as the reference does)
+template <int NC, int MB>
+__global__ __launch_bounds__(TPB) void k_currents(const double2 *__restrict__ psi, const double2 *__restrict__ hpsi,
+                                                  const double2 *__restrict__ gam, double *__restrict__ partial, int V,
+                                                  int nblk, int first_mode)
+{
+    constexpr int SPB = TPB / MB;
+    const int m = threadIdx.x % MB, sl = threadIdx.x / MB;
+    const long site_g = (long)blockIdx.x * SPB + sl;  // over nblk * V
+    const bool ok = site_g < (long)nblk * V;
+    const int blk = ok ? (int)(site_g / V) : 0, s = ok ? (int)(site_g % V) : 0;
+    double j[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (ok) {
+        const double w = (((blk * MB + m) + first_mode) % 2 == 0) ? 1.0 : -1.0;
+        for (int c = 0; c < NC; ++c) {
+            double2 p[4], h[4];
+            for (int sp = 0; sp < 4; ++sp) {
+                const size_t idx = ((size_t)(blk * V + s) * 4 * NC + sp * NC + c) * MB + m;
+                p[sp] = psi[idx];
+                h[sp] = hpsi[idx];
+            }
+            for (int mu = 0; mu < 4; ++mu)
+                for (int a = 0; a < 4; ++a)
+                    for (int b = 0; b < 4; ++b) {
+                        // conj(p_a) p_b and conj(p_a) h_b
+                        const double ppr = p[a].x * p[b].x + p[a].y * p[b].y, ppi = p[a].x * p[b].y - p[a].y * p[b].x;
+                        const double phr = p[a].x * h[b].x + p[a].y * h[b].y, phi = p[a].x * h[b].y - p[a].y * h[b].x;
+                        const double2 gv = gam[((0 * 4 + mu) * 4 + a) * 4 + b], ga = gam[((1 * 4 + mu) * 4 + a) * 4 + b];
+                        const double2 gv2 = gam[((2 * 4 + mu) * 4 + a) * 4 + b], ga2 = gam[((3 * 4 + mu) * 4 + a) * 4 + b];
+                        j[mu] += w * (gv.x * ppr - gv.y * ppi + gv2.x * phr - gv2.y * phi);
+                        j[4 + mu] += w * (ga.x * ppr - ga.y * ppi + ga2.x * phr - ga2.y * phi);
+                    }
+        }
+    }
+    // sum over the MB modes of this site (lanes of one site are contiguous), fixed order
+    __shared__ double sh[TPB][8];
+    for (int k = 0; k < 8; ++k) sh[threadIdx.x][k] = j[k];
+    __syncthreads();
+    if (m == 0 && ok) {
+        double t[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int q = 0; q < MB; ++q)
+            for (int k = 0; k < 8; ++k) t[k] += sh[sl * MB + q][k];
+        for (int k = 0; k < 8; ++k) partial[((size_t)blk * V + s) * 8 + k] = t[k];
+    }
+}
+
+__global__ __launch_bounds__(TPB) void k_currents_finish(const double *__restrict__ partial, double *jv, double *ja,
+                                                         int V, int nblk, double pre)
+{
+    const int s = blockIdx.x * TPB + threadIdx.x;
+    if (s >= V) return;
+    double t[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int b = 0; b < nblk; ++b)
+        for (int k = 0; k < 8; ++k) t[k] += partial[((size_t)b * V + s) * 8 + k];
+    for (int k = 0; k < 4; ++k) { jv[4 * s + k] = pre * t[k]; ja[4 * s + k] = pre * t[4 + k]; }
+}
+
+// multi-shift CG vector update (OverlapU.cpp:164-177), with chi += bz_k beta_k p_k accumulated
+// directly:  for each shift k: chi += cw_k p_k ; p_k = a_k p_k + g_k r   (per-mode coefficients)
+__global__ __launch_bounds__(TPB) void k_mscg_update2(int nshift, double2 *__restrict__ ps, double2 *__restrict__ chi,
+                                                      const double2 *__restrict__ r, const double *__restrict__ coef,
+                                                      const int *__restrict__ active, long per_blk, int nblk, int mb)
+{
+    // element i = (blk, q, m) with per_blk = V*NCOMP elements per mode block
+    const long i = (long)blockIdx.x * TPB + threadIdx.x;
+    const long nel = per_blk * nblk * mb;
+    if (i >= nel) return;
+    const int m = (int)(i % mb);
+    const int blk = (int)(i / ((long)per_blk * mb));
+    const int mode = blk * mb + m;
+    if (!active[mode]) return;
+    const double *c = coef + (size_t)mode * 3 * nshift;
+    const double2 rv = r[i];
+    double2 x = chi[i];
+    for (int k = 0; k < nshift; ++k) {
+        double2 p = ps[(size_t)k * nel + i];
+        x.x += c[3 * k] * p.x;
+        x.y += c[3 * k] * p.y;
+        p.x = c[3 * k + 1] * p.x + c[3 * k + 2] * rv.x;
+        p.y = c[3 * k + 1] * p.y + c[3 * k + 2] * rv.y;
+        ps[(size_t)k * nel + i] = p;
+    }
+    chi[i] = x;
+}
+
+// y += sign * coef[mode*stride] * x for active modes (FP64 fields)
+__global__ __launch_bounds__(TPB) void k_axpy_modes(double2 *__restrict__ y, const double2 *__restrict__ x,
+                                                    const double *__restrict__ coef, int stride, double sign,
+                                                    const int *__restrict__ active, long per_blk, int nblk, int mb)
+{
+    const long i = (long)blockIdx.x * TPB + threadIdx.x;
+    const long nel = per_blk * nblk * mb;
+    if (i >= nel) return;
+    const int m = (int)(i % mb);
+    const int blk = (int)(i / ((long)per_blk * mb));
+    const int mode = blk * mb + m;
+    if (active && !active[mode]) return;
+    const double c = sign * coef[(size_t)mode * stride];
+    const double2 xv = x[i];
+    double2 yv = y[i];
+    yv.x += c * xv.x;
+    yv.y += c * xv.y;
+    y[i] = yv;
+}
+
+inline unsigned nblocks(long n) { return (unsigned)((n + TPB - 1) / TPB); }
+
+template <int NC, typename TI, typename TO>
+hipError_t hop_mb(const Geom &g, const HopArgs &a, hipStream_t st)
+{
+    const long total = (long)a.nblk * g.V * g.mb;
+    const unsigned nb = nblocks(total);
+    switch (g.mb) {
+    case 16: k_hop<NC, TI, TO, 16><<<nb, TPB, 0, st>>>(a, g.n0, g.n1, g.n2); break;
+    case 32: k_hop<NC, TI, TO, 32><<<nb, TPB, 0, st>>>(a, g.n0, g.n1, g.n2); break;
+    case 64: k_hop<NC, TI, TO, 64><<<nb, TPB, 0, st>>>(a, g.n0, g.n1, g.n2); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------------------------
+hipError_t launch_hop(const Geom &g, int pin, int pout, const HopArgs &a, hipStream_t st)
+{
+    if (g.nc == 2) {
+        if (pin == P64 && pout == P64) return hop_mb<2, double, double>(g, a, st);
+        if (pin == P32 && pout == P32) return hop_mb<2, float, float>(g, a, st);
+        if (pin == P32 && pout == P64) return hop_mb<2, float, double>(g, a, st);
+    } else if (g.nc == 3) {
+        if (pin == P64 && pout == P64) return hop_mb<3, double, double>(g, a, st);
+        if (pin == P32 && pout == P32) return hop_mb<3, float, float>(g, a, st);
+        if (pin == P32 && pout == P64) return hop_mb<3, float, double>(g, a, st);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_convert(const Geom &g, int nblk, const void *src, int psrc, void *dst, int pdst, hipStream_t st)
+{
+    const long npairs = (long)nblk * g.V * (4 * g.nc / 2) * g.mb;
+    if (psrc == P64 && pdst == P32)
+        k_d2f<<<nblocks(npairs), TPB, 0, st>>>((const double2 *)src, (float4 *)dst, npairs, g.mb);
+    else if (psrc == P32 && pdst == P64)
+        k_f2d<<<nblocks(npairs), TPB, 0, st>>>((const float4 *)src, (double2 *)dst, npairs, g.mb);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_lincomb(const Geom &g, int nblk, int p, void *out, const void *a, double car, double cai,
+                          const void *b, double cbr, double cbi, hipStream_t st)
+{
+    const long ncplx = (long)nblk * g.V * 4 * g.nc * g.mb;
+    if (p == P64)
+        k_lincomb<double><<<nblocks(ncplx), TPB, 0, st>>>((double *)out, (const double *)a, car, cai, (const double *)b,
+                                                          cbr, cbi, ncplx);
+    else
+        k_lincomb<float><<<nblocks(ncplx), TPB, 0, st>>>((float *)out, (const float *)a, car, cai, (const float *)b, cbr,
+                                                         cbi, ncplx);
+    return hipGetLastError();
+}
+
+hipError_t launch_aos_to_soa(const Geom &g, const double *aos, int first, int count, void *field, hipStream_t st)
+{
+    const long tot = (long)count * g.V * 4 * g.nc;
+    k_aos_to_soa<<<nblocks(tot), TPB, 0, st>>>((const double2 *)aos, (double2 *)field, first, count, g.V, 4 * g.nc, g.mb);
+    return hipGetLastError();
+}
+
+hipError_t launch_soa_to_aos(const Geom &g, const void *field, int first, int count, double *aos, hipStream_t st)
+{
+    const long tot = (long)count * g.V * 4 * g.nc;
+    k_soa_to_aos<<<nblocks(tot), TPB, 0, st>>>((const double2 *)field, (double2 *)aos, first, count, g.V, 4 * g.nc, g.mb);
+    return hipGetLastError();
+}
+
+hipError_t launch_link_setup(const Geom &g, const double *links, const double *phase, double *ul64, float *ul32,
+                             hipStream_t st)
+{
+    const unsigned nb = nblocks(3L * g.V);
+    if (g.nc == 2)
+        k_link_setup<2><<<nb, TPB, 0, st>>>(links, (const double2 *)phase, (double2 *)ul64, (float2 *)ul32, g.V);
+    else
+        k_link_setup<3><<<nb, TPB, 0, st>>>(links, (const double2 *)phase, (double2 *)ul64, (float2 *)ul32, g.V);
+    return hipGetLastError();
+}
+
+hipError_t launch_ucc_to_links(const Geom &g, const double *ucc, double *ul64, float *ul32, hipStream_t st)
+{
+    const long n = 3L * g.V * g.nc * g.nc;
+    k_ucc_setup<<<nblocks(n), TPB, 0, st>>>((const double2 *)ucc, (double2 *)ul64, (float2 *)ul32, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_link_update(const Geom &g, double *links, const double *efield, double dt, int reunit,
+                              hipStream_t st)
+{
+    const unsigned nb = nblocks(3L * g.V);
+    if (g.nc == 2) k_link_update<2><<<nb, TPB, 0, st>>>(links, efield, dt, g.V, reunit);
+    else k_link_update<3><<<nb, TPB, 0, st>>>(links, efield, dt, g.V, reunit);
+    return hipGetLastError();
+}
+
+hipError_t launch_efield_update(const Geom &g, const double *links, double *efield, double dt, hipStream_t st)
+{
+    const unsigned nb = nblocks(3L * g.V);
+    if (g.nc == 2) k_efield_update<2><<<nb, TPB, 0, st>>>(links, efield, dt, g.n0, g.n1, g.n2);
+    else k_efield_update<3><<<nb, TPB, 0, st>>>(links, efield, dt, g.n0, g.n1, g.n2);
+    return hipGetLastError();
+}
+
+size_t mode_dot_partial_len(const Geom &g) { return (size_t)DOT_CHUNKS * g.nblk * g.mb * 2; }
+
+hipError_t launch_mode_dot(const Geom &g, const void *a, const void *b, double *partial, double *out, hipStream_t st)
+{
+    dim3 grid(DOT_CHUNKS, g.nblk);
+    const auto A = (const double2 *)a;
+    const auto B = (const double2 *)b;
+    const auto P = (double2 *)partial;
+#define DOT(NC, MB) k_mode_dot<NC, MB><<<grid, TPB, 0, st>>>(A, B, P, g.V, g.nblk)
+    if (g.nc == 2) {
+        if (g.mb == 16) DOT(2, 16); else if (g.mb == 32) DOT(2, 32); else DOT(2, 64);
+    } else {
+        if (g.mb == 16) DOT(3, 16); else if (g.mb == 32) DOT(3, 32); else DOT(3, 64);
+    }
+#undef DOT
+    const int np = g.nblk * g.mb;
+    k_mode_dot_finish<<<nblocks(np), TPB, 0, st>>>(P, (double2 *)out, np);
+    return hipGetLastError();
+}
+
+size_t reduce_partial_len(const Geom &) { return (size_t)RED_BLOCKS * 4; }
+
+hipError_t launch_energy(const Geom &g, const double *links, const double *efield, double *partial, double *out,
+                         hipStream_t st)
+{
+    if (g.nc == 2) k_energy<2><<<RED_BLOCKS, TPB, 0, st>>>(links, efield, partial, g.n0, g.n1, g.n2);
+    else k_energy<3><<<RED_BLOCKS, TPB, 0, st>>>(links, efield, partial, g.n0, g.n1, g.n2);
+    k_finish<2><<<1, TPB, 0, st>>>(partial, RED_BLOCKS, out, 0, 0, 0, 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_gauss(const Geom &g, const double *links, const double *efield, double *partial, double *out,
+                        hipStream_t st)
+{
+    if (g.nc == 2) k_gauss<2><<<RED_BLOCKS, TPB, 0, st>>>(links, efield, partial, g.n0, g.n1, g.n2);
+    else k_gauss<3><<<RED_BLOCKS, TPB, 0, st>>>(links, efield, partial, g.n0, g.n1, g.n2);
+    k_finish<2><<<1, TPB, 0, st>>>(partial, RED_BLOCKS, out, 1, 0, 0, 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_unitarity(const Geom &g, const double *links, double *partial, double *out, hipStream_t st)
+{
+    if (g.nc == 2) k_unitarity<2><<<RED_BLOCKS, TPB, 0, st>>>(links, partial, g.V);
+    else k_unitarity<3><<<RED_BLOCKS, TPB, 0, st>>>(links, partial, g.V);
+    k_finish<2><<<1, TPB, 0, st>>>(partial, RED_BLOCKS, out, 1, 0, 0, 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_edotb(const Geom &g, const double *links, const double *efield, double *partial, double *out,
+                        hipStream_t st)
+{
+    if (g.nc == 2) k_edotb<2><<<RED_BLOCKS, TPB, 0, st>>>(links, efield, partial, g.n0, g.n1, g.n2);
+    else k_edotb<3><<<RED_BLOCKS, TPB, 0, st>>>(links, efield, partial, g.n0, g.n1, g.n2);
+    k_finish<1><<<1, TPB, 0, st>>>(partial, RED_BLOCKS, out, 0, 0, 0, 0);
+    return hipGetLastError();
+}
+
+size_t currents_partial_len(const Geom &g) { return (size_t)g.nblk * g.V * 8; }
+
+hipError_t launch_currents(const Geom &g, const void *psi, const void *hpsi, const double *gam, int first_mode,
+                           double *partial, double *jv, double *ja, hipStream_t st)
+{
+    const long nsites = (long)g.nblk * g.V;
+    const auto P = (const double2 *)psi;
+    const auto H = (const double2 *)hpsi;
+#define CUR(NC, MB) k_currents<NC, MB><<<(unsigned)((nsites + TPB / MB - 1) / (TPB / MB)), TPB, 0, st>>>(P, H, (const double2 *)gam, partial, g.V, g.nblk, first_mode)
+    if (g.nc == 2) {
+        if (g.mb == 16) CUR(2, 16); else if (g.mb == 32) CUR(2, 32); else CUR(2, 64);
+    } else {
+        if (g.mb == 16) CUR(3, 16); else if (g.mb == 32) CUR(3, 32); else CUR(3, 64);
+    }
+#undef CUR
+    k_currents_finish<<<nblocks(g.V), TPB, 0, st>>>(partial, jv, ja, g.V, g.nblk, 1.0 / (2.0 * g.V));
+    return hipGetLastError();
+}
+
+hipError_t launch_mscg_update(const Geom &g, int nshift, double *ps, double *chi, const double *r, const double *coef,
+                              const int *active, hipStream_t st)
+{
+    const long per_blk = (long)g.V * 4 * g.nc;
+    const long nel = per_blk * g.nblk * g.mb;
+    k_mscg_update2<<<nblocks(nel), TPB, 0, st>>>(nshift, (double2 *)ps, (double2 *)chi, (const double2 *)r, coef, active,
+                                                 per_blk, g.nblk, g.mb);
+    return hipGetLastError();
+}
+
+hipError_t launch_axpy_modes(const Geom &g, void *y, const void *x, const double *coef, int coef_stride, double sign,
+                             const int *active, hipStream_t st)
+{
+    const long per_blk = (long)g.V * 4 * g.nc;
+    const long nel = per_blk * g.nblk * g.mb;
+    k_axpy_modes<<<nblocks(nel), TPB, 0, st>>>((double2 *)y, (const double2 *)x, coef, coef_stride, sign, active, per_blk,
+                                               g.nblk, g.mb);
+    return hipGetLastError();
+}
+
+}  // namespace ovl

@@ -1,0 +1,268 @@
+"""ctypes binding of the C ABI in include/ovl.h (libovl.so, built for gfx950).
+
+There is no CPU fallback: if libovl.so is missing or no HIP device is present, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libovl.so")
+
+OK, E_ARG, E_HIP, E_STATE, E_CONVERGE, E_NOMEM, E_SPECTRUM = 0, -1, -2, -3, -4, -5, -6
+PSI, PSIMID, SCRATCH = 0, 1, 2
+SIGN_CHEBYSHEV, SIGN_ZOLOTAREV = 0, 1
+PREC_FP64, PREC_FP32, PREC_MIXED = 0, 1, 2
+WILSON, OVERLAP = 0, 1
+GAUGE_FROZEN, GAUGE_LINKS, GAUGE_FULL = 0, 1, 2
+
+# exported symbols, in the order of include/ovl.h
+SYMBOLS = [
+    "ovl_create", "ovl_destroy", "ovl_error", "ovl_device_name", "ovl_sign_defaults",
+    "ovl_set_links", "ovl_get_links", "ovl_set_efield", "ovl_get_efield", "ovl_set_u1_flux",
+    "ovl_set_modes", "ovl_get_modes", "ovl_copy_field",
+    "ovl_apply_wilson", "ovl_apply_overlap", "ovl_apply_sign", "ovl_sign_residual", "ovl_ritz",
+    "ovl_wilson_evo_compute", "ovl_overlap_operator_compute", "ovl_zolotarev_coefficients",
+    "ovl_update_links", "ovl_update_efield", "ovl_fermion_update",
+    "ovl_energy", "ovl_gauss_violation", "ovl_unitarity_violation", "ovl_ncs_dot", "ovl_currents",
+    "ovl_mode_norms", "ovl_get_timing", "ovl_set_profiling",
+]
+
+
+class Params(C.Structure):
+    _fields_ = [("nc", C.c_int), ("n", C.c_int * 3), ("n_modes", C.c_int), ("r_wilson", C.c_double),
+                ("m_fermion", C.c_double), ("m_dw", C.c_double), ("mode_block", C.c_int),
+                ("reunitarize", C.c_int)]
+
+
+class SignOpts(C.Structure):
+    _fields_ = [("method", C.c_int), ("precision", C.c_int), ("degree", C.c_int), ("cut", C.c_double),
+                ("tol", C.c_double), ("head", C.c_int), ("lmin", C.c_double), ("lmax", C.c_double),
+                ("mode_chunk", C.c_int)]
+
+
+class StepOpts(C.Structure):
+    _fields_ = [("kind", C.c_int), ("dt", C.c_double), ("gauge", C.c_int), ("sign", SignOpts),
+                ("use_graph", C.c_int)]
+
+
+class Timing(C.Structure):
+    _fields_ = [("total_ms", C.c_double), ("hop_ms", C.c_double), ("hop_launches", C.c_longlong),
+                ("hop_bytes", C.c_double), ("fp64_steps", C.c_int), ("fp32_steps", C.c_int),
+                ("var_ms", C.c_double * 3), ("var_launches", C.c_longlong * 3), ("var_bytes", C.c_double * 3)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+        _lib = C.CDLL(LIB_PATH)
+        _lib.ovl_error.restype = C.c_char_p
+        _lib.ovl_zolotarev_coefficients.restype = C.c_double
+        _lib.ovl_create.argtypes = [C.POINTER(Params), C.POINTER(C.c_void_p)]
+        for name in SYMBOLS:
+            f = getattr(_lib, name)
+            if name not in ("ovl_error", "ovl_zolotarev_coefficients", "ovl_sign_defaults", "ovl_create"):
+                f.restype = C.c_int
+        _lib.ovl_sign_defaults.restype = None
+    return _lib
+
+
+class OvlError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"ovl error {code}: {msg}")
+        self.code = code
+
+
+def _d(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def sign_opts(**kw) -> SignOpts:
+    o = SignOpts()
+    lib().ovl_sign_defaults(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def zolotarev_coefficients(amin, amax, errtol=1e-12, no=25):
+    bz, dz = np.zeros(no), np.zeros(no)
+    eps = lib().ovl_zolotarev_coefficients(C.c_double(amin), C.c_double(amax), C.c_double(errtol), C.c_int(no),
+                                           _d(bz), _d(dz))
+    return eps, bz, dz
+
+
+def device_name() -> str:
+    buf = C.create_string_buffer(256)
+    rc = lib().ovl_device_name(buf, 256)
+    if rc != OK:
+        raise OvlError(rc, "no HIP device")
+    return buf.value.decode()
+
+
+class Context:
+    """One device context: gauge field, two mode-function fields (Psi, PsiMid) and scratch."""
+
+    def __init__(self, nc, n, n_modes, r_wilson=1.0, m_fermion=0.0, m_dw=0.0, mode_block=0, reunitarize=0):
+        self.nc, self.n, self.n_modes = nc, tuple(n), n_modes
+        self.V = int(np.prod(n))
+        p = Params(nc, (C.c_int * 3)(*n), n_modes, r_wilson, m_fermion, m_dw, mode_block, reunitarize)
+        h = C.c_void_p()
+        rc = lib().ovl_create(C.byref(p), C.byref(h))
+        if rc != OK:
+            raise OvlError(rc, "ovl_create failed")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().ovl_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _chk(self, rc, allow=()):
+        if rc != OK and rc not in allow:
+            raise OvlError(rc, lib().ovl_error(self.h).decode())
+        return rc
+
+    # gauge
+    def set_links(self, links):
+        self._chk(lib().ovl_set_links(self.h, _d(np.ascontiguousarray(links, np.float64))))
+
+    def get_links(self):
+        out = np.zeros(self.V * 3 * (4 if self.nc == 2 else 18))
+        self._chk(lib().ovl_get_links(self.h, _d(out)))
+        return out
+
+    def set_efield(self, e):
+        self._chk(lib().ovl_set_efield(self.h, _d(np.ascontiguousarray(e, np.float64))))
+
+    def get_efield(self):
+        out = np.zeros(self.V * 3 * (self.nc * self.nc - 1))
+        self._chk(lib().ovl_get_efield(self.h, _d(out)))
+        return out
+
+    def set_u1_flux(self, nb, q=1.0):
+        self._chk(lib().ovl_set_u1_flux(self.h, C.c_int(nb), C.c_double(q)))
+
+    # modes: complex arrays (count, V*nc*4) in the reference AoS layout
+    def set_modes(self, field, psi, first=0):
+        psi = np.ascontiguousarray(np.atleast_2d(psi), np.complex128)
+        self._chk(lib().ovl_set_modes(self.h, field, psi.ctypes.data_as(C.POINTER(C.c_double)), first, psi.shape[0]))
+
+    def get_modes(self, field, first=0, count=None):
+        count = self.n_modes - first if count is None else count
+        out = np.zeros((count, self.V * self.nc * 4), np.complex128)
+        self._chk(lib().ovl_get_modes(self.h, field, out.ctypes.data_as(C.POINTER(C.c_double)), first, count))
+        return out
+
+    def copy_field(self, src, dst):
+        self._chk(lib().ovl_copy_field(self.h, src, dst))
+
+    # operators
+    def apply_wilson(self, fin, fout):
+        self._chk(lib().ovl_apply_wilson(self.h, fin, fout))
+
+    def apply_overlap(self, fin, fout, opts: SignOpts):
+        return self._chk(lib().ovl_apply_overlap(self.h, fin, fout, C.byref(opts)), allow=(E_CONVERGE,))
+
+    def apply_sign(self, fin, fout, opts: SignOpts):
+        self._chk(lib().ovl_apply_sign(self.h, fin, fout, C.byref(opts)))
+
+    def sign_residual(self, fin, opts: SignOpts) -> float:
+        r = C.c_double()
+        self._chk(lib().ovl_sign_residual(self.h, fin, C.byref(opts), C.byref(r)))
+        return r.value
+
+    def ritz(self, start_flat, sign, delitr=1e-16, nitlz=2000, nrestart=100) -> float:
+        v = np.ascontiguousarray(start_flat, np.complex128)
+        r = C.c_double()
+        self._chk(lib().ovl_ritz(self.h, v.ctypes.data_as(C.POINTER(C.c_double)), C.c_double(sign),
+                                 C.c_double(delitr), nitlz, nrestart, C.byref(r)))
+        return r.value
+
+    def wilson_evo_compute(self, ucc, phi):
+        ucc = np.ascontiguousarray(ucc, np.complex128)
+        phi = np.ascontiguousarray(phi, np.complex128)
+        out = np.zeros_like(phi)
+        self._chk(lib().ovl_wilson_evo_compute(self.h, ucc.ctypes.data_as(C.POINTER(C.c_double)),
+                                               phi.ctypes.data_as(C.POINTER(C.c_double)),
+                                               out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out
+
+    def overlap_operator_compute(self, ucc, phi, lmin, lmax):
+        ucc = np.ascontiguousarray(ucc, np.complex128)
+        phi = np.ascontiguousarray(phi, np.complex128)
+        out = np.zeros_like(phi)
+        self._chk(lib().ovl_overlap_operator_compute(self.h, ucc.ctypes.data_as(C.POINTER(C.c_double)),
+                                                     phi.ctypes.data_as(C.POINTER(C.c_double)),
+                                                     out.ctypes.data_as(C.POINTER(C.c_double)),
+                                                     C.c_double(lmin), C.c_double(lmax)), allow=(E_CONVERGE,))
+        return out
+
+    # dynamics
+    def update_links(self, dt):
+        self._chk(lib().ovl_update_links(self.h, C.c_double(dt)))
+
+    def update_efield(self, dt):
+        self._chk(lib().ovl_update_efield(self.h, C.c_double(dt)))
+
+    def fermion_update(self, opts: StepOpts, nsteps):
+        return self._chk(lib().ovl_fermion_update(self.h, C.byref(opts), nsteps), allow=(E_CONVERGE,))
+
+    # observables
+    def _pair(self, fn):
+        a, b = C.c_double(), C.c_double()
+        self._chk(fn(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def energy(self):
+        return self._pair(lib().ovl_energy)
+
+    def gauss_violation(self):
+        return self._pair(lib().ovl_gauss_violation)
+
+    def unitarity_violation(self):
+        return self._pair(lib().ovl_unitarity_violation)
+
+    def ncs_dot(self):
+        r = C.c_double()
+        self._chk(lib().ovl_ncs_dot(self.h, C.byref(r)))
+        return r.value
+
+    def currents(self, field, opts: SignOpts, first_mode=0):
+        jv, ja = np.zeros(4 * self.V), np.zeros(4 * self.V)
+        q = C.c_double()
+        self._chk(lib().ovl_currents(self.h, field, C.byref(opts), first_mode, _d(jv), _d(ja), C.byref(q)))
+        return jv, ja, q.value
+
+    def mode_norms(self, field):
+        out = np.zeros(self.n_modes)
+        self._chk(lib().ovl_mode_norms(self.h, field, _d(out)))
+        return out
+
+    def timing(self) -> Timing:
+        t = Timing()
+        self._chk(lib().ovl_get_timing(self.h, C.byref(t)))
+        return t
+
+    def set_profiling(self, on: bool):
+        self._chk(lib().ovl_set_profiling(self.h, int(on)))

@@ -162,13 +162,15 @@ def update_links(L: Lat, links, e, dt):
 
 
 def leapfrog(L: Lat, links, e, psi, psimid, opts: StepOpts, nsteps):
+    """Returns (links, psi, psimid, efield) after nsteps."""
     links = np.array(links, np.float64)
+    e = np.array(e, np.float64)
     psi = cz(psi).copy()
     psimid = cz(psimid).copy()
     nmodes = psi.shape[0] if psi.ndim > 1 else 1
-    lib().orc_leapfrog(C.byref(L), dp(links), dp(np.ascontiguousarray(e, np.float64)), C.c_int(nmodes),
+    lib().orc_leapfrog(C.byref(L), dp(links), dp(e), C.c_int(nmodes),
                        dp(psi), dp(psimid), C.byref(opts), C.c_int(nsteps))
-    return links, psi, psimid
+    return links, psi, psimid, e
 
 
 def plaquette_energy(L: Lat, links, e):
@@ -253,3 +255,9 @@ def aos_to_flat(nc, V, psi):
     """AoS [site][colour][spin] -> flat m = site + V(spin + 4 colour) (OverlapU.cpp:163)."""
     a = cz(psi).reshape(V, nc, 4)
     return np.ascontiguousarray(np.transpose(a, (1, 2, 0)).reshape(-1))
+
+
+def efield_update(L: Lat, links, e, dt):
+    e = np.array(e, np.float64)
+    lib().orc_efield_update(C.byref(L), dp(np.ascontiguousarray(links, np.float64)), dp(e), C.c_double(dt))
+    return e

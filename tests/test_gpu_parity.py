@@ -1,0 +1,264 @@
+"""GPU parity: the gfx950 path (through the C ABI) against the CPU oracle and the reference's
+golden vectors.  Tolerances: FP64 kernels 1e-12 relative (integer-free floating point, FMA
+contraction differs from the oracle), FP32 kernels 2e-5, mixed FP32/FP64 sign 1e-10 against FP64."""
+import os
+
+import numpy as np
+import pytest
+
+import orc
+from overlap_amd import configs as cf
+from overlap_amd import ovl
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def make(nc, n, nmodes, seed, sigma=0.4, mb=0, r=1.0, m=0.0, M=0.0, nb=0, esig=0.2):
+    g = cf.rng(seed)
+    links = cf.su2_links(n, sigma, g) if nc == 2 else cf.su3_links(n, sigma, g)
+    e = cf.efield(n, nc, esig, g)
+    psi = cf.spinors(n, nc, nmodes, g)
+    ctx = ovl.Context(nc, n, nmodes, r_wilson=r, m_fermion=m, m_dw=M, mode_block=mb)
+    ctx.set_links(links)
+    ctx.set_efield(e)
+    if nb:
+        ctx.set_u1_flux(nb, 1.0)
+    ctx.set_modes(ovl.PSI, psi)
+    L = orc.lat(nc, n, r, m, M)
+    ucc = orc.links_to_ucc(L, links, orc.u1_phases(n, nb, 1.0) if nb else None)
+    return ctx, L, ucc, links, e, psi
+
+
+@pytest.mark.parametrize("nc,nb,mb", [(2, 0, 16), (2, 0, 64), (3, 3, 32), (3, 0, 64), (3, 2, 16)])
+def test_wilson_batched_vs_oracle(gpu, nc, nb, mb):
+    print(f"[case] wilson nc={nc} nb={nb} mb={mb}", flush=True)
+    ctx, L, ucc, *_, psi = make(nc, (4, 3, 5), 40, 100 + nc + mb, mb=mb, m=0.1, M=1.3, nb=nb)
+    ctx.apply_wilson(ovl.PSI, ovl.PSIMID)
+    out = ctx.get_modes(ovl.PSIMID)
+    for k in range(40):
+        assert rel(out[k], orc.wilson_evo(L, ucc, psi[k])) < 1e-13
+
+
+def test_wilson_dropin_reference_golden(gpu):
+    f = np.load(os.path.join(GOLD, "wilson_su2_4.npz"), allow_pickle=False)
+    n = tuple(int(v) for v in f["n"])
+    for k, (r, m, M) in enumerate(f["params"]):
+        with ovl.Context(2, n, 1, r_wilson=r, m_fermion=m, m_dw=M) as ctx:
+            out = ctx.wilson_evo_compute(f["ucc"], f["phi"][k])
+            assert rel(out, f["psi"][k]) < 1e-14
+
+
+def test_overlap_dropin_reference_golden(gpu):
+    # OverlapOperator::Compute: 25-pole Zolotarev + multi-shift CG to ||r||^2 <= 1e-16 ||b||
+    f = np.load(os.path.join(GOLD, "overlap_su2_4.npz"), allow_pickle=False)
+    n = tuple(int(v) for v in f["n"])
+    r, m, M = f["params"]
+    lmin, lmax = f["interval"]
+    with ovl.Context(2, n, 1, r_wilson=r, m_fermion=m, m_dw=M) as ctx:
+        for k in range(2):
+            out = ctx.overlap_operator_compute(f["ucc"], f["phi"][k], lmin, lmax)
+            assert rel(out, f["psi"][k]) < 1e-9
+
+
+def test_ritz_reference_golden(gpu):
+    f = np.load(os.path.join(GOLD, "ritz_su2_4.npz"), allow_pickle=False)
+    n = tuple(int(v) for v in f["n"])
+    r, m, M = f["params"]
+    ucc = f["ucc"]
+    # the drop-in stencil comes from u_cc; rebuild master links from it (GetGaugeLink)
+    V = int(np.prod(n))
+    u = ucc.reshape(3, V, 2, 2)
+    links = np.empty((V, 3, 4))
+    for mu in range(3):
+        mm = u[mu]
+        links[:, mu, 0] = (0.5j * (-mm[:, 1, 0] - mm[:, 0, 1])).real
+        links[:, mu, 1] = (0.5 * (-mm[:, 1, 0] + mm[:, 0, 1])).real
+        links[:, mu, 2] = (0.5j * (mm[:, 0, 0] - mm[:, 1, 1])).real
+        links[:, mu, 3] = (0.5 * (mm[:, 0, 0] + mm[:, 1, 1])).real
+    with ovl.Context(2, n, 1, r_wilson=r, m_fermion=m, m_dw=M) as ctx:
+        ctx.set_links(links.reshape(-1))
+        lo = ctx.ritz(f["start"], +1.0)
+        hi = ctx.ritz(f["start"], -1.0)
+    assert abs(lo - f["eig"][0]) < 1e-9 * abs(f["eig"][0]) + 1e-12
+    assert abs(hi - f["eig"][1]) < 1e-9 * abs(f["eig"][1])
+
+
+@pytest.mark.parametrize("nc,nb,prec,tol", [(2, 0, ovl.PREC_FP64, 1e-12), (3, 2, ovl.PREC_FP64, 1e-12),
+                                            (3, 2, ovl.PREC_FP32, 2e-5), (2, 1, ovl.PREC_FP32, 2e-5)])
+def test_chebyshev_sign_vs_oracle(gpu, nc, nb, prec, tol):
+    print(f"[case] sign nc={nc} nb={nb} prec={prec}", flush=True)
+    n = (4, 4, 3)
+    ctx, L, ucc, *_, psi = make(nc, n, 6, 7 + nc, M=1.2, nb=nb)
+    deg, cut = 96, 0.1
+    o = ovl.sign_opts(degree=deg, cut=cut, precision=prec)
+    ctx.apply_sign(ovl.PSI, ovl.PSIMID, o)
+    out = ctx.get_modes(ovl.PSIMID)
+    coef = orc.cheb_coeffs(deg, cut)
+    lam = orc.norm_bound(L)
+    for k in range(6):
+        assert rel(out[k], orc.sign_cheb(L, ucc, lam, cut, deg, coef, psi[k])) < tol
+
+
+@pytest.mark.parametrize("nc,nb", [(2, 0), (3, 4)])
+def test_overlap_chebyshev_vs_oracle(gpu, nc, nb):
+    n = (4, 3, 4)
+    ctx, L, ucc, *_, psi = make(nc, n, 5, 17 + nc, m=0.05, M=1.1, nb=nb)
+    deg, cut = 80, 0.08
+    ctx.apply_overlap(ovl.PSI, ovl.PSIMID, ovl.sign_opts(degree=deg, cut=cut))
+    out = ctx.get_modes(ovl.PSIMID)
+    coef = orc.cheb_coeffs(deg, cut)
+    for k in range(5):
+        assert rel(out[k], orc.overlap_evo_cheb(L, ucc, orc.norm_bound(L), cut, deg, coef, psi[k])) < 1e-12
+
+
+@pytest.mark.parametrize("nc,nb", [(2, 0), (3, 8)])
+def test_mixed_precision_sign_reaches_1e10(gpu, nc, nb):
+    # FP32 tail + FP64 head of the Clenshaw recurrence, head chosen for 1e-10
+    n = (6, 6, 6)
+    ctx, L, ucc, *_, psi = make(nc, n, 64, 31 + nc, sigma=0.25, M=1.2, nb=nb)
+    deg = 384 if nc == 3 else 256
+    o64 = ovl.sign_opts(degree=deg, cut=0.05, precision=ovl.PREC_FP64)
+    ctx.apply_sign(ovl.PSI, ovl.PSIMID, o64)
+    ref = ctx.get_modes(ovl.PSIMID)
+    om = ovl.sign_opts(degree=deg, cut=0.05, precision=ovl.PREC_MIXED, tol=1e-10)
+    ctx.apply_sign(ovl.PSI, ovl.SCRATCH, om)
+    t = ctx.timing()
+    mix = ctx.get_modes(ovl.SCRATCH)
+    err = max(np.linalg.norm(mix[k] - ref[k]) / np.linalg.norm(psi[k]) for k in range(64))
+    print(f"[mixed] nc={nc} fp64 steps {t.fp64_steps} fp32 steps {t.fp32_steps} err {err:.3e}", flush=True)
+    assert t.fp32_steps > 0 and t.fp64_steps < deg + 1
+    assert err <= 1e-10
+
+
+def test_sign_residual_and_mode_chunks(gpu):
+    n = (6, 6, 6)
+    ctx, L, ucc, *_, psi = make(3, n, 48, 77, sigma=0.25, M=1.2, nb=8, mb=16)
+    o = ovl.sign_opts(degree=384, cut=0.05)
+    r = ctx.sign_residual(ovl.PSI, o)
+    print(f"[sign residual] {r:.3e}", flush=True)
+    assert r < 1e-10
+    ctx.apply_sign(ovl.PSI, ovl.PSIMID, o)
+    a = ctx.get_modes(ovl.PSIMID)
+    o.mode_chunk = 1
+    ctx.apply_sign(ovl.PSI, ovl.PSIMID, o)
+    b = ctx.get_modes(ovl.PSIMID)
+    assert np.array_equal(a, b)
+
+
+def test_zolotarev_overlap_batched_vs_oracle(gpu):
+    n = (4, 4, 4)
+    ctx, L, ucc, *_, psi = make(2, n, 20, 91, m=0.1, M=1.0, mb=16)
+    o = ovl.sign_opts(method=ovl.SIGN_ZOLOTAREV, lmin=0.2, lmax=30.0)
+    ctx.apply_overlap(ovl.PSI, ovl.PSIMID, o)
+    out = ctx.get_modes(ovl.PSIMID)
+    for k in range(20):
+        assert rel(out[k], orc.overlap_evo_zolo(L, ucc, psi[k], 0.2, 30.0)[0]) < 1e-9
+
+
+def test_wilson_leapfrog_config0(gpu):
+    # SU(2) 8^3, 16 modes, m = 0.1, r = 1, M = 0, dt = 0.05, links moving with E fixed
+    c = cf.CONFIGS["su2_wilson_8"]
+    links, e, psi = c.generate()
+    steps = 40
+    ctx = ovl.Context(2, c.n, c.nmodes, r_wilson=c.r_wilson, m_fermion=c.m_fermion, m_dw=c.m_dw)
+    ctx.set_links(links)
+    ctx.set_efield(e)
+    ctx.set_modes(ovl.PSI, psi)
+    ctx.set_modes(ovl.PSIMID, psi)
+    so = ovl.StepOpts(kind=ovl.WILSON, dt=c.dt, gauge=ovl.GAUGE_LINKS, sign=ovl.sign_opts(), use_graph=0)
+    ctx.fermion_update(so, steps)
+    L = orc.lat(2, c.n, c.r_wilson, c.m_fermion, c.m_dw)
+    oo = orc.StepOpts(orc.WILSON, c.dt, 1, 0, 1.0, 0, 0.0, 0.0, 0.0, 8)
+    lk, p, pm, _ = orc.leapfrog(L, links, e, psi, psi, oo, steps)
+    assert rel(ctx.get_links(), lk) < 1e-13
+    assert rel(ctx.get_modes(ovl.PSI), p) < 1e-11
+    assert rel(ctx.get_modes(ovl.PSIMID), pm) < 1e-11
+
+
+@pytest.mark.parametrize("prec", [ovl.PREC_FP64, ovl.PREC_MIXED])
+def test_overlap_leapfrog_full_gauge_vs_oracle(gpu, prec):
+    n = (4, 4, 4)
+    nc, nb, deg, cut, dt, steps = 3, 2, 96, 0.1, 0.02, 3
+    ctx, L, ucc, links, e, psi = make(nc, n, 8, 123, sigma=0.3, M=1.2, nb=nb)
+    ctx.set_modes(ovl.PSIMID, psi)
+    so = ovl.StepOpts(kind=ovl.OVERLAP, dt=dt, gauge=ovl.GAUGE_FULL,
+                      sign=ovl.sign_opts(degree=deg, cut=cut, precision=prec, tol=1e-11), use_graph=0)
+    ctx.fermion_update(so, steps)
+    oo = orc.StepOpts(orc.OVERLAP_CHEB, dt, 2, nb, 1.0, deg, cut, 0.0, 0.0, 8)
+    lk, p, pm, ee = orc.leapfrog(L, links, e, psi, psi, oo, steps)
+    assert rel(ctx.get_links(), lk) < 1e-13
+    assert rel(ctx.get_efield(), ee) < 1e-12
+    tol = 1e-12 if prec == ovl.PREC_FP64 else 1e-10
+    assert rel(ctx.get_modes(ovl.PSI), p) < tol
+    assert rel(ctx.get_modes(ovl.PSIMID), pm) < tol
+
+
+def test_graph_replay_matches_stream(gpu):
+    n = (6, 4, 4)
+    outs = []
+    for graph in (0, 1, 1):
+        ctx, L, ucc, links, e, psi = make(3, n, 32, 55, sigma=0.3, M=1.2, nb=3)
+        ctx.set_modes(ovl.PSIMID, psi)
+        so = ovl.StepOpts(kind=ovl.OVERLAP, dt=0.02, gauge=ovl.GAUGE_LINKS,
+                          sign=ovl.sign_opts(degree=64, cut=0.1, precision=ovl.PREC_MIXED), use_graph=graph)
+        ctx.fermion_update(so, 3)
+        outs.append((ctx.get_modes(ovl.PSI), ctx.get_modes(ovl.PSIMID), ctx.get_links()))
+        ctx.close()
+    for o in outs[1:]:
+        for a, b in zip(o, outs[0]):
+            assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("nc", [2, 3])
+def test_observables_vs_oracle(gpu, nc):
+    n = (5, 4, 4)
+    ctx, L, ucc, links, e, psi = make(nc, n, 16, 200 + nc, sigma=0.6, M=1.0, esig=0.5)
+    assert np.allclose(ctx.energy(), orc.plaquette_energy(L, links, e), rtol=1e-13, atol=0)
+    g_gpu, g_orc = ctx.gauss_violation(), orc.gauss_violation(L, links, e)
+    assert np.allclose(g_gpu, g_orc, rtol=1e-12, atol=0)
+    u_gpu, u_orc = ctx.unitarity_violation(), orc.unitarity_violation(L, links)
+    assert abs(u_gpu[0] - u_orc[0]) < 1e-15 and abs(u_gpu[1] - u_orc[1]) < 1e-15
+    assert abs(ctx.ncs_dot() - orc.edotb(L, links, e) / (8 * np.pi ** 2)) < 1e-12 * abs(orc.edotb(L, links, e))
+    # one E update
+    ctx.update_efield(0.03)
+    assert rel(ctx.get_efield(), orc.efield_update(L, links, e, 0.03)) < 1e-13
+
+
+def test_currents_vs_oracle(gpu):
+    n = (4, 4, 4)
+    ctx, L, ucc, links, e, psi = make(2, n, 24, 300, sigma=0.3, M=1.0, mb=16)
+    deg, cut = 64, 0.1
+    o = ovl.sign_opts(degree=deg, cut=cut)
+    jv, ja, q = ctx.currents(ovl.PSI, o, first_mode=3)
+    coef = orc.cheb_coeffs(deg, cut)
+    h = np.stack([orc.overlap_evo_cheb(L, ucc, orc.norm_bound(L), cut, deg, coef, p) for p in psi])
+    jv0, ja0 = orc.currents(L, psi, h, first_mode=3)
+    assert np.abs(jv - jv0).max() < 1e-14 and np.abs(ja - ja0).max() < 1e-14
+    assert abs(q - ja0[0::4].sum()) < 1e-13
+
+
+def test_error_paths(gpu):
+    ctx = ovl.Context(2, (4, 4, 4), 4)
+    with pytest.raises(ovl.OvlError) as ex:
+        ctx.apply_wilson(ovl.PSI, ovl.PSIMID)
+    assert ex.value.code == ovl.E_STATE
+    ctx.set_links(cf.su2_links((4, 4, 4), 0.1, cf.rng(1)))
+    with pytest.raises(ovl.OvlError) as ex:
+        ctx.apply_wilson(ovl.PSI, 7)
+    assert ex.value.code == ovl.E_ARG
+    with pytest.raises(ovl.OvlError) as ex:
+        ctx.apply_sign(ovl.PSI, ovl.PSIMID, ovl.sign_opts(cut=1.5))
+    assert ex.value.code == ovl.E_SPECTRUM
+    bad = cf.spinors((4, 4, 4), 2, 1, cf.rng(2))
+    bad[0, 5] = np.nan
+    with pytest.raises(ovl.OvlError) as ex:
+        ctx.set_modes(ovl.PSI, bad)
+    assert ex.value.code == ovl.E_ARG
+    with pytest.raises(ovl.OvlError) as ex:
+        ctx.apply_overlap(ovl.PSI, ovl.PSIMID, ovl.sign_opts(method=ovl.SIGN_ZOLOTAREV, lmin=2.0, lmax=1.0))
+    assert ex.value.code == ovl.E_SPECTRUM

@@ -1,0 +1,382 @@
+"""Pins the CPU oracle: golden vectors of the compiled reference, closed forms, dense
+eigendecompositions and invariances.  CPU only."""
+import os
+
+import numpy as np
+import pytest
+
+import orc
+from overlap_amd import configs as cf
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def gold(name):
+    return np.load(os.path.join(GOLD, name), allow_pickle=False)
+
+
+# ---------------------------------------------------------------- golden vectors (reference)
+def test_wilson_matches_reference_golden():
+    f = gold("wilson_su2_4.npz")
+    n = tuple(int(v) for v in f["n"])
+    for k, (r, m, M) in enumerate(f["params"]):
+        L = orc.lat(2, n, r, m, M)
+        out = orc.wilson_evo(L, f["ucc"], f["phi"][k])
+        assert np.array_equal(out, f["psi"][k])  # same operation order: bit-exact
+
+
+def test_zolotarev_matches_reference_golden():
+    f = gold("zolotarev.npz")
+    for (amin, amax), eps, bz, dz in zip(f["ranges"], f["eps"], f["bz"], f["dz"]):
+        e, b, d = orc.zolotarev(amin, amax)
+        assert e == eps and np.array_equal(b, bz) and np.array_equal(d, dz)
+
+
+def test_overlap_zolotarev_matches_reference_golden():
+    f = gold("overlap_su2_4.npz")
+    n = tuple(int(v) for v in f["n"])
+    L = orc.lat(2, n, *f["params"])
+    lmin, lmax = f["interval"]
+    for k in range(2):
+        out, it = orc.overlap_evo_zolo(L, f["ucc"], f["phi"][k], lmin, lmax)
+        assert it > 10
+        assert np.array_equal(out, f["psi"][k])
+
+
+def test_ritz_matches_reference_golden():
+    f = gold("ritz_su2_4.npz")
+    n = tuple(int(v) for v in f["n"])
+    L = orc.lat(2, n, *f["params"])
+    assert orc.ritz(L, f["ucc"], f["start"], sign=1.0) == f["eig"][0]
+    assert orc.ritz(L, f["ucc"], f["start"], sign=-1.0) == f["eig"][1]
+
+
+def test_su2_algebra_matches_reference_golden():
+    f = gold("su2_algebra.npz")
+    L = orc.lat(2, (3, 3, 3))
+    for c, a, u, e, uu in zip(f["c"], f["alpha"], f["u"], f["iexp"], f["uu"]):
+        assert np.array_equal(orc.su2_iexp(c, a), e)
+        links = np.tile(u, 27 * 3)
+        E = np.tile(a, 27 * 3)
+        out = orc.update_links(L, links, E, -c)  # cU = -dt
+        assert np.array_equal(out[:4], uu)
+
+
+@pytest.mark.skipif(not orc.ref_available(), reason="compiled reference not built here")
+def test_oracle_matches_reference_on_fresh_inputs():
+    g = cf.rng(99)
+    n = (4, 3, 5)
+    links = cf.su2_links(n, 0.4, g)
+    phi = cf.spinors(n, 2, 1, g)[0]
+    L = orc.lat(2, n, 1.0, 0.2, 1.3)
+    ucc = orc.links_to_ucc(L, links)
+    assert np.array_equal(orc.wilson_evo(L, ucc, phi), orc.ref_wilson_evo(L, ucc, phi))
+    assert np.array_equal(orc.overlap_evo_zolo(L, ucc, phi, 0.3, 28.0)[0], orc.ref_overlap_evo(L, ucc, phi, 0.3, 28.0))
+
+
+# ---------------------------------------------------------------- closed forms and invariances
+def dense(L, ucc, fn=orc.hw):
+    V = L.n[0] * L.n[1] * L.n[2]
+    N = V * L.nc * 4
+    H = np.zeros((N, N), complex)
+    for k in range(N):
+        e = np.zeros(N, complex)
+        e[k] = 1
+        H[:, k] = fn(L, ucc, e)
+    return H
+
+
+def unit_links(nc, V):
+    if nc == 2:
+        u = np.zeros((V, 3, 4))
+        u[..., 3] = 1
+        return u.reshape(-1)
+    u = np.zeros((V, 3, 18))
+    u[..., 0] = u[..., 8] = u[..., 16] = 1
+    return u.reshape(-1)
+
+
+@pytest.mark.parametrize("nc", [2, 3])
+def test_free_field_spectrum(nc):
+    # h_W(p) = gamma0 W(p) + alpha.sin p, W = 3r + m - M - sum cos p (FermionField.cpp:145-317 with M)
+    n = (3, 4, 3)
+    r, m, M = 1.0, 0.1, 1.4
+    L = orc.lat(nc, n, r, m, M)
+    V = int(np.prod(n))
+    H = dense(L, orc.links_to_ucc(L, unit_links(nc, V)))
+    assert np.abs(H - H.conj().T).max() < 1e-14
+    ev = np.sort(np.linalg.eigvalsh(H))
+    want = []
+    for px in range(n[0]):
+        for py in range(n[1]):
+            for pz in range(n[2]):
+                p = 2 * np.pi * np.array([px / n[0], py / n[1], pz / n[2]])
+                w = np.sqrt(np.sum(np.sin(p) ** 2) + (3 * r + m - M - np.sum(np.cos(p))) ** 2)
+                want += [w, -w] * (2 * nc)
+    assert np.abs(ev - np.sort(want)).max() < 1e-13
+
+
+def su3_random_gauge(V, g):
+    a = g.normal(size=(V, 8))
+    return cf.su3_exp_i(a)  # (V,3,3)
+
+
+@pytest.mark.parametrize("nc,nb", [(2, 0), (3, 0), (3, 2)])
+def test_hermitian_and_gauge_covariant(nc, nb):
+    g = cf.rng(5)
+    n = (4, 3, 5)
+    V = int(np.prod(n))
+    links = cf.su2_links(n, 0.5, g) if nc == 2 else cf.su3_links(n, 0.5, g)
+    L = orc.lat(nc, n, 1.0, 0.05, 1.2)
+    ph = orc.u1_phases(n, nb, 1.0) if nb else None
+    ucc = orc.links_to_ucc(L, links, ph)
+    a, b = cf.spinors(n, nc, 2, g)
+    ha, hb = orc.hw(L, ucc, a), orc.hw(L, ucc, b)
+    assert abs(np.vdot(a, hb) - np.vdot(ha, b)) < 1e-13
+    # gauge covariance: U_mu(x) -> G(x) U G(x+mu)^+, psi -> G psi
+    if nc == 3:
+        G = su3_random_gauge(V, g)
+        U = cf.links_to_matrices(3, links)
+        x, y, z = np.meshgrid(np.arange(n[0]), np.arange(n[1]), np.arange(n[2]), indexing="ij")
+        site = lambda x, y, z: (x % n[0]) + n[0] * ((y % n[1]) + n[1] * (z % n[2]))  # noqa: E731
+        s = site(x, y, z).ravel(order="F")
+        xs, ys, zs = x.ravel(order="F"), y.ravel(order="F"), z.ravel(order="F")
+        Ug = U.copy()
+        for mu, (dx, dy, dz) in enumerate([(1, 0, 0), (0, 1, 0), (0, 0, 1)]):
+            sp = site(xs + dx, ys + dy, zs + dz)
+            Ug[s, mu] = G[s] @ U[s, mu] @ G[sp].conj().transpose(0, 2, 1)
+        cm = np.swapaxes(Ug, -1, -2).reshape(V, 3, 9)
+        lg = np.empty((V, 3, 18))
+        lg[..., 0::2], lg[..., 1::2] = cm.real, cm.imag
+        uccg = orc.links_to_ucc(L, lg.reshape(-1), ph)
+        rot = lambda v: np.einsum("sij,sjk->sik", G, v.reshape(V, 3, 4)).reshape(-1)  # noqa: E731
+        assert np.abs(orc.hw(L, uccg, rot(a)) - rot(ha)).max() < 1e-13
+
+
+def test_u1_plaquette_is_uniform():
+    # QED.cpp:155-161: -Im U_plaq = sin(2 pi Nb/(Nx Ny)) at every site, including the boundary
+    n = (6, 5, 3)
+    nb = 3
+    ph = orc.u1_phases(n, nb, 1.0).reshape(3, -1)
+    V = int(np.prod(n))
+    for s in range(V):
+        x, y = s % n[0], (s // n[0]) % n[1]
+        z = s // (n[0] * n[1])
+        sx = (x + 1) % n[0] + n[0] * (y + n[1] * z)
+        sy = x + n[0] * ((y + 1) % n[1] + n[1] * z)
+        box = ph[0, s] * ph[1, sx] * np.conj(ph[0, sy]) * np.conj(ph[1, s])
+        assert abs(-box.imag - np.sin(2 * np.pi * nb / (n[0] * n[1]))) < 1e-13
+
+
+def test_su3_embedding_reproduces_su2():
+    g = cf.rng(11)
+    n = (3, 4, 3)
+    V = int(np.prod(n))
+    l2 = cf.su2_links(n, 0.6, g)
+    M2 = cf.links_to_matrices(2, l2)
+    M3 = np.zeros((V, 3, 3, 3), complex)
+    M3[..., :2, :2] = M2
+    M3[..., 2, 2] = 1
+    cm = np.swapaxes(M3, -1, -2).reshape(V, 3, 9)
+    l3 = np.empty((V, 3, 18))
+    l3[..., 0::2], l3[..., 1::2] = cm.real, cm.imag
+    L2, L3 = orc.lat(2, n, 1.0, 0.1, 1.4), orc.lat(3, n, 1.0, 0.1, 1.4)
+    psi3 = cf.spinors(n, 3, 1, g)[0].reshape(V, 3, 4)
+    out3 = orc.wilson_evo(L3, orc.links_to_ucc(L3, l3.reshape(-1)), psi3.reshape(-1)).reshape(V, 3, 4)
+    out2 = orc.wilson_evo(L2, orc.links_to_ucc(L2, l2), np.ascontiguousarray(psi3[:, :2]).reshape(-1)).reshape(V, 2, 4)
+    assert np.abs(out3[:, :2] - out2).max() < 1e-14
+    # colour 2 sees the free operator
+    L1 = orc.lat(2, n, 1.0, 0.1, 1.4)
+    free = np.zeros((V, 2, 4), complex)
+    free[:, 0] = psi3[:, 2]
+    outf = orc.wilson_evo(L1, orc.links_to_ucc(L1, unit_links(2, V)), free.reshape(-1)).reshape(V, 2, 4)
+    assert np.abs(out3[:, 2] - outf[:, 0]).max() < 1e-14
+
+
+def test_chebyshev_sign_matches_dense_eigendecomposition():
+    g = cf.rng(3)
+    n = (3, 3, 3)
+    L = orc.lat(2, n, 1.0, 0.0, 1.0)
+    ucc = orc.links_to_ucc(L, cf.su2_links(n, 0.3, g))
+    H = dense(L, ucc)
+    w, v = np.linalg.eigh(H)
+    lam = orc.norm_bound(L)
+    assert np.abs(w).max() <= lam + 1e-12
+    cut = 0.8 * np.abs(w).min() / lam
+    deg = 300
+    coef = orc.cheb_coeffs(deg, cut)
+    # P(x) ~ x^-1/2 on [cut^2, 1]
+    xs = np.linspace(cut ** 2, 1, 2001)
+    t = (2 * xs - 1 - cut ** 2) / (1 - cut ** 2)
+    P = np.polynomial.chebyshev.chebval(t, coef)
+    assert np.abs(P * np.sqrt(xs) - 1).max() < 1e-12
+    S = v @ np.diag(np.sign(w)) @ v.conj().T
+    phi = cf.spinors(n, 2, 1, g)[0]
+    e = orc.sign_cheb(L, ucc, lam, cut, deg, coef, phi)
+    assert np.abs(e - S @ phi).max() < 1e-11
+    # overlap combination (OverlapU.cpp:188-201)
+    hov = orc.overlap_evo_cheb(L, ucc, lam, cut, deg, coef, phi)
+    g0 = np.tile(np.array([1, 1, -1, -1]), 27 * 2)
+    want = 1.0 * (-1j * (1 + 0) * g0 * phi + (1 - 0) * (-1j) * (S @ phi))
+    assert np.abs(hov - want).max() < 1e-11
+
+
+def test_zolotarev_sign_accuracy_is_float_limited():
+    # the reference's rational approximation is only ~1e-7 accurate (float square roots)
+    g = cf.rng(4)
+    n = (3, 3, 3)
+    L = orc.lat(2, n, 1.0, 0.0, 1.0)
+    ucc = orc.links_to_ucc(L, cf.su2_links(n, 0.3, g))
+    w, v = np.linalg.eigh(dense(L, ucc))
+    lmin, lmax = 0.9 * w.min() ** 2 if w.min() > 0 else 0.9 * np.min(w ** 2), 1.05 * np.max(w ** 2)
+    eps, bz, dz = orc.zolotarev(lmin, lmax)
+    x = np.geomspace(lmin, lmax, 500)
+    r = np.sqrt(x) * (bz[None, :] / (x[:, None] + dz[None, :])).sum(1)
+    assert np.abs(r - 1).max() < 2 * eps + 1e-12
+    assert 1e-9 < eps < 1e-6
+    phi = cf.spinors(n, 2, 1, g)[0]
+    out, _ = orc.overlap_evo_zolo(L, ucc, phi, lmin, lmax)
+    S = v @ np.diag(np.sign(w)) @ v.conj().T
+    g0 = np.tile(np.array([1, 1, -1, -1]), 27 * 2)
+    want = -1j * g0 * phi + (-1j) * (S @ phi)
+    assert np.abs(out - want).max() < 1e-6
+
+
+def test_su3_exponential():
+    g = cf.rng(8)
+    for _ in range(20):
+        a = g.normal(size=8)
+        c = g.normal() * 2.0
+        e = orc.su3_iexp(c, a)
+        m = (e[0::2] + 1j * e[1::2]).reshape(3, 3).T  # (i,j) at i+3j
+        want = cf.su3_exp_i(c * a)
+        assert np.abs(m - want).max() < 1e-13
+        assert abs(np.linalg.det(m) - 1) < 1e-13
+
+
+@pytest.mark.parametrize("nc", [2, 3])
+def test_link_update_exact_for_constant_e(nc):
+    g = cf.rng(9)
+    n = (3, 3, 3)
+    L = orc.lat(nc, n)
+    links = cf.su2_links(n, 0.4, g) if nc == 2 else cf.su3_links(n, 0.4, g)
+    e = cf.efield(n, nc, 0.7, g)
+    two = orc.update_links(L, orc.update_links(L, links, e, 0.05), e, 0.05)
+    one = orc.update_links(L, links, e, 0.1)
+    assert np.abs(two - one).max() < 1e-14
+    mx, rms = orc.unitarity_violation(L, one)
+    assert mx < 1e-14
+
+
+@pytest.mark.parametrize("nc", [2, 3])
+def test_efield_force_is_gradient_of_magnetic_energy(nc):
+    # E' = -dV/d(theta) for U -> exp(-i theta t^a) U ; finite differences of orc_plaquette_energy
+    g = cf.rng(12 + nc)
+    n = (3, 3, 4)
+    L = orc.lat(nc, n)
+    ng = nc * nc - 1
+    links = cf.su2_links(n, 0.8, g) if nc == 2 else cf.su3_links(n, 0.8, g)
+    zero = np.zeros(27 * 4 // 3 * 3 * ng * 0 + int(np.prod(n)) * 3 * ng)
+    dE = orc.efield_update(L, links, zero, 1.0)  # = force
+    h = 1e-5
+    for (s, mu, a) in [(0, 0, 0), (5, 2, 1), (17, 1, ng - 1)]:
+        ep = zero.copy()
+        ep[a + ng * (mu + 3 * s)] = 1.0
+        up = orc.update_links(L, links, ep, h)   # U <- exp(-i h t^a) U
+        um = orc.update_links(L, links, ep, -h)
+        vp = orc.plaquette_energy(L, up, zero)[1]
+        vm = orc.plaquette_energy(L, um, zero)[1]
+        assert abs(dE[a + ng * (mu + 3 * s)] + (vp - vm) / (2 * h)) < 1e-7
+
+
+@pytest.mark.parametrize("nc", [2, 3])
+def test_gauge_leapfrog_conserves_gauss_law_and_energy(nc):
+    g = cf.rng(21)
+    n = (4, 3, 3)
+    L = orc.lat(nc, n)
+    links0 = cf.su2_links(n, 0.5, g) if nc == 2 else cf.su3_links(n, 0.5, g)
+    e0 = cf.efield(n, nc, 0.3, g)
+    drifts = []
+    for dt in (0.02, 0.01):
+        links, e = links0.copy(), e0.copy()
+        g0 = orc.gauss_violation(L, links, e)
+        en0 = sum(orc.plaquette_energy(L, links, e))
+        worst = 0
+        for _ in range(int(round(0.4 / dt))):
+            e = orc.efield_update(L, links, e, dt)
+            links = orc.update_links(L, links, e, dt)
+            worst = max(worst, abs(sum(orc.plaquette_energy(L, links, e)) - en0))
+        g1 = orc.gauss_violation(L, links, e)
+        assert abs(g1[0] - g0[0]) < 1e-11 and abs(g1[1] - g0[1]) < 1e-11
+        drifts.append(worst)
+    assert drifts[1] < 0.6 * drifts[0]
+
+
+def test_gauss_and_edotb_are_gauge_invariant():
+    g = cf.rng(31)
+    n = (4, 4, 3)
+    V = int(np.prod(n))
+    L = orc.lat(3, n)
+    links = cf.su3_links(n, 0.7, g)
+    e = cf.efield(n, 3, 0.4, g)
+    G = su3_random_gauge(V, g)
+    U = cf.links_to_matrices(3, links)
+    t = cf.gell_mann_generators()
+    Em = np.einsum("xma,aij->xmij", e.reshape(V, 3, 8), t)
+    site = lambda x, y, z: (x % n[0]) + n[0] * ((y % n[1]) + n[1] * (z % n[2]))  # noqa: E731
+    Ug, Eg = U.copy(), Em.copy()
+    for s in range(V):
+        x, y, z = s % n[0], (s // n[0]) % n[1], s // (n[0] * n[1])
+        for mu, d in enumerate([(1, 0, 0), (0, 1, 0), (0, 0, 1)]):
+            sp = site(x + d[0], y + d[1], z + d[2])
+            Ug[s, mu] = G[s] @ U[s, mu] @ G[sp].conj().T
+            Eg[s, mu] = G[s] @ Em[s, mu] @ G[s].conj().T
+    eg = 2 * np.einsum("aij,xmji->xma", t, Eg).real.reshape(-1)
+    cm = np.swapaxes(Ug, -1, -2).reshape(V, 3, 9)
+    lg = np.empty((V, 3, 18))
+    lg[..., 0::2], lg[..., 1::2] = cm.real, cm.imag
+    lg = lg.reshape(-1)
+    a = orc.edotb(L, links, e)
+    assert abs(a) > 1e-3
+    assert abs(orc.edotb(L, lg, eg) - a) < 1e-11
+    assert abs(orc.edotb(L, links, 2 * e) - 2 * a) < 1e-11
+    assert orc.edotb(L, links, 0 * e) == 0.0
+    # sum_a (G^a)^2 per site is invariant; the max over single components (GaussLawViolation.cpp:41-46) is not
+    assert abs(orc.gauss_violation(L, links, e)[1] - orc.gauss_violation(L, lg, eg)[1]) < 1e-12
+
+
+def test_currents_against_direct_bilinears():
+    g = cf.rng(41)
+    n = (3, 3, 4)
+    V = int(np.prod(n))
+    L = orc.lat(2, n, 1.0, 0.0, 1.0)
+    psi = cf.spinors(n, 2, 3, g)
+    hpsi = cf.spinors(n, 2, 3, g)
+    jv, ja = orc.currents(L, psi, hpsi, first_mode=1)
+    g0 = np.diag([1, 1, -1, -1]).astype(complex)
+    g5 = np.block([[np.zeros((2, 2)), np.eye(2)], [np.eye(2), np.zeros((2, 2))]]).astype(complex)
+    p = psi.reshape(3, V, 2, 4)
+    h = hpsi.reshape(3, V, 2, 4)
+    w = np.array([-1.0, 1.0, -1.0])  # global modes 1, 2, 3
+    # j^0_V = 1/2V sum w [psi^+ psi + Re psi^+ (-i/2) g0 h]
+    want0 = (w[:, None] * (np.einsum("kvca,kvca->kv", p.conj(), p).real +
+                           np.einsum("kvca,ab,kvcb->kv", p.conj(), -0.5j * g0, h).real)).sum(0) / (2 * V)
+    # j^0_A = 1/2V sum w Re[psi^+ g5 psi + psi^+ (-i/2) g5 g0 h]
+    wantA = (w[:, None] * (np.einsum("kvca,ab,kvcb->kv", p.conj(), g5, p).real +
+                           np.einsum("kvca,ab,kvcb->kv", p.conj(), -0.5j * g5 @ g0, h).real)).sum(0) / (2 * V)
+    assert np.abs(jv[0::4] - want0).max() < 1e-15
+    assert np.abs(ja[0::4] - wantA).max() < 1e-15
+
+
+def test_wilson_leapfrog_norm_and_free_amplification():
+    # staggered leapfrog psi(t+dt) = psi + dt(-ih)psimid is stable and norm-preserving to O(dt^2)
+    g = cf.rng(51)
+    n = (4, 4, 4)
+    L = orc.lat(2, n, 1.0, 0.1, 0.0)
+    links = cf.su2_links(n, 0.3, g)
+    e = np.zeros(int(np.prod(n)) * 9)
+    psi = cf.spinors(n, 2, 2, g)
+    o = orc.StepOpts(orc.WILSON, 0.05, 0, 0, 1.0, 0, 0.0, 0.0, 0.0, 1)
+    _, p1, pm1, _ = orc.leapfrog(L, links, e, psi, psi, o, 40)
+    nrm = np.linalg.norm(p1, axis=1)
+    assert np.all(np.abs(nrm - 1) < 0.02)
