@@ -25,5 +25,22 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+def build_compat_test(verbose: bool = False) -> str:
+    """C++ test driver of the reference-shaped API (tests/compat), linked against libovl.so and
+    the oracle; needs oracle/liboracle.so to exist."""
+    root = os.path.dirname(HERE)
+    src = os.path.join(root, "tests", "compat", "test_compat.cpp")
+    out = os.path.join(root, "tests", "compat", "test_compat")
+    cmd = ["g++", "-O2", "-std=c++17", src, "-o", out, f"-L{HERE}", f"-L{os.path.join(root, 'oracle')}",
+           "-lovl", "-loracle", "-Wl,-rpath,$ORIGIN/../../overlap_amd", "-Wl,-rpath,$ORIGIN/../../oracle"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("compat test build failed")
+    return out
+
+
 if __name__ == "__main__":
     print(build(verbose=True))

@@ -1,0 +1,127 @@
+// Exercises overlap_amd/compat/ovl_reference_api.hpp with objects shaped like the reference's
+// GaugeLinks / ElectricFields / FermionField (LATTICE/GaugeLinks.cpp:11-89, ElectricFields.cpp:9-88,
+// FermionField.cpp:12-623) and checks every call against the CPU oracle.  Exit status 0 = pass.
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <random>
+#include <vector>
+
+#include "../../oracle/ovl_oracle.h"
+#include "../../overlap_amd/compat/ovl_reference_api.hpp"
+
+typedef std::complex<double> COMPLEX;
+typedef struct { COMPLEX f[4]; } wf;
+typedef struct { wf c[2]; } wcf;
+typedef struct { COMPLEX c[2]; } u_c;
+typedef struct { u_c c[2]; } u_cc;
+
+struct GaugeLinks {
+    int Volume, N[3];
+    double a[3] = {1, 1, 1}, aCube = 1;
+    std::vector<double> store;
+    double *U;
+    GaugeLinks(int nx, int ny, int nz) : Volume(nx * ny * nz), N{nx, ny, nz}, store(4 * 3 * nx * ny * nz), U(store.data()) {}
+};
+struct ElectricFields {
+    int Volume, N[3];
+    std::vector<double> store;
+    double *E;
+    ElectricFields(int nx, int ny, int nz) : Volume(nx * ny * nz), N{nx, ny, nz}, store(3 * 3 * nx * ny * nz), E(store.data()) {}
+};
+struct FermionField {
+    int N[3], Volume, NumberOfSamples;
+    std::vector<std::vector<wcf>> modes;
+    FermionField(int nx, int ny, int nz, int ns) : N{nx, ny, nz}, Volume(nx * ny * nz), NumberOfSamples(ns), modes(ns, std::vector<wcf>(nx * ny * nz)) {}
+    wcf *GetMode(int s) { return modes[s].data(); }
+};
+
+static double maxrel(const double *a, const double *b, size_t n)
+{
+    double d = 0, m = 0;
+    for (size_t i = 0; i < n; ++i) { d = std::max(d, std::fabs(a[i] - b[i])); m = std::max(m, std::fabs(b[i])); }
+    return d / m;
+}
+
+int main()
+{
+    const int n[3] = {4, 4, 4}, V = 64, modes = 6;
+    std::mt19937_64 rng(7);
+    std::normal_distribution<double> nd(0.0, 1.0);
+    GaugeLinks U(4, 4, 4);
+    ElectricFields E(4, 4, 4);
+    for (int i = 0; i < 3 * V; ++i) {  // random SU(2) links near 1
+        double v[4] = {0.2 * nd(rng), 0.2 * nd(rng), 0.2 * nd(rng), 1.0};
+        double s = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2] + v[3] * v[3]);
+        for (int k = 0; k < 4; ++k) U.U[4 * i + k] = v[k] / s;
+    }
+    for (auto &e : E.store) e = 0.1 * nd(rng);
+    FermionField Psi(4, 4, 4, modes), PsiMid(4, 4, 4, modes);
+    for (int s = 0; s < modes; ++s)
+        for (int x = 0; x < V; ++x)
+            for (int c = 0; c < 2; ++c)
+                for (int a = 0; a < 4; ++a) Psi.modes[s][x].c[c].f[a] = PsiMid.modes[s][x].c[c].f[a] = COMPLEX(nd(rng), nd(rng)) * 0.05;
+    int fails = 0;
+    ovl_ref::Settings &cfg = ovl_ref::settings();
+    cfg.nc = 2; cfg.n[0] = cfg.n[1] = cfg.n[2] = 4;
+    cfg.r_wilson = 1.0; cfg.m_fermion = 0.1; cfg.m_dw = 1.0;
+    orc_lat L{2, {4, 4, 4}, 1.0, 0.1, 1.0};
+    std::vector<u_cc> ucc(3 * V);
+    orc_links_to_ucc(&L, U.U, nullptr, reinterpret_cast<double *>(ucc.data()));
+
+    // WilsonEvoOperator::Compute
+    std::vector<wcf> out(V), want(V);
+    ovl_ref::WilsonEvoOperator::Compute(ucc.data(), Psi.GetMode(0), out.data());
+    orc_wilson_evo(&L, reinterpret_cast<double *>(ucc.data()), reinterpret_cast<double *>(Psi.GetMode(0)), reinterpret_cast<double *>(want.data()));
+    double e1 = maxrel(reinterpret_cast<double *>(out.data()), reinterpret_cast<double *>(want.data()), 16 * V);
+    std::printf("WilsonEvoOperator::Compute rel err %.3e\n", e1);
+    fails += !(e1 < 1e-13);
+
+    // OverlapOperator::Compute
+    ovl_ref::OverlapOperator::Compute(ucc.data(), Psi.GetMode(1), out.data(), 0.2, 30.0);
+    orc_overlap_evo_zolo(&L, reinterpret_cast<double *>(ucc.data()), reinterpret_cast<double *>(Psi.GetMode(1)), reinterpret_cast<double *>(want.data()), 0.2, 30.0);
+    double e2 = maxrel(reinterpret_cast<double *>(out.data()), reinterpret_cast<double *>(want.data()), 16 * V);
+    std::printf("OverlapOperator::Compute rel err %.3e\n", e2);
+    fails += !(e2 < 1e-9);
+
+    // Dynamics::Fermions::Update, Wilson fermions, 5 steps, links moving (t < tSphaleron)
+    std::vector<double> lk(U.store), ef(E.store);
+    std::vector<wcf> p0(modes * V), pm0(modes * V);
+    for (int s = 0; s < modes; ++s)
+        for (int x = 0; x < V; ++x) { p0[s * V + x] = Psi.modes[s][x]; pm0[s * V + x] = PsiMid.modes[s][x]; }
+    cfg.fermion = OVL_WILSON;
+    cfg.dTau = 0.05;
+    for (int st = 0; st < 5; ++st) ovl_ref::Dynamics::Fermions::Update(&U, &E, &Psi, &PsiMid);
+    orc_step_opts o{};
+    o.kind = ORC_WILSON; o.dt = 0.05; o.update_links = 1; o.threads = 1;
+    orc_leapfrog(&L, lk.data(), ef.data(), modes, reinterpret_cast<double *>(p0.data()), reinterpret_cast<double *>(pm0.data()), &o, 5);
+    double e3 = maxrel(U.U, lk.data(), lk.size());
+    double e4 = 0;
+    for (int s = 0; s < modes; ++s)
+        e4 = std::max(e4, maxrel(reinterpret_cast<double *>(Psi.GetMode(s)), reinterpret_cast<double *>(&p0[s * V]), 16 * V));
+    std::printf("Dynamics::Fermions::Update links %.3e modes %.3e\n", e3, e4);
+    fails += !(e3 < 1e-13) + !(e4 < 1e-12);
+
+    // ChernSimonsNumber::NCsDot
+    double nd1 = ovl_ref::ChernSimonsNumber::NCsDot(&U, &E);
+    double nd2 = orc_edotb(&L, U.U, E.E) / (8.0 * M_PI * M_PI);
+    std::printf("NCsDot %.15e vs %.15e\n", nd1, nd2);
+    fails += !(std::fabs(nd1 - nd2) <= 1e-12 * std::fabs(nd2));
+
+    // ZolotarevCoefficients (host)
+    double bz[25], dz[25], bz2[25], dz2[25];
+    double ea = ovl_ref::OverlapOperations::ZolotarevCoefficients(0.1, 25.0, 1e-12, 25, bz, dz);
+    double eb = orc_zolotarev(0.1, 25.0, 1e-12, 25, bz2, dz2);
+    fails += !(ea == eb && std::memcmp(bz, bz2, sizeof bz) == 0 && std::memcmp(dz, dz2, sizeof dz) == 0);
+
+    // error behaviour: bad range throws with the C-ABI code
+    try {
+        ovl_ref::Dynamics::UpdateGaugeLinks(1, 3, 0, 3, 0, 3, &U, &E, 0.01);
+        fails++;
+    } catch (const ovl_ref::Error &err) {
+        fails += err.code != OVL_E_ARG;
+    }
+    ovl_ref::Shutdown();
+    std::printf("%s (%d failures)\n", fails ? "FAIL" : "PASS", fails);
+    return fails ? 1 : 0;
+}
