@@ -280,7 +280,9 @@ __global__ __launch_bounds__(TPB) void k_aos_to_soa(const double2 *__restrict__ 
     long i = (long)blockIdx.x * TPB + threadIdx.x;
     long tot = (long)count * V * ncomp;
     if (i >= tot) return;
-    const int c = (int)(i % ncomp);
+    // host AoS component = colour*4 + spin; device component = spin*Nc + colour
+    const int ca = (int)(i % ncomp);
+    const int c = (ca % 4) * (ncomp / 4) + ca / 4;
     const long q = i / ncomp;
     const int s = (int)(q % V);
     const int k = (int)(q / V);
@@ -294,7 +296,9 @@ __global__ __launch_bounds__(TPB) void k_soa_to_aos(const double2 *__restrict__ 
     long i = (long)blockIdx.x * TPB + threadIdx.x;
     long tot = (long)count * V * ncomp;
     if (i >= tot) return;
-    const int c = (int)(i % ncomp);
+    // host AoS component = colour*4 + spin; device component = spin*Nc + colour
+    const int ca = (int)(i % ncomp);
+    const int c = (ca % 4) * (ncomp / 4) + ca / 4;
     const long q = i / ncomp;
     const int s = (int)(q % V);
     const int k = (int)(q / V);

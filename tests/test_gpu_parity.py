@@ -41,7 +41,8 @@ def test_wilson_batched_vs_oracle(gpu, nc, nb, mb):
     ctx.apply_wilson(ovl.PSI, ovl.PSIMID)
     out = ctx.get_modes(ovl.PSIMID)
     for k in range(40):
-        assert rel(out[k], orc.wilson_evo(L, ucc, psi[k])) < 1e-13
+        err = rel(out[k], orc.wilson_evo(L, ucc, psi[k]))
+        assert err < 1e-13
 
 
 def test_wilson_dropin_reference_golden(gpu):
@@ -50,7 +51,8 @@ def test_wilson_dropin_reference_golden(gpu):
     for k, (r, m, M) in enumerate(f["params"]):
         with ovl.Context(2, n, 1, r_wilson=r, m_fermion=m, m_dw=M) as ctx:
             out = ctx.wilson_evo_compute(f["ucc"], f["phi"][k])
-            assert rel(out, f["psi"][k]) < 1e-14
+            err = rel(out, f["psi"][k])
+            assert err < 1e-14
 
 
 def test_overlap_dropin_reference_golden(gpu):
@@ -62,7 +64,8 @@ def test_overlap_dropin_reference_golden(gpu):
     with ovl.Context(2, n, 1, r_wilson=r, m_fermion=m, m_dw=M) as ctx:
         for k in range(2):
             out = ctx.overlap_operator_compute(f["ucc"], f["phi"][k], lmin, lmax)
-            assert rel(out, f["psi"][k]) < 1e-9
+            err = rel(out, f["psi"][k])
+            assert err < 1e-9
 
 
 def test_ritz_reference_golden(gpu):
@@ -101,7 +104,8 @@ def test_chebyshev_sign_vs_oracle(gpu, nc, nb, prec, tol):
     coef = orc.cheb_coeffs(deg, cut)
     lam = orc.norm_bound(L)
     for k in range(6):
-        assert rel(out[k], orc.sign_cheb(L, ucc, lam, cut, deg, coef, psi[k])) < tol
+        err = rel(out[k], orc.sign_cheb(L, ucc, lam, cut, deg, coef, psi[k]))
+        assert err < tol
 
 
 @pytest.mark.parametrize("nc,nb", [(2, 0), (3, 4)])
@@ -113,7 +117,8 @@ def test_overlap_chebyshev_vs_oracle(gpu, nc, nb):
     out = ctx.get_modes(ovl.PSIMID)
     coef = orc.cheb_coeffs(deg, cut)
     for k in range(5):
-        assert rel(out[k], orc.overlap_evo_cheb(L, ucc, orc.norm_bound(L), cut, deg, coef, psi[k])) < 1e-12
+        err = rel(out[k], orc.overlap_evo_cheb(L, ucc, orc.norm_bound(L), cut, deg, coef, psi[k]))
+        assert err < 1e-12
 
 
 @pytest.mark.parametrize("nc,nb", [(2, 0), (3, 8)])
@@ -157,7 +162,8 @@ def test_zolotarev_overlap_batched_vs_oracle(gpu):
     ctx.apply_overlap(ovl.PSI, ovl.PSIMID, o)
     out = ctx.get_modes(ovl.PSIMID)
     for k in range(20):
-        assert rel(out[k], orc.overlap_evo_zolo(L, ucc, psi[k], 0.2, 30.0)[0]) < 1e-9
+        err = rel(out[k], orc.overlap_evo_zolo(L, ucc, psi[k], 0.2, 30.0)[0])
+        assert err < 1e-9
 
 
 def test_wilson_leapfrog_config0(gpu):
@@ -175,9 +181,12 @@ def test_wilson_leapfrog_config0(gpu):
     L = orc.lat(2, c.n, c.r_wilson, c.m_fermion, c.m_dw)
     oo = orc.StepOpts(orc.WILSON, c.dt, 1, 0, 1.0, 0, 0.0, 0.0, 0.0, 8)
     lk, p, pm, _ = orc.leapfrog(L, links, e, psi, psi, oo, steps)
-    assert rel(ctx.get_links(), lk) < 1e-13
-    assert rel(ctx.get_modes(ovl.PSI), p) < 1e-11
-    assert rel(ctx.get_modes(ovl.PSIMID), pm) < 1e-11
+    err = rel(ctx.get_links(), lk)
+    assert err < 1e-13
+    err = rel(ctx.get_modes(ovl.PSI), p)
+    assert err < 1e-11
+    err = rel(ctx.get_modes(ovl.PSIMID), pm)
+    assert err < 1e-11
 
 
 @pytest.mark.parametrize("prec", [ovl.PREC_FP64, ovl.PREC_MIXED])
@@ -191,11 +200,15 @@ def test_overlap_leapfrog_full_gauge_vs_oracle(gpu, prec):
     ctx.fermion_update(so, steps)
     oo = orc.StepOpts(orc.OVERLAP_CHEB, dt, 2, nb, 1.0, deg, cut, 0.0, 0.0, 8)
     lk, p, pm, ee = orc.leapfrog(L, links, e, psi, psi, oo, steps)
-    assert rel(ctx.get_links(), lk) < 1e-13
-    assert rel(ctx.get_efield(), ee) < 1e-12
+    err = rel(ctx.get_links(), lk)
+    assert err < 1e-13
+    err = rel(ctx.get_efield(), ee)
+    assert err < 1e-12
     tol = 1e-12 if prec == ovl.PREC_FP64 else 1e-10
-    assert rel(ctx.get_modes(ovl.PSI), p) < tol
-    assert rel(ctx.get_modes(ovl.PSIMID), pm) < tol
+    err = rel(ctx.get_modes(ovl.PSI), p)
+    assert err < tol
+    err = rel(ctx.get_modes(ovl.PSIMID), pm)
+    assert err < tol
 
 
 def test_graph_replay_matches_stream(gpu):
@@ -226,7 +239,8 @@ def test_observables_vs_oracle(gpu, nc):
     assert abs(ctx.ncs_dot() - orc.edotb(L, links, e) / (8 * np.pi ** 2)) < 1e-12 * abs(orc.edotb(L, links, e))
     # one E update
     ctx.update_efield(0.03)
-    assert rel(ctx.get_efield(), orc.efield_update(L, links, e, 0.03)) < 1e-13
+    err = rel(ctx.get_efield(), orc.efield_update(L, links, e, 0.03))
+    assert err < 1e-13
 
 
 def test_currents_vs_oracle(gpu):
@@ -238,7 +252,8 @@ def test_currents_vs_oracle(gpu):
     coef = orc.cheb_coeffs(deg, cut)
     h = np.stack([orc.overlap_evo_cheb(L, ucc, orc.norm_bound(L), cut, deg, coef, p) for p in psi])
     jv0, ja0 = orc.currents(L, psi, h, first_mode=3)
-    assert np.abs(jv - jv0).max() < 1e-14 and np.abs(ja - ja0).max() < 1e-14
+    ev, ea = np.abs(jv - jv0).max(), np.abs(ja - ja0).max()
+    assert ev < 1e-14 and ea < 1e-14
     assert abs(q - ja0[0::4].sum()) < 1e-13
 
 
