@@ -142,6 +142,11 @@ def main():
     if alg_step_bytes:
         roofline["step_alg_bytes"] = alg_step_bytes
         roofline["step_alg_frac"] = round(alg_step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+    roofline["variants"] = {
+        names[i]: {"launches": int(t.var_launches[i]), "ms": round(t.var_ms[i], 3),
+                   "GBps": round(t.var_bytes[i] / max(t.var_ms[i], 1e-9) / 1e6, 1)}
+        for i in range(3) if t.var_launches[i]}
+    log("variants", json.dumps(roofline["variants"]))
 
     # --- graph-captured replay of the same step
     sog = step_opts(c, prec_mixed, True, args.tol, args.chunk)

@@ -73,16 +73,17 @@ def test_ritz_reference_golden(gpu):
     n = tuple(int(v) for v in f["n"])
     r, m, M = f["params"]
     ucc = f["ucc"]
-    # the drop-in stencil comes from u_cc; rebuild master links from it (GetGaugeLink)
+    # master links from u_cc, inverting GetMatrix (SU2/GroupOperations.cpp:213-225); the reference's
+    # own inverse GetGaugeLink (:227-239) returns -u2 in component 2, so it is not used here
     V = int(np.prod(n))
     u = ucc.reshape(3, V, 2, 2)
     links = np.empty((V, 3, 4))
     for mu in range(3):
         mm = u[mu]
-        links[:, mu, 0] = (0.5j * (-mm[:, 1, 0] - mm[:, 0, 1])).real
-        links[:, mu, 1] = (0.5 * (-mm[:, 1, 0] + mm[:, 0, 1])).real
-        links[:, mu, 2] = (0.5j * (mm[:, 0, 0] - mm[:, 1, 1])).real
-        links[:, mu, 3] = (0.5 * (mm[:, 0, 0] + mm[:, 1, 1])).real
+        links[:, mu, 0] = mm[:, 0, 1].imag
+        links[:, mu, 1] = mm[:, 0, 1].real
+        links[:, mu, 2] = mm[:, 0, 0].imag
+        links[:, mu, 3] = mm[:, 0, 0].real
     with ovl.Context(2, n, 1, r_wilson=r, m_fermion=m, m_dw=M) as ctx:
         ctx.set_links(links.reshape(-1))
         lo = ctx.ritz(f["start"], +1.0)
