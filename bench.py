@@ -88,6 +88,10 @@ def main():
     ctx.set_modes(ovl.PSI, psi)
     ctx.set_modes(ovl.PSIMID, psi)
     del psi
+    # a plain copy of one FP64 field (k_lincomb<double>, known bytes): the calibration point of the
+    # FETCH_SIZE / WRITE_SIZE counters in profiles/
+    ctx.copy_field(ovl.PSI, ovl.SCRATCH)
+    log(f"calibration copy: {c.volume * c.nmodes * 4 * c.nc * 16} bytes read and written")
 
     # --- accuracy of the mixed form against the plain FP64 form (one overlap apply, all modes)
     prec_mixed = ovl.PREC_MIXED
