@@ -63,8 +63,11 @@ def main():
     ap.add_argument("--modes", type=int, default=0, help="override the mode count (test runs only)")
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--chunk", type=int, default=0, help="mode blocks per recurrence pass (0 = all)")
+    ap.add_argument("--mb", type=int, default=0, help="modes per SoA block (16/32/64, 0 = choose)")
+    ap.add_argument("--traversal", type=int, default=0, help="0 = XCD boxes, 1 = lexicographic")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--quick", action="store_true", help="timed region only (layout / tile sweeps)")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--cpu-modes", type=int, default=16)
     args = ap.parse_args()
@@ -80,7 +83,8 @@ def main():
     t0 = time.perf_counter()
     links, e, psi = c.generate()
     log(f"inputs generated in {time.perf_counter() - t0:.1f}s")
-    ctx = ovl.Context(c.nc, c.n, c.nmodes, r_wilson=c.r_wilson, m_fermion=c.m_fermion, m_dw=c.m_dw)
+    ctx = ovl.Context(c.nc, c.n, c.nmodes, r_wilson=c.r_wilson, m_fermion=c.m_fermion, m_dw=c.m_dw,
+                      mode_block=args.mb, traversal=args.traversal)
     ctx.set_links(links)
     ctx.set_efield(e)
     if c.nb:
@@ -98,7 +102,9 @@ def main():
     so64 = step_opts(c, ovl.PREC_FP64, False).sign
     som = step_opts(c, prec_mixed, False, args.tol, args.chunk).sign
     res = {}
-    if c.kind == "overlap":
+    if args.quick:
+        args.no_fp64 = args.no_cpu = True
+    if c.kind == "overlap" and not args.quick:
         ctx.apply_overlap(ovl.PSI, ovl.SCRATCH, so64)
         ref = ctx.get_modes(ovl.SCRATCH, 0, min(16, c.nmodes))
         ctx.apply_overlap(ovl.PSI, ovl.SCRATCH, som)
@@ -153,11 +159,13 @@ def main():
     log("variants", json.dumps(roofline["variants"]))
 
     # --- graph-captured replay of the same step
-    sog = step_opts(c, prec_mixed, True, args.tol, args.chunk)
-    t0 = time.perf_counter()
-    ctx.fermion_update(sog, args.steps)
-    graph_ms = 1e3 * (time.perf_counter() - t0) / args.steps
-    log(f"mixed (hipGraph): {graph_ms:.1f} ms/step")
+    graph_ms = None
+    if not args.quick:
+        sog = step_opts(c, prec_mixed, True, args.tol, args.chunk)
+        t0 = time.perf_counter()
+        ctx.fermion_update(sog, args.steps)
+        graph_ms = 1e3 * (time.perf_counter() - t0) / args.steps
+        log(f"mixed (hipGraph): {graph_ms:.1f} ms/step")
 
     # --- plain FP64 form
     fp64 = None
@@ -172,7 +180,7 @@ def main():
         log(f"fp64: {fp64['ms_per_step']:.1f} ms/step")
 
     # --- sign-function residual of the mixed form on the current state
-    if c.kind == "overlap":
+    if c.kind == "overlap" and not args.quick:
         res["sign_residual_mixed"] = ctx.sign_residual(ovl.PSI, som)
         log(f"sign residual (mixed) {res['sign_residual_mixed']:.3e}")
     ctx.close()
@@ -192,7 +200,8 @@ def main():
            "vs_baseline": None, "dtype": "fp32+fp64 (mixed)", "data": "synthetic",
            "config": {"workload": c.name, "nc": c.nc, "lattice": list(c.n), "modes": c.nmodes,
                       "chebyshev_degree": c.cheb_n, "cut": c.cheb_cut, "rho": c.m_dw, "flux_quanta": c.nb,
-                      "dt": c.dt, "tol": args.tol, "parallelism": "single GPU"},
+                      "dt": c.dt, "tol": args.tol, "mode_block": args.mb, "mode_chunk": args.chunk,
+                      "traversal": args.traversal, "parallelism": "single GPU"},
            "roofline": roofline, "cpu_baseline": cpu, "fp64_gpu": fp64,
            "speedup_vs_fp64": (value / fp64["value"]) if fp64 else None,
            "graph_ms_per_step": graph_ms, "accuracy": res}

@@ -56,6 +56,8 @@ typedef struct {
     double m_dw;         /* MDWHeight (Simulation.cpp:200), the overlap rho */
     int mode_block;      /* modes per SoA block: 16, 32 or 64; 0 = choose */
     int reunitarize;     /* 1: project links back onto SU(Nc) after every link update */
+    int traversal;       /* stencil site order: 0 = one spatial box per XCD when the lattice splits
+                            into 8 boxes, 1 = lexicographic */
 } ovl_params;
 
 /* sign function eps(h_W) options */

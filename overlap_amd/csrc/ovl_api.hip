@@ -52,6 +52,7 @@ struct ovl_ctx {
     // reductions / small scratch
     double *partial = nullptr;
     size_t partial_len = 0;
+    int *xcd_order = nullptr;  // device site traversal of the stencil launches (or null)
     double *dout = nullptr;  // 16 doubles
     double *gam = nullptr;   // current coupling matrices
     double *jv = nullptr, *ja = nullptr;
@@ -206,7 +207,48 @@ HopArgs hop_args(const ovl_ctx *c, int prec)
     a.links = prec == P64 ? (const void *)c->ul64 : (const void *)c->ul32;
     a.amass = amass(c);
     a.nblk = c->g.nblk;
+    a.order = c->xcd_order;
     return a;
+}
+
+// Site traversal for the XCD-aware stencil launches: the lattice is cut into 8 boxes (one per XCD,
+// px*py*pz = 8 dividing the extents); inside a box x runs fastest, then the shorter of the two
+// remaining box extents, so all three neighbour reuse distances stay short.  Empty when no split fits.
+std::vector<int> xcd_site_order(int n0, int n1, int n2)
+{
+    const int n[3] = {n0, n1, n2};
+    int best[3] = {0, 0, 0};
+    long best_cost = -1;
+    for (int px = 1; px <= 8; px *= 2)
+        for (int py = 1; px * py <= 8; py *= 2) {
+            const int pz = 8 / (px * py);
+            if (n0 % px || n1 % py || n2 % pz) continue;
+            // prefer compact boxes: minimise the box surface
+            const long bx = n0 / px, by = n1 / py, bz = n2 / pz;
+            const long cost = bx * by + by * bz + bz * bx;
+            if (best_cost < 0 || cost < best_cost) {
+                best_cost = cost;
+                best[0] = px; best[1] = py; best[2] = pz;
+            }
+        }
+    std::vector<int> order;
+    if (best_cost < 0) return order;
+    const int b[3] = {n[0] / best[0], n[1] / best[1], n[2] / best[2]};
+    const bool y_mid = b[1] <= b[2];
+    order.reserve((size_t)n0 * n1 * n2);
+    for (int kz = 0; kz < best[2]; ++kz)
+        for (int ky = 0; ky < best[1]; ++ky)
+            for (int kx = 0; kx < best[0]; ++kx) {
+                const int slow = y_mid ? b[2] : b[1], mid = y_mid ? b[1] : b[2];
+                for (int i = 0; i < slow; ++i)
+                    for (int j = 0; j < mid; ++j)
+                        for (int x = 0; x < b[0]; ++x) {
+                            const int y = y_mid ? j : i, z = y_mid ? i : j;
+                            const int gx = kx * b[0] + x, gy = ky * b[1] + y, gz = kz * b[2] + z;
+                            order.push_back(gx + n0 * (gy + n1 * gz));
+                        }
+            }
+    return order;
 }
 
 void begin_timing(ovl_ctx *c)
@@ -754,6 +796,14 @@ int ovl_create(const ovl_params *p, ovl_ctx **out)
     if ((rc = dalloc(c, (void **)&c->partial, sizeof(double) * c->partial_len)) ||
         (rc = dalloc(c, (void **)&c->dout, sizeof(double) * (16 + 2 * (size_t)c->g.nblk * mb))))
         return bail(rc);
+    if (p->traversal == 0) {
+        const std::vector<int> ord = xcd_site_order(p->n[0], p->n[1], p->n[2]);
+        if (!ord.empty()) {
+            if ((rc = dalloc(c, (void **)&c->xcd_order, sizeof(int) * ord.size()))) return bail(rc);
+            if (hipMemcpy(c->xcd_order, ord.data(), sizeof(int) * ord.size(), hipMemcpyHostToDevice) != hipSuccess)
+                return bail(OVL_E_HIP);
+        }
+    }
     if (hipStreamSynchronize(c->st) != hipSuccess) return bail(OVL_E_HIP);
     *out = c;
     return OVL_OK;
@@ -766,7 +816,7 @@ int ovl_destroy(ovl_ctx *c)
     void *ps[] = {c->links, c->efield, c->ul64, c->ul32, c->phase, c->field[0], c->field[1], c->field[2], c->w64, c->a64,
                   c->b64, c->x32, c->w32, c->a32, c->b32, c->partial, c->dout, c->gam, c->jv, c->ja, c->staging,
                   c->ms_ps, c->ms_r, c->ms_q, c->ms_w, c->ms_chi, c->ms_coef, c->ms_active, c->rz[0], c->rz[1],
-                  c->rz[2], c->rz[3], c->rz[4], c->rz[5]};
+                  c->rz[2], c->rz[3], c->rz[4], c->rz[5], c->xcd_order};
     for (void *p : ps)
         if (p) hipFree(p);
     for (auto e : c->evpool) hipEventDestroy(e);

@@ -38,7 +38,11 @@ struct HopArgs {
     double amass;     // A = 6r + 2m - 2M of h_W = (1/2) gamma0 [A - hopping]
     int x_gamma0;     // 1: G = gamma0
     int nblk;         // blocks covered by this launch
+    const int *order; // XCD traversal: V sites, XCD k owns entries [k V/8, (k+1) V/8); null = lexicographic
 };
+
+// number of XCDs the hardware round-robins workgroups over (MI355X: 8)
+constexpr int NUM_XCD = 8;
 
 enum Prec { P64 = 0, P32 = 1 };
 

@@ -46,6 +46,37 @@ __device__ __forceinline__ void load_sp(const void *__restrict__ base, int V, in
     }
 }
 
+// streaming (non-temporal) variants for operands read or written exactly once per launch, so that
+// they do not evict the stencil input whose neighbours are re-read from L2
+typedef double d2v __attribute__((ext_vector_type(2)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+template <int NC, int MB, typename T>
+__device__ __forceinline__ void load_sp_nt(const void *__restrict__ base, int V, int blk, int s, int m, T *re, T *im)
+{
+    constexpr int NCOMP = 4 * NC;
+    if constexpr (sizeof(T) == 8) {
+        const d2v *p = reinterpret_cast<const d2v *>(base) + ((size_t)(blk * V + s) * NCOMP) * MB + m;
+#pragma unroll
+        for (int c = 0; c < NCOMP; ++c) {
+            const d2v t = __builtin_nontemporal_load(p + (size_t)c * MB);
+            re[c] = t.x;
+            im[c] = t.y;
+        }
+    } else {
+        const f4v *p = reinterpret_cast<const f4v *>(base) + ((size_t)(blk * V + s) * (NCOMP / 2)) * MB + m;
+#pragma unroll
+        for (int c = 0; c < NCOMP / 2; ++c) {
+            const f4v t = __builtin_nontemporal_load(p + (size_t)c * MB);
+            re[2 * c] = t.x;
+            im[2 * c] = t.y;
+            re[2 * c + 1] = t.z;
+            im[2 * c + 1] = t.w;
+        }
+    }
+}
+
+// the output is read by the very next launch, so it is stored normally (it may stay in the MALL)
 template <int NC, int MB, typename T>
 __device__ __forceinline__ void store_sp(void *__restrict__ base, int V, int blk, int s, int m, const T *re,
                                          const T *im)
@@ -61,6 +92,34 @@ __device__ __forceinline__ void store_sp(void *__restrict__ base, int V, int blk
         for (int c = 0; c < NCOMP / 2; ++c)
             p[(size_t)c * MB] = make_float4(re[2 * c], im[2 * c], re[2 * c + 1], im[2 * c + 1]);
     }
+}
+
+// Lane -> (mode m, site s, block blk).  With an XCD site order the hardware's round-robin of
+// workgroups over the 8 XCDs is undone: workgroup b runs on XCD b % 8, and XCD k walks its own
+// contiguous eighth of the traversal (its spatial box of sites, every mode block in turn), so the
+// x/y/z neighbours of a site are re-read from that XCD's L2.
+template <int MB>
+__device__ __forceinline__ bool lane_coords(const int *__restrict__ order, int V, int nblk, int &m, int &s, int &blk)
+{
+    const long total = (long)nblk * V * MB;
+    long lb = blockIdx.x;
+    if (order) lb = (long)(blockIdx.x % NUM_XCD) * (gridDim.x / NUM_XCD) + blockIdx.x / NUM_XCD;
+    const long gl = lb * TPB + threadIdx.x;
+    if (gl >= total) return false;
+    m = (int)(gl % MB);
+    const long r = gl / MB;
+    if (order) {
+        const long per = (long)nblk * V / NUM_XCD;  // traversal slots per XCD
+        const int xc = (int)(r / per);
+        const long rr = r - xc * per;
+        const int v8 = V / NUM_XCD;
+        blk = (int)(rr / v8);
+        s = order[xc * v8 + (int)(rr - (long)blk * v8)];
+    } else {
+        s = (int)(r % V);
+        blk = (int)(r / V);
+    }
+    return true;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -141,13 +200,8 @@ __global__ __launch_bounds__(TPB) void k_hop(HopArgs a, int n0, int n1, int n2)
     constexpr int NCOMP = 4 * NC;
     using LT = typename Vec<TI>::L;
     const int V = n0 * n1 * n2;
-    const long gl = (long)blockIdx.x * TPB + threadIdx.x;
-    const long total = (long)a.nblk * V * MB;
-    if (gl >= total) return;
-    const int m = (int)(gl % MB);
-    const long sl = gl / MB;
-    const int s = (int)(sl % V);
-    const int blk = (int)(sl / V);
+    int m, s, blk;
+    if (!lane_coords<MB>(a.order, V, a.nblk, m, s, blk)) return;
     const int z = s / (n0 * n1);
     const int rem = s - z * n0 * n1;
     const int y = rem / n0;
@@ -200,7 +254,7 @@ __global__ __launch_bounds__(TPB) void k_hop(HopArgs a, int n0, int n1, int n2)
     }
     TO tr[NCOMP], ti[NCOMP];
     if (a.x) {
-        load_sp<NC, MB, TO>(a.x, V, blk, s, m, tr, ti);
+        load_sp_nt<NC, MB, TO>(a.x, V, blk, s, m, tr, ti);
         const TO cr = (TO)a.sx_r, ci = (TO)a.sx_i;
 #pragma unroll
         for (int c = 0; c < NCOMP; ++c) {
@@ -211,19 +265,19 @@ __global__ __launch_bounds__(TPB) void k_hop(HopArgs a, int n0, int n1, int n2)
         }
     }
     if (a.b1) {
-        load_sp<NC, MB, TO>(a.b1, V, blk, s, m, tr, ti);
+        load_sp_nt<NC, MB, TO>(a.b1, V, blk, s, m, tr, ti);
         const TO cr = (TO)a.s1_r, ci = (TO)a.s1_i;
 #pragma unroll
         for (int c = 0; c < NCOMP; ++c) { orr[c] += cr * tr[c] - ci * ti[c]; oi[c] += cr * ti[c] + ci * tr[c]; }
     }
     if (a.b2) {
-        load_sp<NC, MB, TO>(a.b2, V, blk, s, m, tr, ti);
+        load_sp_nt<NC, MB, TO>(a.b2, V, blk, s, m, tr, ti);
         const TO cr = (TO)a.s2_r, ci = (TO)a.s2_i;
 #pragma unroll
         for (int c = 0; c < NCOMP; ++c) { orr[c] += cr * tr[c] - ci * ti[c]; oi[c] += cr * ti[c] + ci * tr[c]; }
     }
     if (a.tgt) {
-        load_sp<NC, MB, TO>(a.tgt, V, blk, s, m, tr, ti);
+        load_sp_nt<NC, MB, TO>(a.tgt, V, blk, s, m, tr, ti);
         const TO cr = (TO)a.st_r, ci = (TO)a.st_i;
 #pragma unroll
         for (int c = 0; c < NCOMP; ++c) { orr[c] += cr * tr[c] - ci * ti[c]; oi[c] += cr * ti[c] + ci * tr[c]; }
@@ -1039,10 +1093,13 @@ hipError_t hop_mb(const Geom &g, const HopArgs &a, hipStream_t st)
 {
     const long total = (long)a.nblk * g.V * g.mb;
     const unsigned nb = nblocks(total);
+    HopArgs b = a;
+    // the XCD traversal needs whole workgroups spread evenly over the XCDs
+    if (total % ((long)TPB * NUM_XCD) != 0 || g.V % NUM_XCD != 0) b.order = nullptr;
     switch (g.mb) {
-    case 16: k_hop<NC, TI, TO, 16><<<nb, TPB, 0, st>>>(a, g.n0, g.n1, g.n2); break;
-    case 32: k_hop<NC, TI, TO, 32><<<nb, TPB, 0, st>>>(a, g.n0, g.n1, g.n2); break;
-    case 64: k_hop<NC, TI, TO, 64><<<nb, TPB, 0, st>>>(a, g.n0, g.n1, g.n2); break;
+    case 16: k_hop<NC, TI, TO, 16><<<nb, TPB, 0, st>>>(b, g.n0, g.n1, g.n2); break;
+    case 32: k_hop<NC, TI, TO, 32><<<nb, TPB, 0, st>>>(b, g.n0, g.n1, g.n2); break;
+    case 64: k_hop<NC, TI, TO, 64><<<nb, TPB, 0, st>>>(b, g.n0, g.n1, g.n2); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -35,7 +35,7 @@ SYMBOLS = [
 class Params(C.Structure):
     _fields_ = [("nc", C.c_int), ("n", C.c_int * 3), ("n_modes", C.c_int), ("r_wilson", C.c_double),
                 ("m_fermion", C.c_double), ("m_dw", C.c_double), ("mode_block", C.c_int),
-                ("reunitarize", C.c_int)]
+                ("reunitarize", C.c_int), ("traversal", C.c_int)]
 
 
 class SignOpts(C.Structure):
@@ -111,10 +111,11 @@ def device_name() -> str:
 class Context:
     """One device context: gauge field, two mode-function fields (Psi, PsiMid) and scratch."""
 
-    def __init__(self, nc, n, n_modes, r_wilson=1.0, m_fermion=0.0, m_dw=0.0, mode_block=0, reunitarize=0):
+    def __init__(self, nc, n, n_modes, r_wilson=1.0, m_fermion=0.0, m_dw=0.0, mode_block=0, reunitarize=0,
+                 traversal=0):
         self.nc, self.n, self.n_modes = nc, tuple(n), n_modes
         self.V = int(np.prod(n))
-        p = Params(nc, (C.c_int * 3)(*n), n_modes, r_wilson, m_fermion, m_dw, mode_block, reunitarize)
+        p = Params(nc, (C.c_int * 3)(*n), n_modes, r_wilson, m_fermion, m_dw, mode_block, reunitarize, traversal)
         h = C.c_void_p()
         rc = lib().ovl_create(C.byref(p), C.byref(h))
         if rc != OK:

@@ -51,12 +51,12 @@ def test_sign_defaults():
 
 
 def test_create_rejects_bad_params_without_touching_device():
-    p = ovl.Params(4, (C.c_int * 3)(8, 8, 8), 16, 1.0, 0.0, 0.0, 0, 0)
+    p = ovl.Params(4, (C.c_int * 3)(8, 8, 8), 16, 1.0, 0.0, 0.0, 0, 0, 0)
     h = C.c_void_p()
     assert ovl.lib().ovl_create(C.byref(p), C.byref(h)) == ovl.E_ARG
-    p = ovl.Params(2, (C.c_int * 3)(8, 2, 8), 16, 1.0, 0.0, 0.0, 0, 0)
+    p = ovl.Params(2, (C.c_int * 3)(8, 2, 8), 16, 1.0, 0.0, 0.0, 0, 0, 0)
     assert ovl.lib().ovl_create(C.byref(p), C.byref(h)) == ovl.E_ARG
-    p = ovl.Params(2, (C.c_int * 3)(8, 8, 8), 16, 1.0, 0.0, 0.0, 48, 0)
+    p = ovl.Params(2, (C.c_int * 3)(8, 8, 8), 16, 1.0, 0.0, 0.0, 48, 0, 0)
     assert ovl.lib().ovl_create(C.byref(p), C.byref(h)) == ovl.E_ARG
 
 

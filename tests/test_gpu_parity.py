@@ -278,3 +278,24 @@ def test_error_paths(gpu):
     with pytest.raises(ovl.OvlError) as ex:
         ctx.apply_overlap(ovl.PSI, ovl.PSIMID, ovl.sign_opts(method=ovl.SIGN_ZOLOTAREV, lmin=2.0, lmax=1.0))
     assert ex.value.code == ovl.E_SPECTRUM
+
+
+@pytest.mark.parametrize("nc,n,nm,mb", [(3, (4, 4, 4), 64, 64), (2, (8, 4, 6), 32, 16), (3, (6, 4, 4), 64, 32)])
+def test_xcd_traversal_is_a_permutation(gpu, nc, n, nm, mb):
+    # the XCD box traversal only reorders lanes: results equal the lexicographic launch bit for bit
+    outs = []
+    for trav in (0, 1):
+        g = cf.rng(404)
+        links = cf.su2_links(n, 0.4, g) if nc == 2 else cf.su3_links(n, 0.4, g)
+        psi = cf.spinors(n, nc, nm, g)
+        with ovl.Context(nc, n, nm, m_fermion=0.1, m_dw=1.2, mode_block=mb, traversal=trav) as ctx:
+            ctx.set_links(links)
+            ctx.set_modes(ovl.PSI, psi)
+            ctx.apply_sign(ovl.PSI, ovl.PSIMID, ovl.sign_opts(degree=32, cut=0.1, precision=ovl.PREC_MIXED, head=8))
+            outs.append(ctx.get_modes(ovl.PSIMID))
+    assert np.array_equal(outs[0], outs[1])
+    L = orc.lat(nc, n, 1.0, 0.1, 1.2)
+    ucc = orc.links_to_ucc(L, links)
+    coef = orc.cheb_coeffs(32, 0.1)
+    err = rel(outs[0][nm - 1], orc.sign_cheb(L, ucc, orc.norm_bound(L), 0.1, 32, coef, psi[nm - 1]))
+    assert err < 1e-5
