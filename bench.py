@@ -116,6 +116,11 @@ def main():
         del ref, mix
         log(f"mixed vs fp64 rel err {res['rel_err_mixed_vs_fp64']:.3e}  head {tm.fp64_steps} tail {tm.fp32_steps}")
 
+    # --- occupancy autotune of the stencil kernel for this shape
+    tune = ctx.autotune()
+    log("autotune", tune)
+    res["autotune"] = tune
+
     # --- warmup + timed region: mixed form, per-launch HIP events on the library stream
     so = step_opts(c, prec_mixed, False, args.tol, args.chunk)
     for _ in range(args.warmup):

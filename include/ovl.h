@@ -163,6 +163,11 @@ typedef struct {
 int ovl_get_timing(ovl_ctx *ctx, ovl_timing *t);
 int ovl_set_profiling(ovl_ctx *ctx, int on);
 
+/* ---- tuning (no counterpart in the reference) ----
+ * times every occupancy variant of the stencil kernel on this context's lattice and modes, for
+ * FP64 and FP32, keeps the fastest for later calls and writes a one-line report */
+int ovl_autotune(ovl_ctx *ctx, char *report, int len);
+
 #ifdef __cplusplus
 }
 #endif

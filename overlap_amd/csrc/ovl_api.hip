@@ -81,6 +81,7 @@ struct ovl_ctx {
     ovl_timing last{};
     hipEvent_t t0 = nullptr, t1 = nullptr;
     bool capturing = false;
+    int wpe[2] = {1, 1};  // stencil occupancy variant per input precision (P64, P32), set by ovl_autotune
 };
 
 namespace {
@@ -191,7 +192,7 @@ int hop(ovl_ctx *c, const Geom &g, int pin, int pout, const HopArgs &a)
         c->evvar[c->evused / 2 - 1] = var;
         hipEventRecord(e0, c->st);
     }
-    HIPCHK(c, ovl::launch_hop(g, pin, pout, a, c->st));
+    HIPCHK(c, ovl::launch_hop(g, pin, pout, a, c->wpe[pin], c->st));
     if (e1) hipEventRecord(e1, c->st);
     const double b = hop_alg_bytes(c, a, pin, pout);
     c->hop_launches++;
@@ -1526,6 +1527,74 @@ int ovl_set_profiling(ovl_ctx *c, int on)
 {
     if (!c) return OVL_E_ARG;
     c->profiling = on != 0;
+    return OVL_OK;
+}
+
+int ovl_autotune(ovl_ctx *c, char *report, int len)
+{
+    // Times the Clenshaw step (pure stencil + epilogue stencil) of every occupancy variant of the
+    // stencil kernel, for both precisions, on this context's lattice and mode count, and keeps the
+    // fastest per precision.  Field contents are left unchanged except the work buffers.
+    if (!c) return OVL_E_ARG;
+    RETCHK(ensure_stencil(c));
+    RETCHK(ensure_work(c, c->g.nblk, true));
+    HIPCHK(c, ovl::launch_convert(c->g, c->g.nblk, c->field[OVL_PSI], P64, c->x32, P32, c->st));
+    HIPCHK(c, ovl::launch_convert(c->g, c->g.nblk, c->field[OVL_PSIMID], P64, c->a32, P32, c->st));
+    HIPCHK(c, ovl::launch_convert(c->g, c->g.nblk, c->field[OVL_PSI], P64, c->b32, P32, c->st));
+    HIPCHK(c, ovl::launch_lincomb(c->g, c->g.nblk, P64, c->a64, c->field[OVL_PSIMID], 1, 0, nullptr, 0, 0, c->st));
+    HIPCHK(c, ovl::launch_lincomb(c->g, c->g.nblk, P64, c->b64, c->field[OVL_PSI], 1, 0, nullptr, 0, 0, c->st));
+    const int variants[4] = {1, 2, 3, 4};
+    std::string rep;
+    const bool prof = c->profiling;
+    c->profiling = false;
+    for (int prec = P64; prec <= P32; ++prec) {
+        void *x = prec == P64 ? c->field[OVL_PSI] : c->x32;
+        void *b1 = prec == P64 ? c->a64 : c->a32, *b2 = prec == P64 ? c->b64 : c->b32;
+        void *w = prec == P64 ? c->w64 : c->w32;
+        double best = 1e30;
+        int best_v = 1;
+        for (int v : variants) {
+            c->wpe[prec] = v;
+            float ms = 0;
+            for (int rep_i = 0; rep_i < 2; ++rep_i) {  // first pass warms up
+                HIPCHK(c, hipEventRecord(c->t0, c->st));
+                for (int k = 0; k < 3; ++k) {
+                    HopArgs a = hop_args(c, prec);
+                    a.in = b1;
+                    a.out = w;
+                    a.sh_r = 1.0;
+                    RETCHK(hop(c, c->g, prec, prec, a));
+                    HopArgs q = hop_args(c, prec);
+                    q.in = w;
+                    q.out = b2;
+                    q.sh_r = 0.1;
+                    q.x = x;
+                    q.sx_r = 0.01;
+                    q.b1 = b1;
+                    q.s1_r = 0.5;
+                    q.b2 = b2;
+                    q.s2_r = -0.5;
+                    RETCHK(hop(c, c->g, prec, prec, q));
+                }
+                HIPCHK(c, hipEventRecord(c->t1, c->st));
+                HIPCHK(c, hipEventSynchronize(c->t1));
+                HIPCHK(c, hipEventElapsedTime(&ms, c->t0, c->t1));
+            }
+            char buf[96];
+            std::snprintf(buf, sizeof buf, "%s wpe=%d %.1f us/step; ", prec == P64 ? "fp64" : "fp32", v, 1e3 * ms / 3);
+            rep += buf;
+            if (ms < best) {
+                best = ms;
+                best_v = v;
+            }
+        }
+        c->wpe[prec] = best_v;
+    }
+    c->profiling = prof;
+    char buf[64];
+    std::snprintf(buf, sizeof buf, "chosen fp64 wpe=%d fp32 wpe=%d", c->wpe[P64], c->wpe[P32]);
+    rep += buf;
+    if (report && len > 0) std::snprintf(report, len, "%s", rep.c_str());
     return OVL_OK;
 }
 

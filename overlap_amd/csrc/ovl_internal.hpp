@@ -47,7 +47,9 @@ constexpr int NUM_XCD = 8;
 enum Prec { P64 = 0, P32 = 1 };
 
 // launchers (ovl_kernels.hip); all enqueue on `st` and return hipError_t
-hipError_t launch_hop(const Geom &g, int pin, int pout, const HopArgs &a, hipStream_t st);
+// wpe: occupancy variant of the stencil kernel (1 = compiler default, 2/3/4 = at least that many
+// waves per SIMD)
+hipError_t launch_hop(const Geom &g, int pin, int pout, const HopArgs &a, int wpe, hipStream_t st);
 hipError_t launch_convert(const Geom &g, int nblk, const void *src, int psrc, void *dst, int pdst, hipStream_t st);
 // out = ca * a + cb * b (same precision p), b may be null
 hipError_t launch_lincomb(const Geom &g, int nblk, int p, void *out, const void *a, double ca_r, double ca_i,

@@ -194,14 +194,24 @@ __device__ __forceinline__ void hop_dir(const T *pr, const T *pi, const LT *__re
 // stencil kernel: out = sh*h_W(in) + sx*G(x) + s1*b1 + s2*b2 + st*tgt,
 // h_W = (1/2) gamma0 [A in - sum_k (1+i gamma_k) U in(x+k) + (1-i gamma_k) U^+ in(x-k)]
 // (WilsonOperator.cpp:30-114 computes -i h_W of one mode; here all modes of `nblk` blocks)
-template <int NC, typename TI, typename TO, int MB>
-__global__ __launch_bounds__(TPB) void k_hop(HopArgs a, int n0, int n1, int n2)
+// WPE: minimum waves per SIMD the register allocation must allow (1 = compiler's choice); the
+// stencil hoists its neighbour loads, so this trades memory-level parallelism per wave against
+// waves in flight.  The host picks the variant per problem shape (autotune).
+template <int NC, typename TI, typename TO, int MB, int WPE>
+__global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(WPE))) void k_hop(HopArgs a, int n0, int n1,
+                                                                                       int n2)
 {
     constexpr int NCOMP = 4 * NC;
     using LT = typename Vec<TI>::L;
     const int V = n0 * n1 * n2;
     int m, s, blk;
     if (!lane_coords<MB>(a.order, V, a.nblk, m, s, blk)) return;
+    if constexpr (MB == 64) {
+        // one site per wavefront: make the site wave-uniform so that the links (shared by all
+        // modes of the site) come through scalar loads instead of 64 identical vector lanes
+        s = __builtin_amdgcn_readfirstlane(s);
+        blk = __builtin_amdgcn_readfirstlane(blk);
+    }
     const int z = s / (n0 * n1);
     const int rem = s - z * n0 * n1;
     const int y = rem / n0;
@@ -1088,8 +1098,19 @@ __global__ __launch_bounds__(TPB) void k_axpy_modes(double2 *__restrict__ y, con
 
 inline unsigned nblocks(long n) { return (unsigned)((n + TPB - 1) / TPB); }
 
+template <int NC, typename TI, typename TO, int MB>
+void hop_wpe(const HopArgs &b, unsigned nb, int wpe, const Geom &g, hipStream_t st)
+{
+    switch (wpe) {
+    case 2: k_hop<NC, TI, TO, MB, 2><<<nb, TPB, 0, st>>>(b, g.n0, g.n1, g.n2); break;
+    case 3: k_hop<NC, TI, TO, MB, 3><<<nb, TPB, 0, st>>>(b, g.n0, g.n1, g.n2); break;
+    case 4: k_hop<NC, TI, TO, MB, 4><<<nb, TPB, 0, st>>>(b, g.n0, g.n1, g.n2); break;
+    default: k_hop<NC, TI, TO, MB, 1><<<nb, TPB, 0, st>>>(b, g.n0, g.n1, g.n2); break;
+    }
+}
+
 template <int NC, typename TI, typename TO>
-hipError_t hop_mb(const Geom &g, const HopArgs &a, hipStream_t st)
+hipError_t hop_mb(const Geom &g, const HopArgs &a, int wpe, hipStream_t st)
 {
     const long total = (long)a.nblk * g.V * g.mb;
     const unsigned nb = nblocks(total);
@@ -1097,9 +1118,9 @@ hipError_t hop_mb(const Geom &g, const HopArgs &a, hipStream_t st)
     // the XCD traversal needs whole workgroups spread evenly over the XCDs
     if (total % ((long)TPB * NUM_XCD) != 0 || g.V % NUM_XCD != 0) b.order = nullptr;
     switch (g.mb) {
-    case 16: k_hop<NC, TI, TO, 16><<<nb, TPB, 0, st>>>(b, g.n0, g.n1, g.n2); break;
-    case 32: k_hop<NC, TI, TO, 32><<<nb, TPB, 0, st>>>(b, g.n0, g.n1, g.n2); break;
-    case 64: k_hop<NC, TI, TO, 64><<<nb, TPB, 0, st>>>(b, g.n0, g.n1, g.n2); break;
+    case 16: hop_wpe<NC, TI, TO, 16>(b, nb, wpe, g, st); break;
+    case 32: hop_wpe<NC, TI, TO, 32>(b, nb, wpe, g, st); break;
+    case 64: hop_wpe<NC, TI, TO, 64>(b, nb, wpe, g, st); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -1110,16 +1131,16 @@ hipError_t hop_mb(const Geom &g, const HopArgs &a, hipStream_t st)
 // ---------------------------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------------------------
-hipError_t launch_hop(const Geom &g, int pin, int pout, const HopArgs &a, hipStream_t st)
+hipError_t launch_hop(const Geom &g, int pin, int pout, const HopArgs &a, int wpe, hipStream_t st)
 {
     if (g.nc == 2) {
-        if (pin == P64 && pout == P64) return hop_mb<2, double, double>(g, a, st);
-        if (pin == P32 && pout == P32) return hop_mb<2, float, float>(g, a, st);
-        if (pin == P32 && pout == P64) return hop_mb<2, float, double>(g, a, st);
+        if (pin == P64 && pout == P64) return hop_mb<2, double, double>(g, a, wpe, st);
+        if (pin == P32 && pout == P32) return hop_mb<2, float, float>(g, a, wpe, st);
+        if (pin == P32 && pout == P64) return hop_mb<2, float, double>(g, a, wpe, st);
     } else if (g.nc == 3) {
-        if (pin == P64 && pout == P64) return hop_mb<3, double, double>(g, a, st);
-        if (pin == P32 && pout == P32) return hop_mb<3, float, float>(g, a, st);
-        if (pin == P32 && pout == P64) return hop_mb<3, float, double>(g, a, st);
+        if (pin == P64 && pout == P64) return hop_mb<3, double, double>(g, a, wpe, st);
+        if (pin == P32 && pout == P32) return hop_mb<3, float, float>(g, a, wpe, st);
+        if (pin == P32 && pout == P64) return hop_mb<3, float, double>(g, a, wpe, st);
     }
     return hipErrorInvalidValue;
 }

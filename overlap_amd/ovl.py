@@ -28,7 +28,7 @@ SYMBOLS = [
     "ovl_wilson_evo_compute", "ovl_overlap_operator_compute", "ovl_zolotarev_coefficients",
     "ovl_update_links", "ovl_update_efield", "ovl_fermion_update",
     "ovl_energy", "ovl_gauss_violation", "ovl_unitarity_violation", "ovl_ncs_dot", "ovl_currents",
-    "ovl_mode_norms", "ovl_get_timing", "ovl_set_profiling",
+    "ovl_mode_norms", "ovl_get_timing", "ovl_set_profiling", "ovl_autotune",
 ]
 
 
@@ -267,3 +267,8 @@ class Context:
 
     def set_profiling(self, on: bool):
         self._chk(lib().ovl_set_profiling(self.h, int(on)))
+
+    def autotune(self) -> str:
+        buf = C.create_string_buffer(1024)
+        self._chk(lib().ovl_autotune(self.h, buf, 1024))
+        return buf.value.decode()
