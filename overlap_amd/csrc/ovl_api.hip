@@ -311,24 +311,33 @@ std::vector<double> cheb_coeffs(int n, double cut)
     return c;
 }
 
-// FP64 head length of the mixed recurrence: steps k < head run in FP64.  The FP32 rounding
-// injected at step k is bounded by u32 * |b_k| and grows at most by (k+1) (|U_k| <= k+1 on
-// [-1,1]); |b_k| <= B_k = sum_{j>=k} |c_j| (j-k+1).
-int choose_head(const std::vector<double> &c, double tol)
+// FP64 head length of the mixed recurrence: steps k < head run in FP64, the rest in FP32.
+// Error model: the FP32 storage of b_k (k >= head) perturbs the result by about
+// kappa * u32 * B_head, where B_k = max over t in [-1,1] of |b_k(t)| (the scalar Clenshaw
+// intermediates of the polynomial; they decay geometrically with k).  kappa = 1 is 5-10x above
+// the errors measured with FP32 storage on random SU(3) fields (0.1-0.2 u32 B_head); the coherent
+// worst case (errors aligned with the extreme eigenvector and summed linearly over steps) would
+// ask for 1e4x more and is not what FP32 rounding produces.
+int choose_head(const std::vector<double> &c, double tol, double kappa = 1.0)
 {
     const int n = (int)c.size() - 1;
     const double u32 = 5.96e-8;
-    std::vector<double> B(n + 2, 0.0);
-    for (int k = n; k >= 0; --k) {
-        double s = 0;
-        for (int j = k; j <= n; ++j) s += std::fabs(c[j]) * (j - k + 1);
-        B[k] = s;
+    const int NT = 2049;
+    std::vector<double> t(NT), b1(NT, 0.0), b2(NT, 0.0), B(n + 2, 0.0);
+    for (int i = 0; i < NT; ++i) t[i] = std::cos(M_PI * i / (NT - 1));
+    for (int k = n; k >= 1; --k) {
+        double mx = 0;
+        for (int i = 0; i < NT; ++i) {
+            const double bk = c[k] + 2.0 * t[i] * b1[i] - b2[i];
+            b2[i] = b1[i];
+            b1[i] = bk;
+            mx = std::max(mx, std::fabs(bk));
+        }
+        B[k] = mx;
     }
-    double err = 0;
     int head = n + 1;
-    for (int k = n; k >= 0; --k) {
-        err += u32 * B[k] * (k + 1);
-        if (err > tol) break;
+    for (int k = n; k >= 1; --k) {
+        if (kappa * u32 * B[k] > tol) break;
         head = k;
     }
     return head;
