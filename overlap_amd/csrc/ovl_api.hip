@@ -314,11 +314,12 @@ std::vector<double> cheb_coeffs(int n, double cut)
 // FP64 head length of the mixed recurrence: steps k < head run in FP64, the rest in FP32.
 // Error model: the FP32 storage of b_k (k >= head) perturbs the result by about
 // kappa * u32 * B_head, where B_k = max over t in [-1,1] of |b_k(t)| (the scalar Clenshaw
-// intermediates of the polynomial; they decay geometrically with k).  kappa = 1 is 5-10x above
-// the errors measured with FP32 storage on random SU(3) fields (0.1-0.2 u32 B_head); the coherent
-// worst case (errors aligned with the extreme eigenvector and summed linearly over steps) would
-// ask for 1e4x more and is not what FP32 rounding produces.
-int choose_head(const std::vector<double> &c, double tol, double kappa = 1.0)
+// intermediates of the polynomial; they decay geometrically with k).  kappa = 2 is 10-20x above
+// the errors measured with FP32 storage on random SU(3) fields (0.1-0.2 u32 B_head; 0.7 u32 B_head
+// on a 6^3 lattice with 8 flux quanta); the coherent worst case (errors aligned with the extreme
+// eigenvector and summed linearly over steps) would ask for 1e4x more and is not what FP32
+// rounding produces.
+int choose_head(const std::vector<double> &c, double tol, double kappa = 2.0)
 {
     const int n = (int)c.size() - 1;
     const double u32 = 5.96e-8;
