@@ -13,8 +13,13 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import csv
 import platform
+import shutil
+import signal
+import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -54,6 +59,121 @@ def cpu_baseline(c: cf.Config, links, e, psi, threads: int, modes: int):
     return c.volume * modes / dt, dt
 
 
+def _pmc_pass(counter: str, child_args: list, outdir: str, timeout: float):
+    """One rocprofv3 counter pass over a short run of this script (a child process); returns the
+    dispatch rows of its counter CSV, or None."""
+    rp = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    d = os.path.join(outdir, counter.lower())
+    cmd = [rp, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "c", "--", sys.executable,
+           os.path.join(ROOT, "bench.py")] + child_args
+    env = dict(os.environ, TMPDIR="/tmp")
+    with open(os.path.join(outdir, counter.lower() + ".log"), "w") as lg:
+        p = subprocess.Popen(cmd, cwd="/tmp", env=env, stdout=lg, stderr=subprocess.STDOUT, start_new_session=True)
+        try:
+            p.wait(timeout=timeout)
+        except subprocess.TimeoutExpired:
+            os.killpg(p.pid, signal.SIGKILL)
+            p.wait()
+            log(f"{counter} pass timed out")
+            return None
+    f = os.path.join(d, "c_counter_collection.csv")
+    if not os.path.exists(f):  # the profiler may end badly after the program; use complete results only
+        log(f"{counter} pass: no counter file (rc {p.returncode})")
+        return None
+    with open(f) as fh:
+        return list(csv.DictReader(fh))
+
+
+def measure_traffic(args, c: cf.Config, prefix: str, n_last: int):
+    """HBM bytes per launch of the dominant stencil variant from FETCH_SIZE and WRITE_SIZE, each
+    scaled by what the same counter reads for the calibration copy of one FP64 field (known bytes
+    read and written).  The counted run is `bench.py --quick --steps 1` of the same workload; the
+    last `n_last` dispatches whose name starts with `prefix` are its timed step."""
+    child = ["--quick", "--steps", "1", "--warmup", "0", "--workload", c.name, "--mb", str(args.mb),
+             "--traversal", str(args.traversal), "--tol", str(args.tol), "--chunk", str(args.chunk)]
+    if args.modes:
+        child += ["--modes", str(args.modes)]
+    outdir = tempfile.mkdtemp(prefix="ovl_pmc_", dir="/tmp")
+    field_bytes = c.volume * c.nmodes * 4 * c.nc * 16
+    field_cplx = c.volume * c.nmodes * 4 * c.nc
+    res = {}
+    try:
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            log(f"traffic: {counter} pass")
+            rows = _pmc_pass(counter, child, outdir, 300)
+            if not rows:
+                return None
+            name = lambda r: r["Kernel_Name"].replace("void ", "").replace("ovl::(anonymous namespace)::", "")  # noqa
+            calib = [r for r in rows if name(r).startswith("k_lincomb<double>") and int(r["Grid_Size"]) == field_cplx]
+            hops = [r for r in rows if name(r).startswith(prefix)]
+            if not calib or len(hops) < n_last:
+                return None
+            factor = field_bytes / (float(calib[0]["Counter_Value"]) * 1024.0)
+            per = [float(r["Counter_Value"]) * 1024.0 * factor for r in hops[-n_last:]]
+            res[counter] = {"factor": factor, "bytes_per_launch": sum(per) / len(per),
+                            "kernel": name(hops[-1]).split("(")[0]}
+    finally:
+        shutil.rmtree(outdir, ignore_errors=True)
+    return res
+
+
+def bench_ensemble(ec: cf.EnsembleConfig, steps: int, warmup: int, tol: float):
+    """Many SU(2) overlap systems of uneven sizes in one context (one stencil launch per Clenshaw
+    step over all systems): mixed and FP64 timing from the same start, per-system error of the
+    mixed form against the FP64 form after `steps` steps."""
+    t0 = time.perf_counter()
+    sy = [ec.system(k) for k in range(ec.n_systems)]
+    dims = [s[0] for s in sy]
+    sites = sum(int(np.prod(d)) for d in dims)
+    log(f"ensemble {ec.name}: {ec.n_systems} systems, {sites} sites, inputs in {time.perf_counter() - t0:.1f}s")
+    ens = ovl.Ensemble(ec.nc, dims, ec.nmodes, r_wilson=ec.r_wilson, m_fermion=ec.m_fermion, m_dw=ec.m_dw)
+
+    def load():
+        for k, (n, nb, links, e, psi) in enumerate(sy):
+            ens.set_system_links(k, links)
+            ens.set_system_efield(k, e)
+            ens.set_system_u1_flux(k, nb, ec.q)
+            ens.set_system_modes(k, ovl.PSI, psi)
+            ens.set_system_modes(k, ovl.PSIMID, psi)
+
+    def opts(prec):
+        so = ovl.sign_opts(degree=ec.cheb_n, cut=ec.cheb_cut, precision=prec, tol=tol)
+        return ovl.StepOpts(kind=ovl.OVERLAP, dt=ec.dt, gauge=ovl.GAUGE_LINKS, sign=so, use_graph=0)
+
+    load()
+    tune = ens.autotune()
+    ens.fermion_update(opts(ovl.PREC_MIXED), warmup)
+    out = {}
+    for name, prec in (("mixed", ovl.PREC_MIXED), ("fp64", ovl.PREC_FP64)):
+        load()
+        ens.set_profiling(True)
+        t0 = time.perf_counter()
+        ens.fermion_update(opts(prec), steps)
+        wall = time.perf_counter() - t0
+        ens.set_profiling(False)
+        t = ens.timing()
+        var = int(np.argmax([t.var_ms[i] for i in range(3)]))
+        out[name] = {"ms_per_step": 1e3 * wall / steps, "value": sites * ec.nmodes * steps / wall,
+                     "kernel": ["k_hop_ens FP64", "k_hop_ens FP32", "k_hop_ens FP32->FP64"][var],
+                     "kernel_GBps": t.var_bytes[var] / max(t.var_ms[var], 1e-9) / 1e6,
+                     "kernel_share_of_step": t.var_ms[var] / max(t.total_ms, 1e-9),
+                     "fp64_steps": t.fp64_steps, "fp32_steps": t.fp32_steps}
+        out[name]["psi"] = [ens.get_system_modes(k, ovl.PSI) for k in range(ec.n_systems)]
+        log(f"ensemble {name}: {out[name]['ms_per_step']:.1f} ms/step {out[name]['value']:.4e} site*mode/s, "
+            f"{out[name]['kernel']} {out[name]['kernel_GBps']:.0f} GB/s")
+    errs = [float(max(np.linalg.norm(a - b) / np.linalg.norm(b) for a, b in zip(m, f)))
+            for m, f in zip(out["mixed"].pop("psi"), out["fp64"].pop("psi"))]
+    res = {"workload": ec.name, "systems": ec.n_systems, "sites": sites, "modes_per_system": ec.nmodes,
+           "chebyshev_degree": ec.cheb_n, "steps": steps, "xcd_imbalance": round(ens.balance(), 4),
+           "autotune": tune, "mixed": out["mixed"], "fp64": out["fp64"],
+           "speedup_vs_fp64": out["mixed"]["value"] / out["fp64"]["value"],
+           "per_system_rel_err_vs_fp64": [float(f"{e:.3e}") for e in errs],
+           "systems_layout": [[d[0], nb] for (d, nb, *_r) in sy]}
+    ens.close()
+    log(f"ensemble per-system err vs fp64: max {max(errs):.3e}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -70,10 +190,18 @@ def main():
     ap.add_argument("--quick", action="store_true", help="timed region only (layout / tile sweeps)")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--cpu-modes", type=int, default=16)
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the ensemble workload")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 counter passes")
+    ap.add_argument("--ensemble-only", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
+
+    if args.ensemble_only:
+        r = bench_ensemble(cf.ENSEMBLES["su2_overlap_ensemble"], args.steps, args.warmup, args.tol)
+        print(json.dumps(r))
+        return
 
     c = cf.CONFIGS[args.workload]
     if args.modes:
@@ -103,7 +231,7 @@ def main():
     som = step_opts(c, prec_mixed, False, args.tol, args.chunk).sign
     res = {}
     if args.quick:
-        args.no_fp64 = args.no_cpu = True
+        args.no_fp64 = args.no_cpu = args.no_ensemble = args.no_traffic = True
     if c.kind == "overlap" and not args.quick:
         ctx.apply_overlap(ovl.PSI, ovl.SCRATCH, so64)
         ref = ctx.get_modes(ovl.SCRATCH, 0, min(16, c.nmodes))
@@ -190,6 +318,28 @@ def main():
         log(f"sign residual (mixed) {res['sign_residual_mixed']:.3e}")
     ctx.close()
 
+    # HBM traffic of the dominant stencil variant, from two counter passes (child processes)
+    if not args.no_traffic:
+        prefix = ["k_hop<%d, double, double" % c.nc, "k_hop<%d, float, float" % c.nc,
+                  "k_hop<%d, float, double" % c.nc][var]
+        tr = measure_traffic(args, c, prefix, int(round(t.var_launches[var] / args.steps)))
+        if tr:
+            traffic = tr["FETCH_SIZE"]["bytes_per_launch"] + tr["WRITE_SIZE"]["bytes_per_launch"]
+            roofline["traffic"] = traffic
+            roofline["traffic_detail"] = {
+                "read_bytes_per_launch": tr["FETCH_SIZE"]["bytes_per_launch"],
+                "write_bytes_per_launch": tr["WRITE_SIZE"]["bytes_per_launch"],
+                "fetch_calibration": round(tr["FETCH_SIZE"]["factor"], 4),
+                "write_calibration": round(tr["WRITE_SIZE"]["factor"], 4),
+                "kernel": tr["FETCH_SIZE"]["kernel"],
+                "traffic_over_algorithmic": round(traffic / bytes_per_launch, 3),
+                "traffic_GBps": round(traffic / (avg_ms * 1e-3) / 1e9, 1)}
+            log("traffic", json.dumps(roofline["traffic_detail"]))
+
+    ens = None
+    if not args.no_ensemble:
+        ens = bench_ensemble(cf.ENSEMBLES["su2_overlap_ensemble"], 2, 1, args.tol)
+
     cpu = None
     if not args.no_cpu:
         links, e, psi = c.generate(nmodes=args.cpu_modes)
@@ -209,7 +359,7 @@ def main():
                       "traversal": args.traversal, "parallelism": "single GPU"},
            "roofline": roofline, "cpu_baseline": cpu, "fp64_gpu": fp64,
            "speedup_vs_fp64": (value / fp64["value"]) if fp64 else None,
-           "graph_ms_per_step": graph_ms, "accuracy": res}
+           "graph_ms_per_step": graph_ms, "accuracy": res, "ensemble": ens}
     print(json.dumps(out))
 
 

@@ -56,8 +56,12 @@ typedef struct {
     double m_dw;         /* MDWHeight (Simulation.cpp:200), the overlap rho */
     int mode_block;      /* modes per SoA block: 16, 32 or 64; 0 = choose */
     int reunitarize;     /* 1: project links back onto SU(Nc) after every link update */
-    int traversal;       /* stencil site order: 0 = one spatial box per XCD when the lattice splits
-                            into 8 boxes, 1 = lexicographic */
+    int traversal;       /* order in which the stencil launches walk (mode, site) lanes over the 8 XCDs:
+                            0 = automatic (ovl_autotune picks among the others),
+                            1 = lexicographic (hardware round-robin of workgroups over XCDs),
+                            2 = one spatial box of sites per XCD (every mode block in turn),
+                            3 = 1/8 of the mode blocks per XCD over the whole lattice (y slabs,
+                                z-major); needs a multiple of 8 mode blocks */
 } ovl_params;
 
 /* sign function eps(h_W) options */
@@ -147,6 +151,46 @@ int ovl_currents(ovl_ctx *ctx, int field, const ovl_sign_opts *o, int first_mode
                  double *ja, double *axial_charge);
 /* per-mode squared norms of a field (host array of n_modes doubles) */
 int ovl_mode_norms(ovl_ctx *ctx, int field, double *norm2);
+
+/* ---- ensembles: many independent lattices in one context ----
+ * No counterpart in the reference, which evolves one lattice per program run
+ * (Simulation.cpp:466-470 calls Dynamics::Fermions::Update on the one global GaugeLinks /
+ * FermionField set); an ensemble context replaces running that program once per configuration.
+ * All systems share Nc, the fermion parameters and the mode count; each has its own lattice
+ * size, links, E field, U(1) flux and modes.  The systems' data are concatenated in system order:
+ * ovl_set_links / ovl_get_links / ovl_set_efield / ovl_get_efield take the concatenation of the
+ * per-system arrays, ovl_mode_norms returns n_systems * n_modes values (system-major).
+ * ovl_copy_field, ovl_apply_wilson, ovl_apply_overlap / ovl_apply_sign / ovl_sign_residual
+ * (Chebyshev sign), ovl_fermion_update, ovl_update_links / ovl_update_efield and ovl_autotune act
+ * on every system at once: each stencil application is one kernel launch over all systems.
+ * ovl_set_modes / ovl_get_modes, the single-lattice observables, ovl_currents, ovl_ritz, the
+ * Zolotarev sign and the single-mode drop-ins return OVL_E_STATE on an ensemble; use the
+ * per-system calls below (which also accept system 0 of an ordinary context). */
+typedef struct {
+    int nc;              /* 2 or 3 colours */
+    int n_systems;       /* number of lattices (1 .. 65536) */
+    const int *dims;     /* 3 * n_systems lattice sizes Nx, Ny, Nz (each >= 3), system-major */
+    int n_modes;         /* mode functions per system and field */
+    double r_wilson, m_fermion, m_dw;
+    int mode_block;      /* 16, 32 or 64; 0 = choose */
+    int reunitarize;
+} ovl_ensemble_params;
+int ovl_create_ensemble(const ovl_ensemble_params *p, ovl_ctx **out);
+int ovl_set_system_links(ovl_ctx *ctx, int sys, const double *links);
+int ovl_get_system_links(ovl_ctx *ctx, int sys, double *links);
+int ovl_set_system_efield(ovl_ctx *ctx, int sys, const double *e);
+int ovl_get_system_efield(ovl_ctx *ctx, int sys, double *e);
+int ovl_set_system_u1_flux(ovl_ctx *ctx, int sys, int nb, double q);
+int ovl_set_system_modes(ovl_ctx *ctx, int sys, int field, const double *aos, int first, int count);
+int ovl_get_system_modes(ovl_ctx *ctx, int sys, int field, double *aos, int first, int count);
+/* obs[OVL_OBS_COUNT] of one system: electric and magnetic energy, Gauss-law violation max and
+ * rms, unitarity violation max and rms, dN_CS/dt -- the quantities of ovl_energy,
+ * ovl_gauss_violation, ovl_unitarity_violation and ovl_ncs_dot */
+enum { OVL_OBS_E_EL = 0, OVL_OBS_E_MAG, OVL_OBS_GAUSS_MAX, OVL_OBS_GAUSS_RMS, OVL_OBS_UNIT_MAX,
+       OVL_OBS_UNIT_RMS, OVL_OBS_NCS_DOT, OVL_OBS_COUNT };
+int ovl_system_observables(ovl_ctx *ctx, int sys, double *obs);
+/* stencil work balance of an ensemble: max over XCDs of the workgroups assigned / mean (1 = even) */
+int ovl_ensemble_balance(ovl_ctx *ctx, double *imbalance);
 
 /* ---- timing of the last ovl_apply_* / ovl_fermion_update call ---- */
 typedef struct {

@@ -151,3 +151,58 @@ CONFIGS = {
     "su3_overlap_24": Config("su3_overlap_24", 3, (24, 24, 24), 256, 100, 0.02, 0.25, 0.1, 3, "overlap",
                              nb=8, q=1.0, m_dw=1.2, cheb_n=384, cheb_cut=0.05),
 }
+
+
+@dataclasses.dataclass(frozen=True)
+class EnsembleConfig:
+    """Many independent SU(2) overlap systems of uneven sizes, one seed each.
+
+    System k uses seed ``seed0 + k``: its generator first draws the lattice size L from
+    ``sizes`` and the flux quanta Nb from ``0..nb_max``, then links, E field and spinors in the
+    order of Config.generate.
+    """
+    name: str
+    n_systems: int
+    seed0: int
+    sizes: tuple
+    nb_max: int
+    nmodes: int
+    steps: int
+    dt: float
+    link_sigma: float
+    e_sigma: float
+    m_dw: float
+    cheb_n: int
+    cheb_cut: float
+    nc: int = 2
+    m_fermion: float = 0.0
+    r_wilson: float = 1.0
+    q: float = 1.0
+
+    def system(self, k: int, nmodes: int | None = None):
+        """(dims, nb, links, e, psi) of system k."""
+        g = rng(self.seed0 + k)
+        L = int(self.sizes[int(g.integers(len(self.sizes)))])
+        nb = int(g.integers(0, self.nb_max + 1))
+        n = (L, L, L)
+        links = su2_links(n, self.link_sigma, g) if self.nc == 2 else su3_links(n, self.link_sigma, g)
+        e = efield(n, self.nc, self.e_sigma, g)
+        psi = spinors(n, self.nc, self.nmodes if nmodes is None else nmodes, g)
+        return n, nb, links, e, psi
+
+    def layout(self):
+        """(dims, nb) of every system without drawing the fields."""
+        out = []
+        for k in range(self.n_systems):
+            g = rng(self.seed0 + k)
+            L = int(self.sizes[int(g.integers(len(self.sizes)))])
+            out.append(((L, L, L), int(g.integers(0, self.nb_max + 1))))
+        return out
+
+
+# 32 SU(2) overlap systems, L in {12, 16, 20, 24}, seeds 100-131, 32 modes each, Nb in 0..6,
+# 50 steps at dt = 0.02, links and E as the 8^3 Wilson workload, kernel as the 16^3 overlap one
+ENSEMBLES = {
+    "su2_overlap_ensemble": EnsembleConfig("su2_overlap_ensemble", 32, 100, (12, 16, 20, 24), 6, 32, 50, 0.02,
+                                           0.3, 0.1, 1.0, 256, 0.05),
+}

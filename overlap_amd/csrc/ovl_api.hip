@@ -18,6 +18,7 @@ using ovl::Geom;
 using ovl::HopArgs;
 using ovl::P32;
 using ovl::P64;
+using ovl::SysDesc;
 typedef std::complex<double> cplx;
 
 struct DevBuf {
@@ -52,7 +53,10 @@ struct ovl_ctx {
     // reductions / small scratch
     double *partial = nullptr;
     size_t partial_len = 0;
-    int *xcd_order = nullptr;  // device site traversal of the stencil launches (or null)
+    int *xcd_order = nullptr;  // device XCD-box site order (TRAV_BOXES) or null
+    int *slab_order[2] = {nullptr, nullptr};  // per precision: site order of TRAV_MODES (y slabs) or null
+    int trav[2] = {ovl::TRAV_LEX, ovl::TRAV_LEX};  // stencil traversal per input precision
+    int slabs[2] = {1, 1};                      // y slabs of TRAV_MODES per input precision
     double *dout = nullptr;  // 16 doubles
     double *gam = nullptr;   // current coupling matrices
     double *jv = nullptr, *ja = nullptr;
@@ -82,6 +86,18 @@ struct ovl_ctx {
     hipEvent_t t0 = nullptr, t1 = nullptr;
     bool capturing = false;
     int wpe[2] = {1, 1};  // stencil occupancy variant per input precision (P64, P32), set by ovl_autotune
+
+    // lattices held: one for an ordinary context, or the systems of an ensemble, stored one after
+    // another in every field (site-blocks q0..), link and E-field array (link index l0..)
+    bool ensemble = false;
+    std::vector<SysDesc> sys;
+    std::vector<int> sys_nb;            // U(1) flux quanta per system (0 = no phases)
+    std::vector<char> sys_links_set;
+    long link_sites = 0;                // sites summed over the systems
+    SysDesc *d_sys = nullptr;           // device copy of sys (ensemble stencil)
+    int2 *d_work = nullptr;             // ensemble stencil work table
+    int ens_grid = 0;
+    double ens_imbalance = 1.0;
 };
 
 namespace {
@@ -132,6 +148,51 @@ double lambda_bound(const ovl_ctx *c)
 
 double amass(const ovl_ctx *c) { return 6.0 * c->prm.r_wilson + 2.0 * c->prm.m_fermion - 2.0 * c->prm.m_dw; }
 
+// per-system views of the concatenated arrays
+int link_reals(const ovl_ctx *c) { return c->g.nc == 2 ? 4 : 18; }
+int n_adj(const ovl_ctx *c) { return c->g.nc * c->g.nc - 1; }
+
+Geom sys_geom(const ovl_ctx *c, int s)
+{
+    Geom g = c->g;
+    const SysDesc &d = c->sys[s];
+    g.n0 = d.n0;
+    g.n1 = d.n1;
+    g.n2 = d.n2;
+    g.V = d.V;
+    g.nblk = d.nblk;
+    return g;
+}
+
+double *sys_links(const ovl_ctx *c, int s) { return c->links + (size_t)c->sys[s].l0 * link_reals(c); }
+double *sys_efield(const ovl_ctx *c, int s) { return c->efield + (size_t)c->sys[s].l0 * n_adj(c); }
+double *sys_ul64(const ovl_ctx *c, int s) { return c->ul64 + (size_t)c->sys[s].l0 * c->g.nc * c->g.nc * 2; }
+float *sys_ul32(const ovl_ctx *c, int s) { return c->ul32 + (size_t)c->sys[s].l0 * c->g.nc * c->g.nc * 2; }
+const double *sys_phase(const ovl_ctx *c, int s)
+{
+    return (c->phase && c->sys_nb[s] != 0) ? c->phase + (size_t)c->sys[s].l0 * 2 : nullptr;
+}
+// FP64 field of the context -> first site-block of system s
+void *sys_field(const ovl_ctx *c, const void *f, int s)
+{
+    return (char *)f + (size_t)c->sys[s].q0 * c->ncomp * c->g.mb * 16;
+}
+
+int nsys(const ovl_ctx *c) { return (int)c->sys.size(); }
+
+int check_sys(ovl_ctx *c, int s)
+{
+    if (s < 0 || s >= nsys(c)) return fail(c, OVL_E_ARG, "system index out of range");
+    return OVL_OK;
+}
+
+int single_lattice(ovl_ctx *c, const char *what)
+{
+    if (c->ensemble)
+        return fail(c, OVL_E_STATE, std::string(what) + " acts on one lattice; on an ensemble use the per-system calls");
+    return OVL_OK;
+}
+
 int check_field(ovl_ctx *c, int f)
 {
     if (f < 0 || f > 2) return fail(c, OVL_E_ARG, "field index must be OVL_PSI, OVL_PSIMID or OVL_SCRATCH");
@@ -142,7 +203,9 @@ int ensure_stencil(ovl_ctx *c)
 {
     if (!c->links_set) return fail(c, OVL_E_STATE, "no gauge links uploaded (ovl_set_links)");
     if (c->stencil_dirty) {
-        HIPCHK(c, ovl::launch_link_setup(c->g, c->links, c->phase, c->ul64, c->ul32, c->st));
+        for (int s = 0; s < nsys(c); ++s)
+            HIPCHK(c, ovl::launch_link_setup(sys_geom(c, s), sys_links(c, s), sys_phase(c, s), sys_ul64(c, s),
+                                             sys_ul32(c, s), c->st));
         c->stencil_dirty = false;
     }
     return OVL_OK;
@@ -177,7 +240,7 @@ double hop_alg_bytes(const ovl_ctx *c, const HopArgs &a, int pin, int pout)
     if (a.b1) b += sites_modes * Sout;
     if (a.b2) b += sites_modes * Sout;
     if (a.tgt) b += sites_modes * Sout;
-    b += 3.0 * c->g.V * c->g.nc * c->g.nc * 2.0 * (pin == P64 ? 8 : 4);
+    b += 3.0 * c->link_sites * c->g.nc * c->g.nc * 2.0 * (pin == P64 ? 8 : 4);
     return b;
 }
 
@@ -208,7 +271,15 @@ HopArgs hop_args(const ovl_ctx *c, int prec)
     a.links = prec == P64 ? (const void *)c->ul64 : (const void *)c->ul32;
     a.amass = amass(c);
     a.nblk = c->g.nblk;
-    a.order = c->xcd_order;
+    a.trav = c->trav[prec];
+    a.order = a.trav == ovl::TRAV_BOXES ? c->xcd_order : (a.trav == ovl::TRAV_MODES ? c->slab_order[prec] : nullptr);
+    if (c->ensemble) {
+        a.trav = ovl::TRAV_LEX;
+        a.order = nullptr;
+        a.ens_sys = c->d_sys;
+        a.ens_work = c->d_work;
+        a.ens_grid = c->ens_grid;
+    }
     return a;
 }
 
@@ -250,6 +321,54 @@ std::vector<int> xcd_site_order(int n0, int n1, int n2)
                         }
             }
     return order;
+}
+
+// Site order of the mode-split traversal: the lattice is cut into `t` slabs along y and each slab
+// is walked z-major (x fastest), so that the planes z-1, z, z+1 of one slab -- the reuse distance
+// of the z neighbours -- are what the XCD's L2 has to hold.
+std::vector<int> slab_site_order(int n0, int n1, int n2, int t)
+{
+    std::vector<int> ord;
+    ord.reserve((size_t)n0 * n1 * n2);
+    for (int k = 0; k < t; ++k) {
+        const int y0 = (int)((long)k * n1 / t), y1 = (int)((long)(k + 1) * n1 / t);
+        for (int z = 0; z < n2; ++z)
+            for (int y = y0; y < y1; ++y)
+                for (int x = 0; x < n0; ++x) ord.push_back(x + n0 * (y + n1 * z));
+    }
+    return ord;
+}
+
+// slab count that keeps three slab planes of one mode block of precision `prec` within ~3 MB
+// of the 4 MB L2 of an XCD
+int default_slabs(const ovl_ctx *c, int prec)
+{
+    const double site_bytes = (double)c->g.mb * c->ncomp * (prec == P64 ? 16 : 8);
+    int t = 1;
+    while (t < c->g.n1 && 3.0 * c->g.n0 * ((c->g.n1 + t - 1) / t) * site_bytes > 3.0 * (1 << 20)) ++t;
+    return t;
+}
+
+// select the traversal of precision `prec`; uploads the slab order when it changes
+int set_traversal(ovl_ctx *c, int prec, int trav, int slabs)
+{
+    if (c->ensemble) return OVL_OK;
+    if (trav == ovl::TRAV_BOXES && !c->xcd_order) trav = ovl::TRAV_LEX;
+    if (trav == ovl::TRAV_MODES && c->g.nblk % ovl::NUM_XCD != 0) trav = ovl::TRAV_LEX;
+    c->trav[prec] = trav;
+    if (trav != ovl::TRAV_MODES) return OVL_OK;
+    slabs = std::max(1, std::min(slabs, c->g.n1));
+    if (slabs == c->slabs[prec] && (slabs == 1 || c->slab_order[prec])) return OVL_OK;
+    c->slabs[prec] = slabs;
+    if (slabs == 1) {
+        if (c->slab_order[prec]) hipFree(c->slab_order[prec]);
+        c->slab_order[prec] = nullptr;
+        return OVL_OK;
+    }
+    const std::vector<int> ord = slab_site_order(c->g.n0, c->g.n1, c->g.n2, slabs);
+    RETCHK(dalloc(c, (void **)&c->slab_order[prec], sizeof(int) * ord.size()));
+    HIPCHK(c, hipMemcpy(c->slab_order[prec], ord.data(), sizeof(int) * ord.size(), hipMemcpyHostToDevice));
+    return OVL_OK;
 }
 
 void begin_timing(ovl_ctx *c)
@@ -591,6 +710,18 @@ int mode_dot(ovl_ctx *c, const Geom &g, const void *a, const void *b, std::vecto
     return OVL_OK;
 }
 
+// per-mode <a,b> of every system (system-major, n_modes per system)
+int mode_dot_all(ovl_ctx *c, const void *a, const void *b, std::vector<cplx> &out)
+{
+    out.clear();
+    std::vector<cplx> d;
+    for (int s = 0; s < nsys(c); ++s) {
+        RETCHK(mode_dot(c, sys_geom(c, s), sys_field(c, a, s), sys_field(c, b, s), d));
+        out.insert(out.end(), d.begin(), d.begin() + c->prm.n_modes);
+    }
+    return OVL_OK;
+}
+
 // (h_W^2 + d0) v  ->  q   via two stencil launches, FP64
 int apply_h2(ovl_ctx *c, const Geom &g, const void *v, void *w, void *q, double d0)
 {
@@ -756,6 +887,105 @@ int ovl_device_name(char *buf, int len)
     return OVL_OK;
 }
 
+// Allocations and device tables common to ordinary and ensemble contexts; c->g, c->sys and
+// c->link_sites are filled in.  On failure the caller destroys c.
+static int ctx_init(ovl_ctx *c)
+{
+    const int mb = c->g.mb, nc = c->g.nc;
+    c->ncomp = 4 * nc;
+    c->blk64_elems = (size_t)c->g.V * c->ncomp * mb;
+    c->blk32_elems = (size_t)c->g.V * (c->ncomp / 2) * mb;
+    c->field64_bytes = c->blk64_elems * 16 * c->g.nblk;
+    c->field32_bytes = c->blk32_elems * 16 * c->g.nblk;
+    c->sys_nb.assign(c->sys.size(), 0);
+    c->sys_links_set.assign(c->sys.size(), 0);
+    if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) return OVL_E_HIP;
+    hipEventCreate(&c->t0);
+    hipEventCreate(&c->t1);
+    const size_t L = (size_t)c->link_sites;
+    const size_t esz = sizeof(double) * n_adj(c) * 3 * L;
+    RETCHK(dalloc(c, (void **)&c->links, sizeof(double) * link_reals(c) * 3 * L));
+    RETCHK(dalloc(c, (void **)&c->efield, esz));
+    RETCHK(dalloc(c, (void **)&c->ul64, 16 * 3 * L * nc * nc));
+    RETCHK(dalloc(c, (void **)&c->ul32, 8 * 3 * L * nc * nc));
+    for (int f = 0; f < 3; ++f) RETCHK(dalloc(c, &c->field[f], c->field64_bytes));
+    for (int f = 0; f < 3; ++f) HIPCHK(c, hipMemsetAsync(c->field[f], 0, c->field64_bytes, c->st));
+    HIPCHK(c, hipMemsetAsync(c->efield, 0, esz, c->st));
+    size_t plen = std::max(std::max(ovl::mode_dot_partial_len(c->g), ovl::reduce_partial_len(c->g)),
+                           ovl::currents_partial_len(c->g));
+    int maxblk = c->g.nblk;
+    for (int s = 0; s < nsys(c); ++s) {
+        const Geom gs = sys_geom(c, s);
+        plen = std::max(plen, std::max(std::max(ovl::mode_dot_partial_len(gs), ovl::reduce_partial_len(gs)),
+                                       ovl::currents_partial_len(gs)));
+        maxblk = std::max(maxblk, gs.nblk);
+    }
+    c->partial_len = plen;
+    RETCHK(dalloc(c, (void **)&c->partial, sizeof(double) * c->partial_len));
+    RETCHK(dalloc(c, (void **)&c->dout, sizeof(double) * (16 + 2 * (size_t)maxblk * mb)));
+    if (!c->ensemble && (c->prm.traversal == 0 || c->prm.traversal == 2)) {
+        const std::vector<int> ord = xcd_site_order(c->g.n0, c->g.n1, c->g.n2);
+        if (!ord.empty()) {
+            RETCHK(dalloc(c, (void **)&c->xcd_order, sizeof(int) * ord.size()));
+            HIPCHK(c, hipMemcpy(c->xcd_order, ord.data(), sizeof(int) * ord.size(), hipMemcpyHostToDevice));
+        }
+    }
+    if (!c->ensemble) {
+        // traversal 0 (automatic): mode split where the mode blocks divide over the XCDs, else
+        // boxes; ovl_autotune may change it
+        for (int p = P64; p <= P32; ++p) {
+            int t = ovl::TRAV_LEX;
+            if (c->prm.traversal == 0) t = (c->g.nblk % ovl::NUM_XCD == 0) ? ovl::TRAV_MODES : ovl::TRAV_BOXES;
+            if (c->prm.traversal == 2) t = ovl::TRAV_BOXES;
+            if (c->prm.traversal == 3) t = ovl::TRAV_MODES;
+            RETCHK(set_traversal(c, p, t, default_slabs(c, p)));
+        }
+    }
+    if (c->ensemble) {
+        // Work table of the ensemble stencil: the hardware hands workgroup b to XCD b % 8, so
+        // placing a system's workgroups at b = k + 8 j keeps the whole system (and the L2 reuse
+        // of its neighbour loads) on XCD k.  Systems go largest first to the least loaded XCD;
+        // shorter XCD lists are padded with idle entries.
+        const int ns = nsys(c);
+        std::vector<long> wg(ns);
+        std::vector<int> idx(ns);
+        for (int s = 0; s < ns; ++s) {
+            const SysDesc &d = c->sys[s];
+            wg[s] = ((long)d.V * d.nblk * mb + ovl::STENCIL_TPB - 1) / ovl::STENCIL_TPB;
+            idx[s] = s;
+        }
+        std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return wg[x] > wg[y]; });
+        std::vector<std::vector<int2>> bins(ovl::NUM_XCD);
+        std::vector<long> load(ovl::NUM_XCD, 0);
+        for (int s : idx) {
+            const int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            for (long j = 0; j < wg[s]; ++j) bins[k].push_back(make_int2(s, (int)j));
+            load[k] += wg[s];
+        }
+        const long lmax = *std::max_element(load.begin(), load.end());
+        long ltot = 0;
+        for (long l : load) ltot += l;
+        if ((long)ovl::NUM_XCD * lmax > (1L << 30)) return fail(c, OVL_E_ARG, "ensemble too large for one launch");
+        std::vector<int2> tab((size_t)ovl::NUM_XCD * lmax, make_int2(-1, 0));
+        for (int k = 0; k < ovl::NUM_XCD; ++k)
+            for (size_t j = 0; j < bins[k].size(); ++j) tab[(size_t)k + ovl::NUM_XCD * j] = bins[k][j];
+        c->ens_grid = (int)tab.size();
+        c->ens_imbalance = (double)lmax * ovl::NUM_XCD / (double)ltot;
+        RETCHK(dalloc(c, (void **)&c->d_work, sizeof(int2) * tab.size()));
+        RETCHK(dalloc(c, (void **)&c->d_sys, sizeof(SysDesc) * c->sys.size()));
+        HIPCHK(c, hipMemcpy(c->d_work, tab.data(), sizeof(int2) * tab.size(), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(c->d_sys, c->sys.data(), sizeof(SysDesc) * c->sys.size(), hipMemcpyHostToDevice));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return OVL_OK;
+}
+
+static int pick_mode_block(int mode_block, int n_modes)
+{
+    if (mode_block != 0) return mode_block;
+    return n_modes >= 64 ? 64 : (n_modes > 16 ? 32 : 16);
+}
+
 int ovl_create(const ovl_params *p, ovl_ctx **out)
 {
     if (!p || !out) return OVL_E_ARG;
@@ -764,58 +994,86 @@ int ovl_create(const ovl_params *p, ovl_ctx **out)
     if (p->n[0] < 3 || p->n[1] < 3 || p->n[2] < 3) return OVL_E_ARG;
     if (p->n_modes < 1) return OVL_E_ARG;
     if (p->mode_block != 0 && p->mode_block != 16 && p->mode_block != 32 && p->mode_block != 64) return OVL_E_ARG;
+    if (p->traversal < 0 || p->traversal > 3) return OVL_E_ARG;
     const long V = (long)p->n[0] * p->n[1] * p->n[2];
     if (V > (1L << 26)) return OVL_E_ARG;
+    const int mb = pick_mode_block(p->mode_block, p->n_modes);
+    const int nblk = (p->n_modes + mb - 1) / mb;
+    if ((long)nblk * V * mb > (1L << 31) - 1) return OVL_E_ARG;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return OVL_E_HIP;
     ovl_ctx *c = new ovl_ctx();
     c->prm = *p;
-    int mb = p->mode_block;
-    if (mb == 0) mb = p->n_modes >= 64 ? 64 : (p->n_modes > 16 ? 32 : 16);
+    c->prm.mode_block = mb;
     c->g.nc = p->nc;
     c->g.n0 = p->n[0];
     c->g.n1 = p->n[1];
     c->g.n2 = p->n[2];
     c->g.V = (int)V;
     c->g.mb = mb;
-    c->g.nblk = (p->n_modes + mb - 1) / mb;
-    c->ncomp = 4 * p->nc;
-    c->blk64_elems = (size_t)V * c->ncomp * mb;
-    c->blk32_elems = (size_t)V * (c->ncomp / 2) * mb;
-    c->field64_bytes = c->blk64_elems * 16 * c->g.nblk;
-    c->field32_bytes = c->blk32_elems * 16 * c->g.nblk;
-    int rc = OVL_OK;
-    auto bail = [&](int r) {
+    c->g.nblk = nblk;
+    c->sys.push_back(SysDesc{p->n[0], p->n[1], p->n[2], (int)V, nblk, 0, 0, 0});
+    c->link_sites = V;
+    const int rc = ctx_init(c);
+    if (rc != OVL_OK) {
         ovl_destroy(c);
-        return r;
-    };
-    if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) return bail(OVL_E_HIP);
-    hipEventCreate(&c->t0);
-    hipEventCreate(&c->t1);
-    const int ng = p->nc * p->nc - 1;
-    const size_t lsz = (p->nc == 2 ? 4 : 18) * 3 * V * sizeof(double);
-    if ((rc = dalloc(c, (void **)&c->links, lsz)) || (rc = dalloc(c, (void **)&c->efield, sizeof(double) * ng * 3 * V)) ||
-        (rc = dalloc(c, (void **)&c->ul64, 16 * 3 * V * p->nc * p->nc)) ||
-        (rc = dalloc(c, (void **)&c->ul32, 8 * 3 * V * p->nc * p->nc)))
-        return bail(rc);
-    for (int f = 0; f < 3; ++f)
-        if ((rc = dalloc(c, &c->field[f], c->field64_bytes))) return bail(rc);
-    for (int f = 0; f < 3; ++f) hipMemsetAsync(c->field[f], 0, c->field64_bytes, c->st);
-    hipMemsetAsync(c->efield, 0, sizeof(double) * ng * 3 * V, c->st);
-    c->partial_len = std::max(std::max(ovl::mode_dot_partial_len(c->g), ovl::reduce_partial_len(c->g)),
-                              ovl::currents_partial_len(c->g));
-    if ((rc = dalloc(c, (void **)&c->partial, sizeof(double) * c->partial_len)) ||
-        (rc = dalloc(c, (void **)&c->dout, sizeof(double) * (16 + 2 * (size_t)c->g.nblk * mb))))
-        return bail(rc);
-    if (p->traversal == 0) {
-        const std::vector<int> ord = xcd_site_order(p->n[0], p->n[1], p->n[2]);
-        if (!ord.empty()) {
-            if ((rc = dalloc(c, (void **)&c->xcd_order, sizeof(int) * ord.size()))) return bail(rc);
-            if (hipMemcpy(c->xcd_order, ord.data(), sizeof(int) * ord.size(), hipMemcpyHostToDevice) != hipSuccess)
-                return bail(OVL_E_HIP);
-        }
+        return rc;
     }
-    if (hipStreamSynchronize(c->st) != hipSuccess) return bail(OVL_E_HIP);
+    *out = c;
+    return OVL_OK;
+}
+
+int ovl_create_ensemble(const ovl_ensemble_params *p, ovl_ctx **out)
+{
+    if (!p || !out || !p->dims) return OVL_E_ARG;
+    *out = nullptr;
+    if (p->nc != 2 && p->nc != 3) return OVL_E_ARG;
+    if (p->n_systems < 1 || p->n_systems > 65536 || p->n_modes < 1) return OVL_E_ARG;
+    if (p->mode_block != 0 && p->mode_block != 16 && p->mode_block != 32 && p->mode_block != 64) return OVL_E_ARG;
+    const int mb = pick_mode_block(p->mode_block, p->n_modes);
+    const int nblk = (p->n_modes + mb - 1) / mb;
+    std::vector<SysDesc> sys;
+    long q0 = 0, sites = 0;
+    for (int s = 0; s < p->n_systems; ++s) {
+        const int *n = p->dims + 3 * s;
+        if (n[0] < 3 || n[1] < 3 || n[2] < 3) return OVL_E_ARG;
+        const long V = (long)n[0] * n[1] * n[2];
+        if (V > (1L << 26) || (long)nblk * V * mb > (1L << 31) - 1) return OVL_E_ARG;
+        sys.push_back(SysDesc{n[0], n[1], n[2], (int)V, nblk, (int)q0, (int)(3 * sites), 0});
+        q0 += (long)nblk * V;
+        sites += V;
+        if (q0 * mb > (1L << 31) - 1 || 3 * sites > (1L << 31) - 1) return OVL_E_ARG;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return OVL_E_HIP;
+    ovl_ctx *c = new ovl_ctx();
+    c->ensemble = true;
+    c->prm.nc = p->nc;
+    c->prm.n[0] = (int)q0;  // the concatenation seen as one flat run of site-blocks
+    c->prm.n[1] = 1;
+    c->prm.n[2] = 1;
+    c->prm.n_modes = p->n_modes;
+    c->prm.r_wilson = p->r_wilson;
+    c->prm.m_fermion = p->m_fermion;
+    c->prm.m_dw = p->m_dw;
+    c->prm.mode_block = mb;
+    c->prm.reunitarize = p->reunitarize;
+    c->prm.traversal = 1;
+    // pointwise work (conversions, linear combinations) runs over all systems as one flat field
+    c->g.nc = p->nc;
+    c->g.n0 = (int)q0;
+    c->g.n1 = 1;
+    c->g.n2 = 1;
+    c->g.V = (int)q0;
+    c->g.mb = mb;
+    c->g.nblk = 1;
+    c->sys = sys;
+    c->link_sites = sites;
+    const int rc = ctx_init(c);
+    if (rc != OVL_OK) {
+        ovl_destroy(c);
+        return rc;
+    }
     *out = c;
     return OVL_OK;
 }
@@ -827,7 +1085,7 @@ int ovl_destroy(ovl_ctx *c)
     void *ps[] = {c->links, c->efield, c->ul64, c->ul32, c->phase, c->field[0], c->field[1], c->field[2], c->w64, c->a64,
                   c->b64, c->x32, c->w32, c->a32, c->b32, c->partial, c->dout, c->gam, c->jv, c->ja, c->staging,
                   c->ms_ps, c->ms_r, c->ms_q, c->ms_w, c->ms_chi, c->ms_coef, c->ms_active, c->rz[0], c->rz[1],
-                  c->rz[2], c->rz[3], c->rz[4], c->rz[5], c->xcd_order};
+                  c->rz[2], c->rz[3], c->rz[4], c->rz[5], c->xcd_order, c->d_sys, c->d_work, c->slab_order[0], c->slab_order[1]};
     for (void *p : ps)
         if (p) hipFree(p);
     for (auto e : c->evpool) hipEventDestroy(e);
@@ -858,14 +1116,34 @@ static bool all_finite(const double *p, size_t n)
     return true;
 }
 
+static void refresh_links_set(ovl_ctx *c)
+{
+    c->links_set = true;
+    for (char f : c->sys_links_set) c->links_set = c->links_set && f;
+}
+
+// host <-> device copies of `n` doubles at device offset `off` (elements)
+static int put_doubles(ovl_ctx *c, double *dst, const double *src, size_t n, const char *what)
+{
+    if (!all_finite(src, n)) return fail(c, OVL_E_ARG, std::string("non-finite ") + what);
+    HIPCHK(c, hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return OVL_OK;
+}
+
+static int get_doubles(ovl_ctx *c, double *dst, const double *src, size_t n)
+{
+    HIPCHK(c, hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return OVL_OK;
+}
+
 int ovl_set_links(ovl_ctx *c, const double *links)
 {
     if (!c || !links) return OVL_E_ARG;
-    const size_t n = (size_t)(c->g.nc == 2 ? 4 : 18) * 3 * c->g.V;
-    if (!all_finite(links, n)) return fail(c, OVL_E_ARG, "non-finite gauge link");
-    HIPCHK(c, hipMemcpyAsync(c->links, links, n * sizeof(double), hipMemcpyHostToDevice, c->st));
-    HIPCHK(c, hipStreamSynchronize(c->st));
-    c->links_set = true;
+    RETCHK(put_doubles(c, c->links, links, (size_t)link_reals(c) * 3 * c->link_sites, "gauge link"));
+    std::fill(c->sys_links_set.begin(), c->sys_links_set.end(), 1);
+    refresh_links_set(c);
     c->stencil_dirty = true;
     return OVL_OK;
 }
@@ -873,19 +1151,13 @@ int ovl_set_links(ovl_ctx *c, const double *links)
 int ovl_get_links(ovl_ctx *c, double *links)
 {
     if (!c || !links) return OVL_E_ARG;
-    const size_t n = (size_t)(c->g.nc == 2 ? 4 : 18) * 3 * c->g.V;
-    HIPCHK(c, hipMemcpyAsync(links, c->links, n * sizeof(double), hipMemcpyDeviceToHost, c->st));
-    HIPCHK(c, hipStreamSynchronize(c->st));
-    return OVL_OK;
+    return get_doubles(c, links, c->links, (size_t)link_reals(c) * 3 * c->link_sites);
 }
 
 int ovl_set_efield(ovl_ctx *c, const double *e)
 {
     if (!c || !e) return OVL_E_ARG;
-    const size_t n = (size_t)(c->g.nc * c->g.nc - 1) * 3 * c->g.V;
-    if (!all_finite(e, n)) return fail(c, OVL_E_ARG, "non-finite electric field");
-    HIPCHK(c, hipMemcpyAsync(c->efield, e, n * sizeof(double), hipMemcpyHostToDevice, c->st));
-    HIPCHK(c, hipStreamSynchronize(c->st));
+    RETCHK(put_doubles(c, c->efield, e, (size_t)n_adj(c) * 3 * c->link_sites, "electric field"));
     c->efield_set = true;
     return OVL_OK;
 }
@@ -893,28 +1165,63 @@ int ovl_set_efield(ovl_ctx *c, const double *e)
 int ovl_get_efield(ovl_ctx *c, double *e)
 {
     if (!c || !e) return OVL_E_ARG;
-    const size_t n = (size_t)(c->g.nc * c->g.nc - 1) * 3 * c->g.V;
-    HIPCHK(c, hipMemcpyAsync(e, c->efield, n * sizeof(double), hipMemcpyDeviceToHost, c->st));
-    HIPCHK(c, hipStreamSynchronize(c->st));
+    return get_doubles(c, e, c->efield, (size_t)n_adj(c) * 3 * c->link_sites);
+}
+
+int ovl_set_system_links(ovl_ctx *c, int sys, const double *links)
+{
+    if (!c || !links) return OVL_E_ARG;
+    RETCHK(check_sys(c, sys));
+    RETCHK(put_doubles(c, sys_links(c, sys), links, (size_t)link_reals(c) * 3 * c->sys[sys].V, "gauge link"));
+    c->sys_links_set[sys] = 1;
+    refresh_links_set(c);
+    c->stencil_dirty = true;
     return OVL_OK;
 }
 
-int ovl_set_u1_flux(ovl_ctx *c, int nb, double q)
+int ovl_get_system_links(ovl_ctx *c, int sys, double *links)
 {
-    if (!c || !std::isfinite(q)) return OVL_E_ARG;
-    c->nb = nb;
-    c->q = q;
+    if (!c || !links) return OVL_E_ARG;
+    RETCHK(check_sys(c, sys));
+    return get_doubles(c, links, sys_links(c, sys), (size_t)link_reals(c) * 3 * c->sys[sys].V);
+}
+
+int ovl_set_system_efield(ovl_ctx *c, int sys, const double *e)
+{
+    if (!c || !e) return OVL_E_ARG;
+    RETCHK(check_sys(c, sys));
+    RETCHK(put_doubles(c, sys_efield(c, sys), e, (size_t)n_adj(c) * 3 * c->sys[sys].V, "electric field"));
+    c->efield_set = true;
+    return OVL_OK;
+}
+
+int ovl_get_system_efield(ovl_ctx *c, int sys, double *e)
+{
+    if (!c || !e) return OVL_E_ARG;
+    RETCHK(check_sys(c, sys));
+    return get_doubles(c, e, sys_efield(c, sys), (size_t)n_adj(c) * 3 * c->sys[sys].V);
+}
+
+// U(1) phases of system s (QED.cpp:131-153), multiplied into its stencil links
+static int set_flux(ovl_ctx *c, int sys, int nb, double q)
+{
+    c->stencil_dirty = true;
+    c->sys_nb[sys] = nb;
     if (nb == 0) {
-        if (c->phase) hipFree(c->phase);
-        c->phase = nullptr;
-        c->stencil_dirty = true;
+        bool any = false;
+        for (int v : c->sys_nb) any = any || v != 0;
+        if (!any && c->phase) {
+            HIPCHK(c, hipStreamSynchronize(c->st));
+            hipFree(c->phase);
+            c->phase = nullptr;
+        }
         return OVL_OK;
     }
-    const int n0 = c->g.n0, n1 = c->g.n1, V = c->g.V;
+    const SysDesc &d = c->sys[sys];
+    const int n0 = d.n0, n1 = d.n1, V = d.V;
     std::vector<double> ph(6 * (size_t)V);
     for (int s = 0; s < V; ++s) {
         const int z = s / (n0 * n1), y = (s - z * n0 * n1) / n0, x = s - n0 * (y + n1 * z);
-        (void)z;
         // QED.cpp:136-143: U_y = exp(-2 pi i Nb x/(Nx Ny)), U_x(Nx-1,y) = exp(2 pi i Nb y Nx/(Nx Ny))
         const double ay = -2.0 * M_PI * (double)(nb * x) / (double)(n0 * n1);
         const double ax = (x == n0 - 1) ? 2.0 * M_PI * (double)(nb * y * n0) / (double)(n0 * n1) : 0.0;
@@ -926,38 +1233,75 @@ int ovl_set_u1_flux(ovl_ctx *c, int nb, double q)
         ph[2 * (2 * (size_t)V + s)] = 1.0;
         ph[2 * (2 * (size_t)V + s) + 1] = 0.0;
     }
-    RETCHK(dalloc(c, (void **)&c->phase, sizeof(double) * ph.size()));
-    HIPCHK(c, hipMemcpyAsync(c->phase, ph.data(), sizeof(double) * ph.size(), hipMemcpyHostToDevice, c->st));
+    if (!c->phase) RETCHK(dalloc(c, (void **)&c->phase, sizeof(double) * 6 * (size_t)c->link_sites));
+    HIPCHK(c, hipMemcpyAsync(c->phase + (size_t)d.l0 * 2, ph.data(), sizeof(double) * ph.size(), hipMemcpyHostToDevice,
+                             c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
-    c->stencil_dirty = true;
+    return OVL_OK;
+}
+
+int ovl_set_u1_flux(ovl_ctx *c, int nb, double q)
+{
+    if (!c || !std::isfinite(q)) return OVL_E_ARG;
+    c->nb = nb;
+    c->q = q;
+    for (int s = 0; s < nsys(c); ++s) RETCHK(set_flux(c, s, nb, q));
+    return OVL_OK;
+}
+
+int ovl_set_system_u1_flux(ovl_ctx *c, int sys, int nb, double q)
+{
+    if (!c || !std::isfinite(q)) return OVL_E_ARG;
+    RETCHK(check_sys(c, sys));
+    return set_flux(c, sys, nb, q);
+}
+
+static int modes_io(ovl_ctx *c, int sys, int field, double *aos, int first, int count, bool upload)
+{
+    RETCHK(check_field(c, field));
+    if (first < 0 || count < 1 || first + count > c->prm.n_modes) return fail(c, OVL_E_ARG, "mode range out of bounds");
+    const Geom g = sys_geom(c, sys);
+    void *f = sys_field(c, c->field[field], sys);
+    const size_t n = (size_t)count * g.V * c->ncomp * 2;
+    RETCHK(stage(c, n * sizeof(double)));
+    if (upload) {
+        if (!all_finite(aos, n)) return fail(c, OVL_E_ARG, "non-finite spinor component");
+        HIPCHK(c, hipMemcpyAsync(c->staging, aos, n * sizeof(double), hipMemcpyHostToDevice, c->st));
+        HIPCHK(c, ovl::launch_aos_to_soa(g, c->staging, first, count, f, c->st));
+    } else {
+        HIPCHK(c, ovl::launch_soa_to_aos(g, f, first, count, c->staging, c->st));
+        HIPCHK(c, hipMemcpyAsync(aos, c->staging, n * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->st));
     return OVL_OK;
 }
 
 int ovl_set_modes(ovl_ctx *c, int field, const double *aos, int first, int count)
 {
     if (!c || !aos) return OVL_E_ARG;
-    RETCHK(check_field(c, field));
-    if (first < 0 || count < 1 || first + count > c->prm.n_modes) return fail(c, OVL_E_ARG, "mode range out of bounds");
-    const size_t n = (size_t)count * c->g.V * c->ncomp * 2;
-    if (!all_finite(aos, n)) return fail(c, OVL_E_ARG, "non-finite spinor component");
-    RETCHK(stage(c, n * sizeof(double)));
-    HIPCHK(c, hipMemcpyAsync(c->staging, aos, n * sizeof(double), hipMemcpyHostToDevice, c->st));
-    HIPCHK(c, ovl::launch_aos_to_soa(c->g, c->staging, first, count, c->field[field], c->st));
-    HIPCHK(c, hipStreamSynchronize(c->st));
-    return OVL_OK;
+    RETCHK(single_lattice(c, "ovl_set_modes"));
+    return modes_io(c, 0, field, const_cast<double *>(aos), first, count, true);
 }
 
 int ovl_get_modes(ovl_ctx *c, int field, double *aos, int first, int count)
 {
     if (!c || !aos) return OVL_E_ARG;
-    RETCHK(check_field(c, field));
-    if (first < 0 || count < 1 || first + count > c->prm.n_modes) return fail(c, OVL_E_ARG, "mode range out of bounds");
-    const size_t n = (size_t)count * c->g.V * c->ncomp * 2;
-    RETCHK(stage(c, n * sizeof(double)));
-    HIPCHK(c, ovl::launch_soa_to_aos(c->g, c->field[field], first, count, c->staging, c->st));
-    HIPCHK(c, hipMemcpyAsync(aos, c->staging, n * sizeof(double), hipMemcpyDeviceToHost, c->st));
-    HIPCHK(c, hipStreamSynchronize(c->st));
-    return OVL_OK;
+    RETCHK(single_lattice(c, "ovl_get_modes"));
+    return modes_io(c, 0, field, aos, first, count, false);
+}
+
+int ovl_set_system_modes(ovl_ctx *c, int sys, int field, const double *aos, int first, int count)
+{
+    if (!c || !aos) return OVL_E_ARG;
+    RETCHK(check_sys(c, sys));
+    return modes_io(c, sys, field, const_cast<double *>(aos), first, count, true);
+}
+
+int ovl_get_system_modes(ovl_ctx *c, int sys, int field, double *aos, int first, int count)
+{
+    if (!c || !aos) return OVL_E_ARG;
+    RETCHK(check_sys(c, sys));
+    return modes_io(c, sys, field, aos, first, count, false);
 }
 
 int ovl_copy_field(ovl_ctx *c, int src, int dst)
@@ -994,6 +1338,7 @@ static int check_sign_opts(ovl_ctx *c, const ovl_sign_opts *o)
     if (o->precision < OVL_PREC_FP64 || o->precision > OVL_PREC_MIXED) return fail(c, OVL_E_ARG, "unknown precision");
     if (o->method == OVL_SIGN_ZOLOTAREV && o->precision != OVL_PREC_FP64)
         return fail(c, OVL_E_ARG, "the Zolotarev sign runs in FP64 only");
+    if (o->method == OVL_SIGN_ZOLOTAREV) RETCHK(single_lattice(c, "the Zolotarev sign"));
     return OVL_OK;
 }
 
@@ -1032,8 +1377,8 @@ int ovl_mode_norms(ovl_ctx *c, int field, double *norm2)
     if (!c || !norm2) return OVL_E_ARG;
     RETCHK(check_field(c, field));
     std::vector<cplx> d;
-    RETCHK(mode_dot(c, c->g, c->field[field], c->field[field], d));
-    for (int i = 0; i < c->prm.n_modes; ++i) norm2[i] = d[i].real();
+    RETCHK(mode_dot_all(c, c->field[field], c->field[field], d));
+    for (size_t i = 0; i < d.size(); ++i) norm2[i] = d[i].real();
     return OVL_OK;
 }
 
@@ -1056,15 +1401,15 @@ int ovl_sign_residual(ovl_ctx *c, int in, const ovl_sign_opts *o, double *rel)
     std::vector<cplx> dx, dd;
     if (r == OVL_OK) {
         ovl::launch_lincomb(c->g, c->g.nblk, P64, t2, t2, 1, 0, c->field[in], -1, 0, c->st);
-        r = mode_dot(c, c->g, t2, t2, dd);
-        if (r == OVL_OK) r = mode_dot(c, c->g, c->field[in], c->field[in], dx);
+        r = mode_dot_all(c, t2, t2, dd);
+        if (r == OVL_OK) r = mode_dot_all(c, c->field[in], c->field[in], dx);
     }
     hipStreamSynchronize(c->st);
     hipFree(t1);
     hipFree(t2);
     if (r != OVL_OK) return r;
     double worst = 0;
-    for (int i = 0; i < c->prm.n_modes; ++i)
+    for (size_t i = 0; i < dx.size(); ++i)
         if (dx[i].real() > 0) worst = std::max(worst, std::sqrt(dd[i].real() / dx[i].real()));
     *rel = worst;
     return OVL_OK;
@@ -1074,6 +1419,7 @@ int ovl_sign_residual(ovl_ctx *c, int in, const ovl_sign_opts *o, double *rel)
 static int dropin_setup(ovl_ctx *c, const double *ucc, const double *phi)
 {
     if (!ucc || !phi) return fail(c, OVL_E_ARG, "null argument");
+    RETCHK(single_lattice(c, "the single-mode drop-ins"));
     const size_t nu = (size_t)3 * c->g.V * c->g.nc * c->g.nc * 2, np = (size_t)c->g.V * c->ncomp * 2;
     if (!all_finite(ucc, nu) || !all_finite(phi, np)) return fail(c, OVL_E_ARG, "non-finite input");
     RETCHK(stage(c, nu * sizeof(double)));
@@ -1143,6 +1489,7 @@ int ovl_ritz(ovl_ctx *c, const double *start_flat, double sign, double delitr, i
              double *eigenvalue)
 {
     if (!c || !start_flat || !eigenvalue) return OVL_E_ARG;
+    RETCHK(single_lattice(c, "ovl_ritz"));
     RETCHK(ensure_stencil(c));
     Geom g = c->g;
     g.nblk = 1;
@@ -1287,7 +1634,9 @@ int ovl_update_links(ovl_ctx *c, double dt)
 {
     if (!c || !std::isfinite(dt)) return OVL_E_ARG;
     if (!c->links_set) return fail(c, OVL_E_STATE, "no gauge links uploaded");
-    HIPCHK(c, ovl::launch_link_update(c->g, c->links, c->efield, dt, c->prm.reunitarize, c->st));
+    for (int s = 0; s < nsys(c); ++s)
+        HIPCHK(c, ovl::launch_link_update(sys_geom(c, s), sys_links(c, s), sys_efield(c, s), dt, c->prm.reunitarize,
+                                          c->st));
     c->stencil_dirty = true;
     HIPCHK(c, hipStreamSynchronize(c->st));
     return OVL_OK;
@@ -1297,7 +1646,8 @@ int ovl_update_efield(ovl_ctx *c, double dt)
 {
     if (!c || !std::isfinite(dt)) return OVL_E_ARG;
     if (!c->links_set) return fail(c, OVL_E_STATE, "no gauge links uploaded");
-    HIPCHK(c, ovl::launch_efield_update(c->g, c->links, c->efield, dt, c->st));
+    for (int s = 0; s < nsys(c); ++s)
+        HIPCHK(c, ovl::launch_efield_update(sys_geom(c, s), sys_links(c, s), sys_efield(c, s), dt, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     return OVL_OK;
 }
@@ -1305,9 +1655,13 @@ int ovl_update_efield(ovl_ctx *c, double dt)
 static int gauge_half(ovl_ctx *c, int mode, double h)
 {
     if (mode == OVL_GAUGE_FROZEN) return OVL_OK;
-    if (mode == OVL_GAUGE_FULL) HIPCHK(c, ovl::launch_efield_update(c->g, c->links, c->efield, h, c->st));
-    HIPCHK(c, ovl::launch_link_update(c->g, c->links, c->efield, h, c->prm.reunitarize, c->st));
-    HIPCHK(c, ovl::launch_link_setup(c->g, c->links, c->phase, c->ul64, c->ul32, c->st));
+    for (int s = 0; s < nsys(c); ++s) {
+        const Geom g = sys_geom(c, s);
+        if (mode == OVL_GAUGE_FULL)
+            HIPCHK(c, ovl::launch_efield_update(g, sys_links(c, s), sys_efield(c, s), h, c->st));
+        HIPCHK(c, ovl::launch_link_update(g, sys_links(c, s), sys_efield(c, s), h, c->prm.reunitarize, c->st));
+        HIPCHK(c, ovl::launch_link_setup(g, sys_links(c, s), sys_phase(c, s), sys_ul64(c, s), sys_ul32(c, s), c->st));
+    }
     return OVL_OK;
 }
 
@@ -1412,11 +1766,10 @@ static int fetch_out(ovl_ctx *c, double *h, int n)
     return OVL_OK;
 }
 
-int ovl_energy(ovl_ctx *c, double *e_el, double *e_mag)
+static int obs_energy(ovl_ctx *c, int s, double *e_el, double *e_mag)
 {
-    if (!c || !e_el || !e_mag) return OVL_E_ARG;
-    if (!c->links_set) return fail(c, OVL_E_STATE, "no gauge links uploaded");
-    HIPCHK(c, ovl::launch_energy(c->g, c->links, c->efield, c->partial, c->dout, c->st));
+    if (!c->sys_links_set[s]) return fail(c, OVL_E_STATE, "no gauge links uploaded");
+    HIPCHK(c, ovl::launch_energy(sys_geom(c, s), sys_links(c, s), sys_efield(c, s), c->partial, c->dout, c->st));
     double h[2];
     RETCHK(fetch_out(c, h, 2));
     *e_el = h[0];
@@ -1424,38 +1777,80 @@ int ovl_energy(ovl_ctx *c, double *e_el, double *e_mag)
     return OVL_OK;
 }
 
-int ovl_gauss_violation(ovl_ctx *c, double *vmax, double *vrms)
+static int obs_gauss(ovl_ctx *c, int s, double *vmax, double *vrms)
 {
-    if (!c || !vmax || !vrms) return OVL_E_ARG;
-    if (!c->links_set) return fail(c, OVL_E_STATE, "no gauge links uploaded");
-    HIPCHK(c, ovl::launch_gauss(c->g, c->links, c->efield, c->partial, c->dout, c->st));
+    if (!c->sys_links_set[s]) return fail(c, OVL_E_STATE, "no gauge links uploaded");
+    HIPCHK(c, ovl::launch_gauss(sys_geom(c, s), sys_links(c, s), sys_efield(c, s), c->partial, c->dout, c->st));
     double h[2];
     RETCHK(fetch_out(c, h, 2));
     *vmax = h[0];
-    *vrms = std::sqrt(h[1] / ((c->g.nc * c->g.nc - 1) * (double)c->g.V));
+    *vrms = std::sqrt(h[1] / (n_adj(c) * (double)c->sys[s].V));
     return OVL_OK;
+}
+
+static int obs_unitarity(ovl_ctx *c, int s, double *vmax, double *vrms)
+{
+    if (!c->sys_links_set[s]) return fail(c, OVL_E_STATE, "no gauge links uploaded");
+    HIPCHK(c, ovl::launch_unitarity(sys_geom(c, s), sys_links(c, s), c->partial, c->dout, c->st));
+    double h[2];
+    RETCHK(fetch_out(c, h, 2));
+    *vmax = h[0];
+    *vrms = std::sqrt(h[1] / (3.0 * c->sys[s].V));
+    return OVL_OK;
+}
+
+static int obs_ncs_dot(ovl_ctx *c, int s, double *ncsdot)
+{
+    if (!c->sys_links_set[s]) return fail(c, OVL_E_STATE, "no gauge links uploaded");
+    HIPCHK(c, ovl::launch_edotb(sys_geom(c, s), sys_links(c, s), sys_efield(c, s), c->partial, c->dout, c->st));
+    double h[1];
+    RETCHK(fetch_out(c, h, 1));
+    *ncsdot = 0.125 * h[0] / (M_PI * M_PI);  // ChernSimonsDerivative.cpp:47
+    return OVL_OK;
+}
+
+int ovl_energy(ovl_ctx *c, double *e_el, double *e_mag)
+{
+    if (!c || !e_el || !e_mag) return OVL_E_ARG;
+    RETCHK(single_lattice(c, "ovl_energy"));
+    return obs_energy(c, 0, e_el, e_mag);
+}
+
+int ovl_gauss_violation(ovl_ctx *c, double *vmax, double *vrms)
+{
+    if (!c || !vmax || !vrms) return OVL_E_ARG;
+    RETCHK(single_lattice(c, "ovl_gauss_violation"));
+    return obs_gauss(c, 0, vmax, vrms);
 }
 
 int ovl_unitarity_violation(ovl_ctx *c, double *vmax, double *vrms)
 {
     if (!c || !vmax || !vrms) return OVL_E_ARG;
-    if (!c->links_set) return fail(c, OVL_E_STATE, "no gauge links uploaded");
-    HIPCHK(c, ovl::launch_unitarity(c->g, c->links, c->partial, c->dout, c->st));
-    double h[2];
-    RETCHK(fetch_out(c, h, 2));
-    *vmax = h[0];
-    *vrms = std::sqrt(h[1] / (3.0 * c->g.V));
-    return OVL_OK;
+    RETCHK(single_lattice(c, "ovl_unitarity_violation"));
+    return obs_unitarity(c, 0, vmax, vrms);
 }
 
 int ovl_ncs_dot(ovl_ctx *c, double *ncsdot)
 {
     if (!c || !ncsdot) return OVL_E_ARG;
-    if (!c->links_set) return fail(c, OVL_E_STATE, "no gauge links uploaded");
-    HIPCHK(c, ovl::launch_edotb(c->g, c->links, c->efield, c->partial, c->dout, c->st));
-    double h[1];
-    RETCHK(fetch_out(c, h, 1));
-    *ncsdot = 0.125 * h[0] / (M_PI * M_PI);  // ChernSimonsDerivative.cpp:47
+    RETCHK(single_lattice(c, "ovl_ncs_dot"));
+    return obs_ncs_dot(c, 0, ncsdot);
+}
+
+int ovl_system_observables(ovl_ctx *c, int sys, double *obs)
+{
+    if (!c || !obs) return OVL_E_ARG;
+    RETCHK(check_sys(c, sys));
+    RETCHK(obs_energy(c, sys, &obs[OVL_OBS_E_EL], &obs[OVL_OBS_E_MAG]));
+    RETCHK(obs_gauss(c, sys, &obs[OVL_OBS_GAUSS_MAX], &obs[OVL_OBS_GAUSS_RMS]));
+    RETCHK(obs_unitarity(c, sys, &obs[OVL_OBS_UNIT_MAX], &obs[OVL_OBS_UNIT_RMS]));
+    return obs_ncs_dot(c, sys, &obs[OVL_OBS_NCS_DOT]);
+}
+
+int ovl_ensemble_balance(ovl_ctx *c, double *imbalance)
+{
+    if (!c || !imbalance) return OVL_E_ARG;
+    *imbalance = c->ens_imbalance;
     return OVL_OK;
 }
 
@@ -1498,6 +1893,7 @@ int ovl_currents(ovl_ctx *c, int field, const ovl_sign_opts *o, int first_mode, 
     RETCHK(check_field(c, field));
     RETCHK(check_sign_opts(c, o));
     if (field == OVL_SCRATCH) return fail(c, OVL_E_ARG, "currents use the scratch field for -i h_ov psi");
+    RETCHK(single_lattice(c, "ovl_currents"));
     if (o->method == OVL_SIGN_ZOLOTAREV)
         RETCHK(zolo_overlap(c, c->g, c->field[field], c->field[OVL_SCRATCH], o->lmin, o->lmax, 0, 0.0));
     else
@@ -1542,9 +1938,11 @@ int ovl_set_profiling(ovl_ctx *c, int on)
 
 int ovl_autotune(ovl_ctx *c, char *report, int len)
 {
-    // Times the Clenshaw step (pure stencil + epilogue stencil) of every occupancy variant of the
-    // stencil kernel, for both precisions, on this context's lattice and mode count, and keeps the
-    // fastest per precision.  Field contents are left unchanged except the work buffers.
+    // Times the Clenshaw step (pure stencil + epilogue stencil) of every lane traversal (when the
+    // context was created with traversal 0; y-slab counts of the mode split around the L2 estimate)
+    // and every occupancy variant of the stencil kernel, for both precisions, on this context's
+    // lattice and modes, and keeps the fastest per precision.  Field contents are left unchanged
+    // except the work buffers.
     if (!c) return OVL_E_ARG;
     RETCHK(ensure_stencil(c));
     RETCHK(ensure_work(c, c->g.nblk, true));
@@ -1553,56 +1951,78 @@ int ovl_autotune(ovl_ctx *c, char *report, int len)
     HIPCHK(c, ovl::launch_convert(c->g, c->g.nblk, c->field[OVL_PSI], P64, c->b32, P32, c->st));
     HIPCHK(c, ovl::launch_lincomb(c->g, c->g.nblk, P64, c->a64, c->field[OVL_PSIMID], 1, 0, nullptr, 0, 0, c->st));
     HIPCHK(c, ovl::launch_lincomb(c->g, c->g.nblk, P64, c->b64, c->field[OVL_PSI], 1, 0, nullptr, 0, 0, c->st));
-    const int variants[4] = {1, 2, 3, 4};
     std::string rep;
     const bool prof = c->profiling;
     c->profiling = false;
+    static const char *tname[3] = {"lex", "boxes", "modes"};
     for (int prec = P64; prec <= P32; ++prec) {
+        // candidate traversals (trav, slabs)
+        std::vector<std::pair<int, int>> cand;
+        if (c->ensemble || c->prm.traversal != 0) {
+            cand.push_back({c->trav[prec], c->slabs[prec]});
+        } else {
+            if (c->xcd_order) cand.push_back({ovl::TRAV_BOXES, 1});
+            if (c->g.nblk % ovl::NUM_XCD == 0) {
+                const int t0 = default_slabs(c, prec);
+                for (int t : {t0 / 2, t0, 2 * t0})
+                    if (t >= 1 && t <= c->g.n1 && (cand.empty() || cand.back() != std::make_pair((int)ovl::TRAV_MODES, t)))
+                        cand.push_back({ovl::TRAV_MODES, t});
+            }
+            if (cand.empty()) cand.push_back({ovl::TRAV_LEX, 1});
+        }
         void *x = prec == P64 ? c->field[OVL_PSI] : c->x32;
         void *b1 = prec == P64 ? c->a64 : c->a32, *b2 = prec == P64 ? c->b64 : c->b32;
         void *w = prec == P64 ? c->w64 : c->w32;
         double best = 1e30;
         int best_v = 1;
-        for (int v : variants) {
-            c->wpe[prec] = v;
-            float ms = 0;
-            for (int rep_i = 0; rep_i < 2; ++rep_i) {  // first pass warms up
-                HIPCHK(c, hipEventRecord(c->t0, c->st));
-                for (int k = 0; k < 3; ++k) {
-                    HopArgs a = hop_args(c, prec);
-                    a.in = b1;
-                    a.out = w;
-                    a.sh_r = 1.0;
-                    RETCHK(hop(c, c->g, prec, prec, a));
-                    HopArgs q = hop_args(c, prec);
-                    q.in = w;
-                    q.out = b2;
-                    q.sh_r = 0.1;
-                    q.x = x;
-                    q.sx_r = 0.01;
-                    q.b1 = b1;
-                    q.s1_r = 0.5;
-                    q.b2 = b2;
-                    q.s2_r = -0.5;
-                    RETCHK(hop(c, c->g, prec, prec, q));
+        std::pair<int, int> best_t = cand[0];
+        for (const auto &tr : cand) {
+            RETCHK(set_traversal(c, prec, tr.first, tr.second));
+            for (int v = 1; v <= 4; ++v) {
+                c->wpe[prec] = v;
+                float ms = 0;
+                for (int rep_i = 0; rep_i < 2; ++rep_i) {  // first pass warms up
+                    HIPCHK(c, hipEventRecord(c->t0, c->st));
+                    for (int k = 0; k < 3; ++k) {
+                        HopArgs a = hop_args(c, prec);
+                        a.in = b1;
+                        a.out = w;
+                        a.sh_r = 1.0;
+                        RETCHK(hop(c, c->g, prec, prec, a));
+                        HopArgs q = hop_args(c, prec);
+                        q.in = w;
+                        q.out = b2;
+                        q.sh_r = 0.1;
+                        q.x = x;
+                        q.sx_r = 0.01;
+                        q.b1 = b1;
+                        q.s1_r = 0.5;
+                        q.b2 = b2;
+                        q.s2_r = -0.5;
+                        RETCHK(hop(c, c->g, prec, prec, q));
+                    }
+                    HIPCHK(c, hipEventRecord(c->t1, c->st));
+                    HIPCHK(c, hipEventSynchronize(c->t1));
+                    HIPCHK(c, hipEventElapsedTime(&ms, c->t0, c->t1));
                 }
-                HIPCHK(c, hipEventRecord(c->t1, c->st));
-                HIPCHK(c, hipEventSynchronize(c->t1));
-                HIPCHK(c, hipEventElapsedTime(&ms, c->t0, c->t1));
-            }
-            char buf[96];
-            std::snprintf(buf, sizeof buf, "%s wpe=%d %.1f us/step; ", prec == P64 ? "fp64" : "fp32", v, 1e3 * ms / 3);
-            rep += buf;
-            if (ms < best) {
-                best = ms;
-                best_v = v;
+                char buf[96];
+                std::snprintf(buf, sizeof buf, "%s %s/%d wpe=%d %.1f us; ", prec == P64 ? "fp64" : "fp32",
+                              tname[c->trav[prec]], c->slabs[prec], v, 1e3 * ms / 3);
+                rep += buf;
+                if (ms < best) {
+                    best = ms;
+                    best_v = v;
+                    best_t = {c->trav[prec], c->slabs[prec]};
+                }
             }
         }
+        RETCHK(set_traversal(c, prec, best_t.first, best_t.second));
         c->wpe[prec] = best_v;
     }
     c->profiling = prof;
-    char buf[64];
-    std::snprintf(buf, sizeof buf, "chosen fp64 wpe=%d fp32 wpe=%d", c->wpe[P64], c->wpe[P32]);
+    char buf[160];
+    std::snprintf(buf, sizeof buf, "chosen fp64 %s/%d wpe=%d, fp32 %s/%d wpe=%d", tname[c->trav[P64]], c->slabs[P64],
+                  c->wpe[P64], tname[c->trav[P32]], c->slabs[P32], c->wpe[P32]);
     rep += buf;
     if (report && len > 0) std::snprintf(report, len, "%s", rep.c_str());
     return OVL_OK;

@@ -16,6 +16,17 @@
 
 namespace ovl {
 
+// One lattice of an ensemble context: its fields are site-blocks [q0, q0 + nblk*V) of the
+// concatenated fields (block-major inside the system), its links start at link index l0 (= 3 x
+// the number of sites of the systems before it).
+struct SysDesc {
+    int n0, n1, n2, V;
+    int nblk;
+    int q0;
+    int l0;
+    int pad;
+};
+
 struct Geom {
     int nc, n0, n1, n2, V;
     int mb;      // modes per block
@@ -38,11 +49,23 @@ struct HopArgs {
     double amass;     // A = 6r + 2m - 2M of h_W = (1/2) gamma0 [A - hopping]
     int x_gamma0;     // 1: G = gamma0
     int nblk;         // blocks covered by this launch
-    const int *order; // XCD traversal: V sites, XCD k owns entries [k V/8, (k+1) V/8); null = lexicographic
+    int trav;         // lane traversal, TRAV_* (falls back to TRAV_LEX where the launch does not split evenly)
+    const int *order; // TRAV_BOXES: XCD k's box is order[k V/8 .. (k+1) V/8); TRAV_MODES: site order
+                      // inside a mode block (null = lexicographic)
+    // ensemble launch (null = one lattice): system table and per-workgroup work table
+    // (int2 {system, workgroup within the system}, system -1 = idle), ens_grid workgroups
+    const SysDesc *ens_sys;
+    const int2 *ens_work;
+    int ens_grid;
 };
+
+// lane traversals of the stencil launch (see lane_coords)
+enum { TRAV_LEX = 0, TRAV_BOXES = 1, TRAV_MODES = 2 };
 
 // number of XCDs the hardware round-robins workgroups over (MI355X: 8)
 constexpr int NUM_XCD = 8;
+// threads per workgroup of the stencil kernels
+constexpr int STENCIL_TPB = 256;
 
 enum Prec { P64 = 0, P32 = 1 };
 

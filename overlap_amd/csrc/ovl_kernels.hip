@@ -12,7 +12,7 @@ namespace ovl {
 
 namespace {
 
-constexpr int TPB = 256;  // threads per block for the streaming kernels
+constexpr int TPB = STENCIL_TPB;  // threads per block for the streaming kernels
 
 template <typename T> struct Vec;
 template <> struct Vec<double> { using F = double2; using L = double2; static constexpr int PER = 1; };
@@ -94,27 +94,36 @@ __device__ __forceinline__ void store_sp(void *__restrict__ base, int V, int blk
     }
 }
 
-// Lane -> (mode m, site s, block blk).  With an XCD site order the hardware's round-robin of
-// workgroups over the 8 XCDs is undone: workgroup b runs on XCD b % 8, and XCD k walks its own
-// contiguous eighth of the traversal (its spatial box of sites, every mode block in turn), so the
-// x/y/z neighbours of a site are re-read from that XCD's L2.
+// Lane -> (mode m, site s, block blk) for the three traversals (TRAV_*).  The hardware hands
+// workgroup b to XCD b % 8; the XCD traversals undo that round-robin so that XCD k runs the
+// contiguous k-th eighth of the logical lanes:
+//   TRAV_BOXES: the eighth is XCD k's spatial box of sites (order[k V/8 .. (k+1) V/8)), every
+//     mode block in turn, so x/y/z neighbours are re-read from that XCD's L2 except at the box
+//     faces;
+//   TRAV_MODES: the eighth is nblk/8 whole mode blocks over the whole lattice, sites in the order
+//     of `order` (y slabs, z-major) or lexicographic, so no site is fetched by two XCDs and the
+//     z-neighbour planes of a slab stay in L2.
 template <int MB>
-__device__ __forceinline__ bool lane_coords(const int *__restrict__ order, int V, int nblk, int &m, int &s, int &blk)
+__device__ __forceinline__ bool lane_coords(const HopArgs &a, int V, int &m, int &s, int &blk)
 {
-    const long total = (long)nblk * V * MB;
+    const long total = (long)a.nblk * V * MB;
     long lb = blockIdx.x;
-    if (order) lb = (long)(blockIdx.x % NUM_XCD) * (gridDim.x / NUM_XCD) + blockIdx.x / NUM_XCD;
+    if (a.trav != TRAV_LEX) lb = (long)(blockIdx.x % NUM_XCD) * (gridDim.x / NUM_XCD) + blockIdx.x / NUM_XCD;
     const long gl = lb * TPB + threadIdx.x;
     if (gl >= total) return false;
     m = (int)(gl % MB);
     const long r = gl / MB;
-    if (order) {
-        const long per = (long)nblk * V / NUM_XCD;  // traversal slots per XCD
+    if (a.trav == TRAV_BOXES) {
+        const long per = (long)a.nblk * V / NUM_XCD;  // traversal slots per XCD
         const int xc = (int)(r / per);
         const long rr = r - xc * per;
         const int v8 = V / NUM_XCD;
         blk = (int)(rr / v8);
-        s = order[xc * v8 + (int)(rr - (long)blk * v8)];
+        s = a.order[xc * v8 + (int)(rr - (long)blk * v8)];
+    } else if (a.trav == TRAV_MODES) {
+        blk = (int)(r / V);  // blocks are contiguous per XCD: XCD k owns [k nblk/8, (k+1) nblk/8)
+        const int i = (int)(r - (long)blk * V);
+        s = a.order ? a.order[i] : i;
     } else {
         s = (int)(r % V);
         blk = (int)(r / V);
@@ -191,38 +200,28 @@ __device__ __forceinline__ void hop_dir(const T *pr, const T *pi, const LT *__re
     }
 }
 
-// stencil kernel: out = sh*h_W(in) + sx*G(x) + s1*b1 + s2*b2 + st*tgt,
+// One (site, mode) lane of the stencil: out = sh*h_W(in) + sx*G(x) + s1*b1 + s2*b2 + st*tgt,
 // h_W = (1/2) gamma0 [A in - sum_k (1+i gamma_k) U in(x+k) + (1-i gamma_k) U^+ in(x-k)]
-// (WilsonOperator.cpp:30-114 computes -i h_W of one mode; here all modes of `nblk` blocks)
-// WPE: minimum waves per SIMD the register allocation must allow (1 = compiler's choice); the
-// stencil hoists its neighbour loads, so this trades memory-level parallelism per wave against
-// waves in flight.  The host picks the variant per problem shape (autotune).
-template <int NC, typename TI, typename TO, int MB, int WPE>
-__global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(WPE))) void k_hop(HopArgs a, int n0, int n1,
-                                                                                       int n2)
+// (WilsonOperator.cpp:30-114 computes -i h_W of one mode; here every mode of every block).
+// The site s lives on an n0 x n1 x n2 lattice; its spinors are site-block q0 + s of the fields
+// (q0 = first site-block of this lane's mode block), its links start at site index l0.
+template <int NC, typename TI, typename TO, int MB>
+__device__ __forceinline__ void hop_site(const HopArgs &a, int n0, int n1, int n2, int s, int q0, int l0, int m)
 {
     constexpr int NCOMP = 4 * NC;
     using LT = typename Vec<TI>::L;
     const int V = n0 * n1 * n2;
-    int m, s, blk;
-    if (!lane_coords<MB>(a.order, V, a.nblk, m, s, blk)) return;
-    if constexpr (MB == 64) {
-        // one site per wavefront: make the site wave-uniform so that the links (shared by all
-        // modes of the site) come through scalar loads instead of 64 identical vector lanes
-        s = __builtin_amdgcn_readfirstlane(s);
-        blk = __builtin_amdgcn_readfirstlane(blk);
-    }
     const int z = s / (n0 * n1);
     const int rem = s - z * n0 * n1;
     const int y = rem / n0;
     const int x = rem - y * n0;
 
-    const LT *__restrict__ L = reinterpret_cast<const LT *>(a.links);
+    const LT *__restrict__ L = reinterpret_cast<const LT *>(a.links) + (size_t)l0 * NC * NC;
     constexpr int NN = NC * NC;
 
     TI xr[NCOMP], xi[NCOMP];
     TI pr[NCOMP], pi[NCOMP];
-    load_sp<NC, MB, TI>(a.in, V, blk, s, m, pr, pi);
+    load_sp<NC, MB, TI>(a.in, 0, 0, q0 + s, m, pr, pi);
     const TI am = (TI)a.amass;
 #pragma unroll
     for (int c = 0; c < NCOMP; ++c) { xr[c] = am * pr[c]; xi[c] = am * pi[c]; }
@@ -230,25 +229,25 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(WPE))) void
     {  // x direction
         const int sp = s + ((x + 1 == n0) ? 1 - n0 : 1);
         const int sm = s + ((x == 0) ? n0 - 1 : -1);
-        load_sp<NC, MB, TI>(a.in, V, blk, sp, m, pr, pi);
+        load_sp<NC, MB, TI>(a.in, 0, 0, q0 + sp, m, pr, pi);
         hop_dir<NC, 0, 1>(pr, pi, L + ((size_t)0 * V + s) * NN, xr, xi);
-        load_sp<NC, MB, TI>(a.in, V, blk, sm, m, pr, pi);
+        load_sp<NC, MB, TI>(a.in, 0, 0, q0 + sm, m, pr, pi);
         hop_dir<NC, 0, -1>(pr, pi, L + ((size_t)0 * V + sm) * NN, xr, xi);
     }
     {  // y direction
         const int sp = s + ((y + 1 == n1) ? (1 - n1) * n0 : n0);
         const int sm = s + ((y == 0) ? (n1 - 1) * n0 : -n0);
-        load_sp<NC, MB, TI>(a.in, V, blk, sp, m, pr, pi);
+        load_sp<NC, MB, TI>(a.in, 0, 0, q0 + sp, m, pr, pi);
         hop_dir<NC, 1, 1>(pr, pi, L + ((size_t)1 * V + s) * NN, xr, xi);
-        load_sp<NC, MB, TI>(a.in, V, blk, sm, m, pr, pi);
+        load_sp<NC, MB, TI>(a.in, 0, 0, q0 + sm, m, pr, pi);
         hop_dir<NC, 1, -1>(pr, pi, L + ((size_t)1 * V + sm) * NN, xr, xi);
     }
     {  // z direction
         const int sp = s + ((z + 1 == n2) ? (1 - n2) * n0 * n1 : n0 * n1);
         const int sm = s + ((z == 0) ? (n2 - 1) * n0 * n1 : -n0 * n1);
-        load_sp<NC, MB, TI>(a.in, V, blk, sp, m, pr, pi);
+        load_sp<NC, MB, TI>(a.in, 0, 0, q0 + sp, m, pr, pi);
         hop_dir<NC, 2, 1>(pr, pi, L + ((size_t)2 * V + s) * NN, xr, xi);
-        load_sp<NC, MB, TI>(a.in, V, blk, sm, m, pr, pi);
+        load_sp<NC, MB, TI>(a.in, 0, 0, q0 + sm, m, pr, pi);
         hop_dir<NC, 2, -1>(pr, pi, L + ((size_t)2 * V + sm) * NN, xr, xi);
     }
 
@@ -263,8 +262,9 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(WPE))) void
         oi[c] = shr * hi + shi * hr;
     }
     TO tr[NCOMP], ti[NCOMP];
+    const int q = q0 + s;
     if (a.x) {
-        load_sp_nt<NC, MB, TO>(a.x, V, blk, s, m, tr, ti);
+        load_sp_nt<NC, MB, TO>(a.x, 0, 0, q, m, tr, ti);
         const TO cr = (TO)a.sx_r, ci = (TO)a.sx_i;
 #pragma unroll
         for (int c = 0; c < NCOMP; ++c) {
@@ -275,24 +275,65 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(WPE))) void
         }
     }
     if (a.b1) {
-        load_sp_nt<NC, MB, TO>(a.b1, V, blk, s, m, tr, ti);
+        load_sp_nt<NC, MB, TO>(a.b1, 0, 0, q, m, tr, ti);
         const TO cr = (TO)a.s1_r, ci = (TO)a.s1_i;
 #pragma unroll
         for (int c = 0; c < NCOMP; ++c) { orr[c] += cr * tr[c] - ci * ti[c]; oi[c] += cr * ti[c] + ci * tr[c]; }
     }
     if (a.b2) {
-        load_sp_nt<NC, MB, TO>(a.b2, V, blk, s, m, tr, ti);
+        load_sp_nt<NC, MB, TO>(a.b2, 0, 0, q, m, tr, ti);
         const TO cr = (TO)a.s2_r, ci = (TO)a.s2_i;
 #pragma unroll
         for (int c = 0; c < NCOMP; ++c) { orr[c] += cr * tr[c] - ci * ti[c]; oi[c] += cr * ti[c] + ci * tr[c]; }
     }
     if (a.tgt) {
-        load_sp_nt<NC, MB, TO>(a.tgt, V, blk, s, m, tr, ti);
+        load_sp_nt<NC, MB, TO>(a.tgt, 0, 0, q, m, tr, ti);
         const TO cr = (TO)a.st_r, ci = (TO)a.st_i;
 #pragma unroll
         for (int c = 0; c < NCOMP; ++c) { orr[c] += cr * tr[c] - ci * ti[c]; oi[c] += cr * ti[c] + ci * tr[c]; }
     }
-    store_sp<NC, MB, TO>(a.out, V, blk, s, m, orr, oi);
+    store_sp<NC, MB, TO>(a.out, 0, 0, q, m, orr, oi);
+}
+
+// WPE: minimum waves per SIMD the register allocation must allow (1 = compiler's choice); the
+// stencil hoists its neighbour loads, so this trades memory-level parallelism per wave against
+// waves in flight.  The host picks the variant per problem shape (autotune).
+template <int NC, typename TI, typename TO, int MB, int WPE>
+__global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(WPE))) void k_hop(HopArgs a, int n0, int n1,
+                                                                                       int n2)
+{
+    const int V = n0 * n1 * n2;
+    int m, s, blk;
+    if (!lane_coords<MB>(a, V, m, s, blk)) return;
+    if constexpr (MB == 64) {
+        // one site per wavefront: make the site wave-uniform so that the links (shared by all
+        // modes of the site) come through scalar loads instead of 64 identical vector lanes
+        s = __builtin_amdgcn_readfirstlane(s);
+        blk = __builtin_amdgcn_readfirstlane(blk);
+    }
+    hop_site<NC, TI, TO, MB>(a, n0, n1, n2, s, blk * V, 0, m);
+}
+
+// Ensemble launch: many independent lattices of different sizes in one grid.  Workgroup b works on
+// workgroup ens_work[b].y of system ens_work[b].x; the host lays the table out so that the
+// workgroups of one system all run on one XCD (whole systems per XCD L2, balanced by size).
+template <int NC, typename TI, typename TO, int MB, int WPE>
+__global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(WPE))) void k_hop_ens(
+    HopArgs a)
+{
+    const int2 w = a.ens_work[blockIdx.x];
+    if (w.x < 0) return;
+    const SysDesc d = a.ens_sys[w.x];
+    const int gl = w.y * TPB + threadIdx.x;  // lane within the system
+    if (gl >= d.V * d.nblk * MB) return;
+    const int m = gl % MB;
+    int r = gl / MB;
+    int s = r % d.V, blk = r / d.V;
+    if constexpr (MB == 64) {
+        s = __builtin_amdgcn_readfirstlane(s);
+        blk = __builtin_amdgcn_readfirstlane(blk);
+    }
+    hop_site<NC, TI, TO, MB>(a, d.n0, d.n1, d.n2, s, d.q0 + blk * d.V, d.l0, m);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1109,14 +1150,38 @@ void hop_wpe(const HopArgs &b, unsigned nb, int wpe, const Geom &g, hipStream_t 
     }
 }
 
+template <int NC, typename TI, typename TO, int MB>
+void hop_ens_wpe(const HopArgs &b, int wpe, hipStream_t st)
+{
+    const unsigned nb = (unsigned)b.ens_grid;
+    switch (wpe) {
+    case 2: k_hop_ens<NC, TI, TO, MB, 2><<<nb, TPB, 0, st>>>(b); break;
+    case 3: k_hop_ens<NC, TI, TO, MB, 3><<<nb, TPB, 0, st>>>(b); break;
+    case 4: k_hop_ens<NC, TI, TO, MB, 4><<<nb, TPB, 0, st>>>(b); break;
+    default: k_hop_ens<NC, TI, TO, MB, 1><<<nb, TPB, 0, st>>>(b); break;
+    }
+}
+
 template <int NC, typename TI, typename TO>
 hipError_t hop_mb(const Geom &g, const HopArgs &a, int wpe, hipStream_t st)
 {
+    if (a.ens_sys) {
+        if (a.ens_grid <= 0) return hipErrorInvalidValue;
+        switch (g.mb) {
+        case 16: hop_ens_wpe<NC, TI, TO, 16>(a, wpe, st); break;
+        case 32: hop_ens_wpe<NC, TI, TO, 32>(a, wpe, st); break;
+        case 64: hop_ens_wpe<NC, TI, TO, 64>(a, wpe, st); break;
+        default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
     const long total = (long)a.nblk * g.V * g.mb;
     const unsigned nb = nblocks(total);
     HopArgs b = a;
-    // the XCD traversal needs whole workgroups spread evenly over the XCDs
-    if (total % ((long)TPB * NUM_XCD) != 0 || g.V % NUM_XCD != 0) b.order = nullptr;
+    // the XCD traversals need whole workgroups spread evenly over the XCDs
+    const bool even = total % ((long)TPB * NUM_XCD) == 0;
+    if (b.trav == TRAV_BOXES && (!even || g.V % NUM_XCD != 0 || !b.order)) b.trav = TRAV_LEX;
+    if (b.trav == TRAV_MODES && (!even || a.nblk % NUM_XCD != 0)) b.trav = TRAV_LEX;
     switch (g.mb) {
     case 16: hop_wpe<NC, TI, TO, 16>(b, nb, wpe, g, st); break;
     case 32: hop_wpe<NC, TI, TO, 32>(b, nb, wpe, g, st); break;

@@ -28,14 +28,25 @@ SYMBOLS = [
     "ovl_wilson_evo_compute", "ovl_overlap_operator_compute", "ovl_zolotarev_coefficients",
     "ovl_update_links", "ovl_update_efield", "ovl_fermion_update",
     "ovl_energy", "ovl_gauss_violation", "ovl_unitarity_violation", "ovl_ncs_dot", "ovl_currents",
-    "ovl_mode_norms", "ovl_get_timing", "ovl_set_profiling", "ovl_autotune",
+    "ovl_mode_norms",
+    "ovl_create_ensemble", "ovl_set_system_links", "ovl_get_system_links", "ovl_set_system_efield",
+    "ovl_get_system_efield", "ovl_set_system_u1_flux", "ovl_set_system_modes", "ovl_get_system_modes",
+    "ovl_system_observables", "ovl_ensemble_balance",
+    "ovl_get_timing", "ovl_set_profiling", "ovl_autotune",
 ]
+OBS_NAMES = ("e_el", "e_mag", "gauss_max", "gauss_rms", "unit_max", "unit_rms", "ncs_dot")
 
 
 class Params(C.Structure):
     _fields_ = [("nc", C.c_int), ("n", C.c_int * 3), ("n_modes", C.c_int), ("r_wilson", C.c_double),
                 ("m_fermion", C.c_double), ("m_dw", C.c_double), ("mode_block", C.c_int),
                 ("reunitarize", C.c_int), ("traversal", C.c_int)]
+
+
+class EnsembleParams(C.Structure):
+    _fields_ = [("nc", C.c_int), ("n_systems", C.c_int), ("dims", C.POINTER(C.c_int)), ("n_modes", C.c_int),
+                ("r_wilson", C.c_double), ("m_fermion", C.c_double), ("m_dw", C.c_double),
+                ("mode_block", C.c_int), ("reunitarize", C.c_int)]
 
 
 class SignOpts(C.Structure):
@@ -67,9 +78,10 @@ def lib():
         _lib.ovl_error.restype = C.c_char_p
         _lib.ovl_zolotarev_coefficients.restype = C.c_double
         _lib.ovl_create.argtypes = [C.POINTER(Params), C.POINTER(C.c_void_p)]
+        _lib.ovl_create_ensemble.argtypes = [C.POINTER(EnsembleParams), C.POINTER(C.c_void_p)]
         for name in SYMBOLS:
             f = getattr(_lib, name)
-            if name not in ("ovl_error", "ovl_zolotarev_coefficients", "ovl_sign_defaults", "ovl_create"):
+            if name not in ("ovl_error", "ovl_zolotarev_coefficients", "ovl_sign_defaults"):
                 f.restype = C.c_int
         _lib.ovl_sign_defaults.restype = None
     return _lib
@@ -115,6 +127,7 @@ class Context:
                  traversal=0):
         self.nc, self.n, self.n_modes = nc, tuple(n), n_modes
         self.V = int(np.prod(n))
+        self.dims, self.Vs = [self.n], [self.V]
         p = Params(nc, (C.c_int * 3)(*n), n_modes, r_wilson, m_fermion, m_dw, mode_block, reunitarize, traversal)
         h = C.c_void_p()
         rc = lib().ovl_create(C.byref(p), C.byref(h))
@@ -269,6 +282,84 @@ class Context:
         self._chk(lib().ovl_set_profiling(self.h, int(on)))
 
     def autotune(self) -> str:
-        buf = C.create_string_buffer(1024)
-        self._chk(lib().ovl_autotune(self.h, buf, 1024))
+        buf = C.create_string_buffer(4096)
+        self._chk(lib().ovl_autotune(self.h, buf, 4096))
         return buf.value.decode()
+
+    # per-system calls (system 0 of an ordinary context)
+    def _sys_volume(self, s):
+        if not 0 <= s < len(self.Vs):
+            raise OvlError(E_ARG, "system index out of range")
+        return self.Vs[s]
+
+    def set_system_links(self, s, links):
+        self._chk(lib().ovl_set_system_links(self.h, s, _d(np.ascontiguousarray(links, np.float64))))
+
+    def get_system_links(self, s):
+        out = np.zeros(self._sys_volume(s) * 3 * (4 if self.nc == 2 else 18))
+        self._chk(lib().ovl_get_system_links(self.h, s, _d(out)))
+        return out
+
+    def set_system_efield(self, s, e):
+        self._chk(lib().ovl_set_system_efield(self.h, s, _d(np.ascontiguousarray(e, np.float64))))
+
+    def get_system_efield(self, s):
+        out = np.zeros(self._sys_volume(s) * 3 * (self.nc * self.nc - 1))
+        self._chk(lib().ovl_get_system_efield(self.h, s, _d(out)))
+        return out
+
+    def set_system_u1_flux(self, s, nb, q=1.0):
+        self._chk(lib().ovl_set_system_u1_flux(self.h, s, C.c_int(nb), C.c_double(q)))
+
+    def set_system_modes(self, s, field, psi, first=0):
+        psi = np.ascontiguousarray(np.atleast_2d(psi), np.complex128)
+        self._chk(lib().ovl_set_system_modes(self.h, s, field, psi.ctypes.data_as(C.POINTER(C.c_double)), first,
+                                             psi.shape[0]))
+
+    def get_system_modes(self, s, field, first=0, count=None):
+        count = self.n_modes - first if count is None else count
+        out = np.zeros((count, self._sys_volume(s) * self.nc * 4), np.complex128)
+        self._chk(lib().ovl_get_system_modes(self.h, s, field, out.ctypes.data_as(C.POINTER(C.c_double)), first,
+                                             count))
+        return out
+
+    def system_observables(self, s) -> dict:
+        out = np.zeros(len(OBS_NAMES))
+        self._chk(lib().ovl_system_observables(self.h, s, _d(out)))
+        return dict(zip(OBS_NAMES, out.tolist()))
+
+
+class Ensemble(Context):
+    """Many independent lattices (systems) of different sizes in one device context.
+
+    Every stencil application runs as one kernel launch over all systems; links, E fields, U(1)
+    flux and modes are set and read per system.
+    """
+
+    def __init__(self, nc, dims, n_modes, r_wilson=1.0, m_fermion=0.0, m_dw=0.0, mode_block=0, reunitarize=0):
+        self.nc, self.n_modes = nc, n_modes
+        self.dims = [tuple(int(v) for v in d) for d in dims]
+        self.Vs = [int(np.prod(d)) for d in self.dims]
+        self.V = sum(self.Vs)
+        flat = (C.c_int * (3 * len(self.dims)))(*[v for d in self.dims for v in d])
+        p = EnsembleParams(nc, len(self.dims), C.cast(flat, C.POINTER(C.c_int)), n_modes, r_wilson, m_fermion, m_dw,
+                           mode_block, reunitarize)
+        h = C.c_void_p()
+        rc = lib().ovl_create_ensemble(C.byref(p), C.byref(h))
+        if rc != OK:
+            raise OvlError(rc, "ovl_create_ensemble failed")
+        self.h = h
+
+    @property
+    def n_systems(self):
+        return len(self.dims)
+
+    def balance(self) -> float:
+        r = C.c_double()
+        self._chk(lib().ovl_ensemble_balance(self.h, C.byref(r)))
+        return r.value
+
+    def mode_norms(self, field):
+        out = np.zeros(self.n_systems * self.n_modes)
+        self._chk(lib().ovl_mode_norms(self.h, field, _d(out)))
+        return out.reshape(self.n_systems, self.n_modes)

@@ -280,20 +280,26 @@ def test_error_paths(gpu):
     assert ex.value.code == ovl.E_SPECTRUM
 
 
-@pytest.mark.parametrize("nc,n,nm,mb", [(3, (4, 4, 4), 64, 64), (2, (8, 4, 6), 32, 16), (3, (6, 4, 4), 64, 32)])
-def test_xcd_traversal_is_a_permutation(gpu, nc, n, nm, mb):
-    # the XCD box traversal only reorders lanes: results equal the lexicographic launch bit for bit
+@pytest.mark.parametrize("nc,n,nm,mb", [(3, (4, 4, 4), 64, 64), (2, (8, 4, 6), 32, 16), (3, (6, 4, 4), 64, 32),
+                                        (3, (6, 4, 4), 128, 16), (2, (8, 4, 6), 128, 16), (3, (4, 4, 8), 512, 64)])
+def test_traversals_are_permutations(gpu, nc, n, nm, mb):
+    # every lane traversal (lexicographic, XCD boxes, XCD mode split with y slabs, and whatever the
+    # autotuner picks) only reorders lanes: results equal bit for bit
+    print(f"[case] traversals nc={nc} n={n} modes={nm} mb={mb}", flush=True)
     outs = []
-    for trav in (0, 1):
-        g = cf.rng(404)
-        links = cf.su2_links(n, 0.4, g) if nc == 2 else cf.su3_links(n, 0.4, g)
-        psi = cf.spinors(n, nc, nm, g)
+    g = cf.rng(404)
+    links = cf.su2_links(n, 0.4, g) if nc == 2 else cf.su3_links(n, 0.4, g)
+    psi = cf.spinors(n, nc, nm, g)
+    for trav in (1, 2, 3, 0):
         with ovl.Context(nc, n, nm, m_fermion=0.1, m_dw=1.2, mode_block=mb, traversal=trav) as ctx:
             ctx.set_links(links)
             ctx.set_modes(ovl.PSI, psi)
+            if trav == 0:
+                print("  autotune:", ctx.autotune(), flush=True)
             ctx.apply_sign(ovl.PSI, ovl.PSIMID, ovl.sign_opts(degree=32, cut=0.1, precision=ovl.PREC_MIXED, head=8))
             outs.append(ctx.get_modes(ovl.PSIMID))
-    assert np.array_equal(outs[0], outs[1])
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0])
     L = orc.lat(nc, n, 1.0, 0.1, 1.2)
     ucc = orc.links_to_ucc(L, links)
     coef = orc.cheb_coeffs(32, 0.1)
