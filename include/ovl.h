@@ -192,6 +192,21 @@ int ovl_system_observables(ovl_ctx *ctx, int sys, double *obs);
 /* stencil work balance of an ensemble: max over XCDs of the workgroups assigned / mean (1 = even) */
 int ovl_ensemble_balance(ovl_ctx *ctx, double *imbalance);
 
+/* GaussLawRestoration::Restore (GAUSSLAWFIXING/GaussRestore.cpp:129-160) on system `sys` (0 for an
+ * ordinary context): relaxation sweeps E_mu(x) -= gamma (G(x) - U_mu(x) G(x+mu) U_mu(x)^+) with
+ * gamma = 7/24, 7/48, 7/72 per iteration until max |G^a| <= tol (the reference's 10^(1 - digits)) or
+ * max_iter iterations (the reference: 10000); iters and vmax (violation of the final field) may be NULL */
+int ovl_gauss_restore(ovl_ctx *ctx, int sys, double tol, int max_iter, int *iters, double *vmax);
+
+/* ---- configuration files ----
+ * IO::SaveConfiguration / IO::SaveGaugeLinks (IO/SaveConfiguration.cpp:3-63) and
+ * IO::LoadConfiguration (IO/LoadConfiguration.cpp:3-144) for system `sys` (0 for an ordinary
+ * context): text, one line per link "x y z mu <link reals>" and "x y z mu <E components>", sites
+ * in storage order, 17 significant digits (FP64 round trip).  efile may be NULL (links only).
+ * Loading checks that every link appears exactly once (the reference exits on a wrong count). */
+int ovl_save_configuration(ovl_ctx *ctx, int sys, const char *ufile, const char *efile);
+int ovl_load_configuration(ovl_ctx *ctx, int sys, const char *ufile, const char *efile);
+
 /* ---- timing of the last ovl_apply_* / ovl_fermion_update call ---- */
 typedef struct {
     double total_ms;          /* event-timed wall time of the call on the stream */

@@ -16,6 +16,8 @@
  *   orc_efield_update     Gauss-law conservation and energy conservation  (the reference's
  *                         plaquette macros are absent: formula restated from the Hamiltonian)
  *   orc_edotb             gauge invariance, E=0, linearity in E
+ *   orc_gauss_restore     finite-difference gradient of (1/2) sum G^2 (the step direction) and
+ *                         convergence to G = 0 (tests); the reference macro for G is absent
  *   orc_currents          vector charge = sum_s w_s |psi_s|^2 / (2V) identity; hermiticity
  */
 #include "ovl_oracle.h"
@@ -940,6 +942,102 @@ void orc_gauss_violation(const orc_lat *L, const double *links, const double *ef
     }
     *gmax = mx;
     *grms = sqrt(sq / (ng * V));
+}
+
+/* G^a(x) of every site, signed ([site][a]), as orc_gauss_violation forms it */
+static void gauss_field(const orc_lat *L, const double *links, const double *efield, double *g)
+{
+    int V = lat_vol(L), nc = L->nc, ng = ngen(nc);
+    int d[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    for (int s = 0; s < V; s++) {
+        int x, y, z;
+        coords_of(L->n, s, &x, &y, &z);
+        cplx G[9] = {0};
+        for (int mu = 0; mu < 3; mu++) {
+            cplx e0[9], e1[9], u[9], ud[9], t[9];
+            alg_matrix(nc, efield + ng * (mu + 3 * s), e0);
+            int sm = site_of(L->n, x - d[mu][0], y - d[mu][1], z - d[mu][2]);
+            alg_matrix(nc, efield + ng * (mu + 3 * sm), e1);
+            link_matrix(nc, links, sm, mu, u);
+            m_dag(nc, u, ud);
+            m_mul(nc, ud, e1, t);
+            m_mul(nc, t, u, t);
+            for (int i = 0; i < nc * nc; i++) G[i] += e0[i] - t[i];
+        }
+        for (int a = 0; a < ng; a++) {
+            cplx ta[9], tr = 0;
+            gen_matrix(nc, a, ta);
+            for (int i = 0; i < nc; i++)
+                for (int k = 0; k < nc; k++) tr += ta[i * nc + k] * G[k * nc + i];
+            g[ng * s + a] = 2.0 * creal(tr);
+        }
+    }
+}
+
+/* GaussLawRestoration::UpdateStep (GaussRestore.cpp:115-120): UpdateDeviation stores G^a(x)
+ * (:13-61), UpdateElectricFields sets E_mu(x) -= gamma (G(x) - V G(x+mu) V^+) with V = U_mu(x)
+ * (:74-112, AdjointMultiplication SU2/AlgebraOperations.cpp:76-83).  Returns max |G^a| of the
+ * field before the update. */
+double orc_gauss_restore_step(const orc_lat *L, const double *links, double *efield, double gamma)
+{
+    int V = lat_vol(L), nc = L->nc, ng = ngen(nc);
+    int d[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    double *g = malloc(sizeof(double) * (size_t)ng * V);
+    double mx = 0;
+    gauss_field(L, links, efield, g);
+    for (int i = 0; i < ng * V; i++) mx = fmax(mx, fabs(g[i]));
+    for (int s = 0; s < V; s++) {
+        int x, y, z;
+        coords_of(L->n, s, &x, &y, &z);
+        for (int mu = 0; mu < 3; mu++) {
+            int sp = site_of(L->n, x + d[mu][0], y + d[mu][1], z + d[mu][2]);
+            cplx u[9], ud[9], gm[9], t[9];
+            link_matrix(nc, links, s, mu, u);
+            m_dag(nc, u, ud);
+            alg_matrix(nc, g + ng * sp, gm);
+            m_mul(nc, u, gm, t);
+            m_mul(nc, t, ud, t);
+            for (int a = 0; a < ng; a++) {
+                cplx ta[9], tr = 0;
+                gen_matrix(nc, a, ta);
+                for (int i = 0; i < nc; i++)
+                    for (int j = 0; j < nc; j++) tr += ta[i * nc + j] * t[j * nc + i];
+                efield[ng * (mu + 3 * s) + a] -= gamma * (g[ng * s + a] - 2.0 * creal(tr));
+            }
+        }
+    }
+    free(g);
+    return mx;
+}
+
+/* GaussLawRestoration::Restore (GaussRestore.cpp:129-160): the three steps gamma = 7/24, 7/48,
+ * 7/72 are repeated while the maximum measured by the last step exceeds tol and fewer than
+ * max_iter iterations ran.  The reference's maximum is max_a,x G^a(x) (signed, through a macro
+ * absent from the reference); this one is max |G^a(x)|.  Returns the iteration count; *vmax =
+ * max |G^a| of the final field. */
+int orc_gauss_restore(const orc_lat *L, const double *links, double *efield, double tol, int max_iter,
+                      double *vmax)
+{
+    int V = lat_vol(L), ng = ngen(L->nc);
+    double *g = malloc(sizeof(double) * (size_t)ng * V);
+    const double gam[3] = {7.0 / 24.0, 7.0 / 48.0, 7.0 / 72.0};
+    double mx = 0;
+    gauss_field(L, links, efield, g);
+    for (int i = 0; i < ng * V; i++) mx = fmax(mx, fabs(g[i]));
+    int it = 0;
+    while (it < max_iter && mx > tol) {
+        for (int k = 0; k < 3; k++) {
+            double m = orc_gauss_restore_step(L, links, efield, gam[k]);
+            if (k == 2) mx = m;
+        }
+        it++;
+    }
+    gauss_field(L, links, efield, g);
+    mx = 0;
+    for (int i = 0; i < ng * V; i++) mx = fmax(mx, fabs(g[i]));
+    *vmax = mx;
+    free(g);
+    return it;
 }
 
 /* UnitarityViolation.cpp:13-51 with UnitarityNorm (SU2/GroupOperations.cpp:172-184) generalised:

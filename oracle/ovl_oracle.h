@@ -84,6 +84,11 @@ void orc_plaquette_energy(const orc_lat *L, const double *links, const double *e
 void orc_gauss_violation(const orc_lat *L, const double *links, const double *efield,
                          double *gmax, double *grms);
 void orc_unitarity_violation(const orc_lat *L, const double *links, double *umax, double *urms);
+/* GaussLawRestoration::UpdateStep (GaussRestore.cpp:115-120); returns max |G^a| before the step */
+double orc_gauss_restore_step(const orc_lat *L, const double *links, double *efield, double gamma);
+/* GaussLawRestoration::Restore (GaussRestore.cpp:129-160); returns iterations, *vmax final max |G^a| */
+int orc_gauss_restore(const orc_lat *L, const double *links, double *efield, double tol, int max_iter,
+                      double *vmax);
 double orc_edotb(const orc_lat *L, const double *links, const double *efield);
 void orc_currents(const orc_lat *L, const double *psi, const double *hpsi, int nmodes,
                   int first_mode, double *jv, double *ja);

@@ -11,6 +11,8 @@
 #include <complex>
 #include <cstdio>
 #include <cstring>
+#include <fstream>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -1780,7 +1782,8 @@ static int obs_energy(ovl_ctx *c, int s, double *e_el, double *e_mag)
 static int obs_gauss(ovl_ctx *c, int s, double *vmax, double *vrms)
 {
     if (!c->sys_links_set[s]) return fail(c, OVL_E_STATE, "no gauge links uploaded");
-    HIPCHK(c, ovl::launch_gauss(sys_geom(c, s), sys_links(c, s), sys_efield(c, s), c->partial, c->dout, c->st));
+    HIPCHK(c, ovl::launch_gauss(sys_geom(c, s), sys_links(c, s), sys_efield(c, s), c->partial, c->dout, nullptr,
+                                c->st));
     double h[2];
     RETCHK(fetch_out(c, h, 2));
     *vmax = h[0];
@@ -1845,6 +1848,47 @@ int ovl_system_observables(ovl_ctx *c, int sys, double *obs)
     RETCHK(obs_gauss(c, sys, &obs[OVL_OBS_GAUSS_MAX], &obs[OVL_OBS_GAUSS_RMS]));
     RETCHK(obs_unitarity(c, sys, &obs[OVL_OBS_UNIT_MAX], &obs[OVL_OBS_UNIT_RMS]));
     return obs_ncs_dot(c, sys, &obs[OVL_OBS_NCS_DOT]);
+}
+
+// GaussLawRestoration::Restore (GaussRestore.cpp:129-160): relaxation sweeps of E with step
+// sizes 7/24, 7/48, 7/72 until max_a,x |G^a(x)| <= tol (checked on the violation measured at the
+// start of the last sweep, as the reference does) or max_iter iterations
+int ovl_gauss_restore(ovl_ctx *c, int sys, double tol, int max_iter, int *iters, double *vmax)
+{
+    if (!c || !(tol >= 0.0) || max_iter < 0) return OVL_E_ARG;
+    RETCHK(check_sys(c, sys));
+    if (!c->sys_links_set[sys]) return fail(c, OVL_E_STATE, "no gauge links uploaded");
+    const Geom g = sys_geom(c, sys);
+    double *gf = nullptr;
+    RETCHK(dalloc(c, (void **)&gf, sizeof(double) * n_adj(c) * (size_t)g.V));
+    const double *L = sys_links(c, sys);
+    double *E = sys_efield(c, sys);
+    auto deviation = [&](double *mx) -> int {
+        HIPCHK(c, ovl::launch_gauss(g, L, E, c->partial, c->dout, gf, c->st));
+        if (mx) {
+            double h[2];
+            RETCHK(fetch_out(c, h, 2));
+            *mx = h[0];
+        }
+        return OVL_OK;
+    };
+    double mx = 0;
+    int it = 0, r = deviation(&mx);
+    const double gam[3] = {7.0 / 24.0, 7.0 / 48.0, 7.0 / 72.0};
+    while (r == OVL_OK && it < max_iter && mx > tol) {
+        for (int k = 0; k < 3 && r == OVL_OK; ++k) {
+            r = deviation(k == 2 ? &mx : nullptr);
+            if (r == OVL_OK && ovl::launch_gauss_restore(g, L, gf, E, gam[k], c->st) != hipSuccess)
+                r = fail(c, OVL_E_HIP, "gauss restore launch failed");
+        }
+        ++it;
+    }
+    if (r == OVL_OK) r = deviation(&mx);  // violation of the final field
+    hipStreamSynchronize(c->st);
+    hipFree(gf);
+    if (iters) *iters = it;
+    if (vmax) *vmax = mx;
+    return r;
 }
 
 int ovl_ensemble_balance(ovl_ctx *c, double *imbalance)
@@ -1919,6 +1963,95 @@ int ovl_currents(ovl_ctx *c, int field, const ovl_sign_opts *o, int first_mode, 
         for (int i = 0; i < c->g.V; ++i) s += ha[4 * i];
         *axial_charge = s;
     }
+    return OVL_OK;
+}
+
+// ---- configuration files (IO/SaveConfiguration.cpp:3-63, IO/LoadConfiguration.cpp:3-144) ----
+// One line per link, sites in storage order (z, y, x outermost to innermost, then mu):
+//   links   "x y z mu v_0 v_1 ... " with the link's stored reals (SU(2): u0..u3; SU(3): re im of
+//           the 9 entries, column-major), each followed by a space (MatrixToString)
+//   E field "x y z mu E_0 ... E_{Nc^2-2}"
+// written with 17 significant digits so that FP64 values read back bit for bit.
+int ovl_save_configuration(ovl_ctx *c, int sys, const char *ufile, const char *efile)
+{
+    if (!c || !ufile) return OVL_E_ARG;
+    RETCHK(check_sys(c, sys));
+    const SysDesc &d = c->sys[sys];
+    const int lr = link_reals(c), ng = n_adj(c);
+    std::vector<double> u((size_t)lr * 3 * d.V), e;
+    RETCHK(get_doubles(c, u.data(), sys_links(c, sys), u.size()));
+    if (efile) {
+        e.resize((size_t)ng * 3 * d.V);
+        RETCHK(get_doubles(c, e.data(), sys_efield(c, sys), e.size()));
+    }
+    FILE *fu = std::fopen(ufile, "w");
+    if (!fu) return fail(c, OVL_E_ARG, std::string("cannot open ") + ufile);
+    FILE *fe = efile ? std::fopen(efile, "w") : nullptr;
+    if (efile && !fe) {
+        std::fclose(fu);
+        return fail(c, OVL_E_ARG, std::string("cannot open ") + efile);
+    }
+    for (int z = 0; z < d.n2; ++z)
+        for (int y = 0; y < d.n1; ++y)
+            for (int x = 0; x < d.n0; ++x)
+                for (int mu = 0; mu < 3; ++mu) {
+                    const size_t l = (size_t)mu + 3 * ((size_t)x + (size_t)d.n0 * (y + (size_t)d.n1 * z));
+                    std::fprintf(fu, "%d %d %d %d ", x, y, z, mu);
+                    for (int a = 0; a < lr; ++a) std::fprintf(fu, "%.17g ", u[l * lr + a]);
+                    std::fputc('\n', fu);
+                    if (fe) {
+                        std::fprintf(fe, "%d %d %d %d", x, y, z, mu);
+                        for (int a = 0; a < ng; ++a) std::fprintf(fe, " %.17g", e[l * ng + a]);
+                        std::fputc('\n', fe);
+                    }
+                }
+    const bool ok = std::fclose(fu) == 0 && (!fe || std::fclose(fe) == 0);
+    if (!ok) return fail(c, OVL_E_ARG, "write error");
+    return OVL_OK;
+}
+
+// reads "x y z mu v_0 .. v_{nv-1}" lines into out[nv * (mu + 3 site)]; every link exactly once
+static int read_link_file(ovl_ctx *c, const SysDesc &d, const char *path, int nv, std::vector<double> &out)
+{
+    std::ifstream in(path);
+    if (!in) return fail(c, OVL_E_ARG, std::string("cannot open ") + path);
+    out.assign((size_t)nv * 3 * d.V, 0.0);
+    std::vector<char> seen((size_t)3 * d.V, 0);
+    std::string line;
+    long count = 0, lineno = 0;
+    while (std::getline(in, line)) {
+        ++lineno;
+        if (line.find_first_not_of(" \t\r") == std::string::npos) continue;
+        std::istringstream ls(line);
+        int x, y, z, mu;
+        if (!(ls >> x >> y >> z >> mu) || x < 0 || x >= d.n0 || y < 0 || y >= d.n1 || z < 0 || z >= d.n2 || mu < 0 ||
+            mu > 2)
+            return fail(c, OVL_E_ARG, std::string(path) + ":" + std::to_string(lineno) + ": bad site or direction");
+        const size_t l = (size_t)mu + 3 * ((size_t)x + (size_t)d.n0 * (y + (size_t)d.n1 * z));
+        for (int a = 0; a < nv; ++a)
+            if (!(ls >> out[l * nv + a]) || !std::isfinite(out[l * nv + a]))
+                return fail(c, OVL_E_ARG, std::string(path) + ":" + std::to_string(lineno) + ": expected " +
+                                              std::to_string(nv) + " finite values");
+        if (seen[l]) return fail(c, OVL_E_ARG, std::string(path) + ":" + std::to_string(lineno) + ": link given twice");
+        seen[l] = 1;
+        ++count;
+    }
+    if (count != 3L * d.V)
+        return fail(c, OVL_E_ARG, std::string(path) + ": " + std::to_string(count) + " links read, " +
+                                      std::to_string(3L * d.V) + " expected");
+    return OVL_OK;
+}
+
+int ovl_load_configuration(ovl_ctx *c, int sys, const char *ufile, const char *efile)
+{
+    if (!c || !ufile) return OVL_E_ARG;
+    RETCHK(check_sys(c, sys));
+    const SysDesc d = c->sys[sys];
+    std::vector<double> u, e;
+    RETCHK(read_link_file(c, d, ufile, link_reals(c), u));
+    if (efile) RETCHK(read_link_file(c, d, efile, n_adj(c), e));
+    RETCHK(ovl_set_system_links(c, sys, u.data()));
+    if (efile) RETCHK(ovl_set_system_efield(c, sys, e.data()));
     return OVL_OK;
 }
 

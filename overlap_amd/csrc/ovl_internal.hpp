@@ -93,8 +93,12 @@ size_t mode_dot_partial_len(const Geom &g);
 // global observables, written to device `out`
 hipError_t launch_energy(const Geom &g, const double *links, const double *efield, double *partial, double *out,
                          hipStream_t st);
+// gfield (optional): G^a(x) stored as [site][a]
 hipError_t launch_gauss(const Geom &g, const double *links, const double *efield, double *partial, double *out,
-                        hipStream_t st);
+                        double *gfield, hipStream_t st);
+// E_mu(x) -= gamma (G(x) - U_mu(x) G(x+mu) U_mu(x)^+) from a stored G
+hipError_t launch_gauss_restore(const Geom &g, const double *links, const double *gfield, double *efield, double gamma,
+                                hipStream_t st);
 hipError_t launch_unitarity(const Geom &g, const double *links, double *partial, double *out, hipStream_t st);
 hipError_t launch_edotb(const Geom &g, const double *links, const double *efield, double *partial, double *out,
                         hipStream_t st);

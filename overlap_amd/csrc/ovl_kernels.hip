@@ -808,43 +808,81 @@ __global__ __launch_bounds__(TPB) void k_energy(const double *__restrict__ links
     if (threadIdx.x == 0) { partial[2 * blockIdx.x] = v[0]; partial[2 * blockIdx.x + 1] = v[1]; }
 }
 
-// Gauss law G(x) = sum_mu E_mu(x) - U_mu(x-mu)^+ E_mu(x-mu) U_mu(x-mu): max |G^a| and sum G^2
+// components X^a = 2 tr(t^a X) of a Hermitian traceless matrix, via ReTr(i t^a (-i X))
+template <int NC> __device__ __forceinline__ void alg_components(const Mat<NC> &X, double *out)
+{
+    Mat<NC> mx;
+#pragma unroll
+    for (int k = 0; k < NC * NC; ++k) mx.e[k] = {X.e[k].i, -X.e[k].r};
+    retr_it<NC>(mx, out);
+#pragma unroll
+    for (int a = 0; a < NC * NC - 1; ++a) out[a] *= 2.0;
+}
+
+// Gauss law G(x) = sum_mu E_mu(x) - U_mu(x-mu)^+ E_mu(x-mu) U_mu(x-mu), components G^a
+template <int NC>
+__device__ void gauss_site(const double *__restrict__ links, const double *__restrict__ e, int s, int n0, int n1,
+                           int n2, double *comp)
+{
+    constexpr int NG = NC * NC - 1;
+    Mat<NC> G;
+#pragma unroll
+    for (int k = 0; k < NC * NC; ++k) G.e[k] = {0, 0};
+    for (int mu = 0; mu < 3; ++mu) {
+        const int sm = nbr(s, mu, -1, n0, n1, n2);
+        Mat<NC> e0 = alg_to_mat<NC>(e + NG * (mu + 3 * s));
+        Mat<NC> e1 = alg_to_mat<NC>(e + NG * (mu + 3 * sm));
+        Mat<NC> u = get_link<NC>(links, sm, mu);
+        Mat<NC> t = mmul<NC>(mmul<NC>(mdag<NC>(u), e1), u);
+#pragma unroll
+        for (int k = 0; k < NC * NC; ++k) G.e[k] = {G.e[k].r + e0.e[k].r - t.e[k].r, G.e[k].i + e0.e[k].i - t.e[k].i};
+    }
+    alg_components<NC>(G, comp);
+}
+
+// max |G^a| and sum G^2 (GaussLawViolation.cpp:13-61); with g != null also stores G^a(x)
 template <int NC>
 __global__ __launch_bounds__(TPB) void k_gauss(const double *__restrict__ links, const double *__restrict__ e,
-                                               double *partial, int n0, int n1, int n2)
+                                               double *partial, double *__restrict__ g, int n0, int n1, int n2)
 {
     const int V = n0 * n1 * n2;
     constexpr int NG = NC * NC - 1;
     double v[2] = {0, 0};
     for (int s = blockIdx.x * TPB + threadIdx.x; s < V; s += RED_BLOCKS * TPB) {
-        Mat<NC> G;
-#pragma unroll
-        for (int k = 0; k < NC * NC; ++k) G.e[k] = {0, 0};
-        for (int mu = 0; mu < 3; ++mu) {
-            const int sm = nbr(s, mu, -1, n0, n1, n2);
-            Mat<NC> e0 = alg_to_mat<NC>(e + NG * (mu + 3 * s));
-            Mat<NC> e1 = alg_to_mat<NC>(e + NG * (mu + 3 * sm));
-            Mat<NC> u = get_link<NC>(links, sm, mu);
-            Mat<NC> t = mmul<NC>(mmul<NC>(mdag<NC>(u), e1), u);
-#pragma unroll
-            for (int k = 0; k < NC * NC; ++k) G.e[k] = {G.e[k].r + e0.e[k].r - t.e[k].r, G.e[k].i + e0.e[k].i - t.e[k].i};
-        }
-        // components G^a = 2 tr(t^a G) = 2 ReTr(i t^a (-i G)) ; use ReTr(i t^a X) with X = -iG
-        Mat<NC> mg;
-#pragma unroll
-        for (int k = 0; k < NC * NC; ++k) mg.e[k] = {G.e[k].i, -G.e[k].r};
         double comp[NG];
-        retr_it<NC>(mg, comp);
+        gauss_site<NC>(links, e, s, n0, n1, n2, comp);
 #pragma unroll
         for (int a = 0; a < NG; ++a) {
-            const double g = fabs(2.0 * comp[a]);
-            v[0] = fmax(v[0], g);
-            v[1] += g * g;
+            if (g) g[(size_t)NG * s + a] = comp[a];
+            const double q = fabs(comp[a]);
+            v[0] = fmax(v[0], q);
+            v[1] += q * q;
         }
     }
     const int ism[2] = {1, 0};
     block_reduce<2>(v, ism);
     if (threadIdx.x == 0) { partial[2 * blockIdx.x] = v[0]; partial[2 * blockIdx.x + 1] = v[1]; }
+}
+
+// one relaxation step of GaussLawRestoration::UpdateElectricFields (GaussRestore.cpp:74-112):
+//   E_mu(x) -= gamma (G(x) - U_mu(x) G(x+mu) U_mu(x)^+)
+// i.e. gradient descent on (1/2) sum_x G(x)^2 in E
+template <int NC>
+__global__ __launch_bounds__(TPB) void k_gauss_restore(const double *__restrict__ links, const double *__restrict__ g,
+                                                       double *__restrict__ e, double gamma, int n0, int n1, int n2)
+{
+    const int V = n0 * n1 * n2;
+    constexpr int NG = NC * NC - 1;
+    const int i = blockIdx.x * TPB + threadIdx.x;
+    if (i >= 3 * V) return;
+    const int s = i / 3, mu = i - 3 * s;
+    const int sp = nbr(s, mu, 1, n0, n1, n2);
+    const Mat<NC> u = get_link<NC>(links, s, mu);
+    const Mat<NC> t = mmul<NC>(mmul<NC>(u, alg_to_mat<NC>(g + (size_t)NG * sp)), mdag<NC>(u));
+    double up[NG];
+    alg_components<NC>(t, up);
+#pragma unroll
+    for (int a = 0; a < NG; ++a) e[(size_t)NG * (mu + 3 * s) + a] -= gamma * (g[(size_t)NG * s + a] - up[a]);
 }
 
 template <int NC>
@@ -1316,11 +1354,20 @@ hipError_t launch_energy(const Geom &g, const double *links, const double *efiel
 }
 
 hipError_t launch_gauss(const Geom &g, const double *links, const double *efield, double *partial, double *out,
-                        hipStream_t st)
+                        double *gfield, hipStream_t st)
 {
-    if (g.nc == 2) k_gauss<2><<<RED_BLOCKS, TPB, 0, st>>>(links, efield, partial, g.n0, g.n1, g.n2);
-    else k_gauss<3><<<RED_BLOCKS, TPB, 0, st>>>(links, efield, partial, g.n0, g.n1, g.n2);
+    if (g.nc == 2) k_gauss<2><<<RED_BLOCKS, TPB, 0, st>>>(links, efield, partial, gfield, g.n0, g.n1, g.n2);
+    else k_gauss<3><<<RED_BLOCKS, TPB, 0, st>>>(links, efield, partial, gfield, g.n0, g.n1, g.n2);
     k_finish<2><<<1, TPB, 0, st>>>(partial, RED_BLOCKS, out, 1, 0, 0, 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_gauss_restore(const Geom &g, const double *links, const double *gfield, double *efield, double gamma,
+                                hipStream_t st)
+{
+    const unsigned nb = nblocks(3L * g.V);
+    if (g.nc == 2) k_gauss_restore<2><<<nb, TPB, 0, st>>>(links, gfield, efield, gamma, g.n0, g.n1, g.n2);
+    else k_gauss_restore<3><<<nb, TPB, 0, st>>>(links, gfield, efield, gamma, g.n0, g.n1, g.n2);
     return hipGetLastError();
 }
 

@@ -31,7 +31,8 @@ SYMBOLS = [
     "ovl_mode_norms",
     "ovl_create_ensemble", "ovl_set_system_links", "ovl_get_system_links", "ovl_set_system_efield",
     "ovl_get_system_efield", "ovl_set_system_u1_flux", "ovl_set_system_modes", "ovl_get_system_modes",
-    "ovl_system_observables", "ovl_ensemble_balance",
+    "ovl_system_observables", "ovl_ensemble_balance", "ovl_gauss_restore",
+    "ovl_save_configuration", "ovl_load_configuration",
     "ovl_get_timing", "ovl_set_profiling", "ovl_autotune",
 ]
 OBS_NAMES = ("e_el", "e_mag", "gauss_max", "gauss_rms", "unit_max", "unit_rms", "ncs_dot")
@@ -322,6 +323,20 @@ class Context:
         self._chk(lib().ovl_get_system_modes(self.h, s, field, out.ctypes.data_as(C.POINTER(C.c_double)), first,
                                              count))
         return out
+
+    def gauss_restore(self, tol=1e-12, max_iter=10000, s=0):
+        """Returns (iterations, max |G^a| of the final field)."""
+        it, mx = C.c_int(), C.c_double()
+        self._chk(lib().ovl_gauss_restore(self.h, s, C.c_double(tol), max_iter, C.byref(it), C.byref(mx)))
+        return it.value, mx.value
+
+    def save_configuration(self, ufile, efile=None, s=0):
+        self._chk(lib().ovl_save_configuration(self.h, s, os.fsencode(ufile),
+                                               None if efile is None else os.fsencode(efile)))
+
+    def load_configuration(self, ufile, efile=None, s=0):
+        self._chk(lib().ovl_load_configuration(self.h, s, os.fsencode(ufile),
+                                               None if efile is None else os.fsencode(efile)))
 
     def system_observables(self, s) -> dict:
         out = np.zeros(len(OBS_NAMES))

@@ -48,6 +48,7 @@ def lib():
         _lib.orc_zolotarev.restype = C.c_double
         _lib.orc_ritz.restype = C.c_double
         _lib.orc_edotb.restype = C.c_double
+        _lib.orc_gauss_restore_step.restype = C.c_double
         _lib.orc_zolotrr.restype = C.c_int
         _lib.orc_overlap_evo_zolo.restype = C.c_int
     return _lib
@@ -185,6 +186,23 @@ def gauss_violation(L: Lat, links, e):
     lib().orc_gauss_violation(C.byref(L), dp(np.ascontiguousarray(links, np.float64)),
                               dp(np.ascontiguousarray(e, np.float64)), C.byref(a), C.byref(b))
     return a.value, b.value
+
+
+def gauss_restore(L: Lat, links, e, tol, max_iter):
+    """Returns (E after the restoration, iterations, final max |G^a|)."""
+    e = np.array(e, np.float64)
+    vmax = C.c_double()
+    it = lib().orc_gauss_restore(C.byref(L), dp(np.ascontiguousarray(links, np.float64)), dp(e), C.c_double(tol),
+                                 C.c_int(max_iter), C.byref(vmax))
+    return e, it, vmax.value
+
+
+def gauss_restore_step(L: Lat, links, e, gamma):
+    """Returns (E after one relaxation step, max |G^a| before it)."""
+    e = np.array(e, np.float64)
+    m = lib().orc_gauss_restore_step(C.byref(L), dp(np.ascontiguousarray(links, np.float64)), dp(e),
+                                     C.c_double(gamma))
+    return e, m
 
 
 def unitarity_violation(L: Lat, links):

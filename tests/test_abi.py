@@ -60,6 +60,21 @@ def test_create_rejects_bad_params_without_touching_device():
     assert ovl.lib().ovl_create(C.byref(p), C.byref(h)) == ovl.E_ARG
 
 
+def test_create_ensemble_rejects_bad_params_without_touching_device():
+    def create(nc, dims, nmodes=8, mb=0):
+        flat = (C.c_int * max(1, 3 * len(dims)))(*[v for d in dims for v in d])
+        p = ovl.EnsembleParams(nc, len(dims), C.cast(flat, C.POINTER(C.c_int)), nmodes, 1.0, 0.0, 1.0, mb, 0)
+        h = C.c_void_p()
+        return ovl.lib().ovl_create_ensemble(C.byref(p), C.byref(h))
+    assert create(4, [(4, 4, 4)]) == ovl.E_ARG
+    assert create(2, []) == ovl.E_ARG
+    assert create(2, [(4, 4, 4), (4, 2, 4)]) == ovl.E_ARG
+    assert create(2, [(4, 4, 4)], nmodes=0) == ovl.E_ARG
+    assert create(2, [(4, 4, 4)], mb=24) == ovl.E_ARG
+    p = ovl.Params(2, (C.c_int * 3)(8, 8, 8), 16, 1.0, 0.0, 0.0, 0, 0, 4)  # traversal 4 does not exist
+    assert ovl.lib().ovl_create(C.byref(p), C.byref(C.c_void_p())) == ovl.E_ARG
+
+
 @pytest.mark.skipif(bool(os.environ.get("HIP_VISIBLE_DEVICES")) or os.path.exists("/dev/kfd"),
                     reason="a GPU may be present")
 def test_create_reports_missing_device():

@@ -305,3 +305,21 @@ def test_traversals_are_permutations(gpu, nc, n, nm, mb):
     coef = orc.cheb_coeffs(32, 0.1)
     err = rel(outs[0][nm - 1], orc.sign_cheb(L, ucc, orc.norm_bound(L), 0.1, 32, coef, psi[nm - 1]))
     assert err < 1e-5
+
+
+@pytest.mark.parametrize("nc", [2, 3])
+def test_gauss_restore_vs_oracle(gpu, nc):
+    print(f"[case] gauss restore nc={nc}", flush=True)
+    n = (5, 4, 3)
+    ctx, L, ucc, links, e, psi = make(nc, n, 4, 450 + nc, sigma=0.5, esig=0.3)
+    it, vmax = ctx.gauss_restore(tol=0.0, max_iter=7)
+    e_ref, it_ref, vmax_ref = orc.gauss_restore(L, links, e, 0.0, 7)
+    assert it == it_ref == 7
+    err = rel(ctx.get_efield(), e_ref)
+    assert err < 1e-12
+    assert abs(vmax - vmax_ref) < 1e-12 * max(vmax_ref, 1e-3)
+    # to convergence; links untouched
+    it, vmax = ctx.gauss_restore(tol=1e-11, max_iter=5000)
+    assert vmax <= 1e-11 and it < 5000
+    assert np.array_equal(ctx.get_links(), links)
+    assert ctx.gauss_violation()[0] == vmax

@@ -345,6 +345,45 @@ def test_gauss_and_edotb_are_gauge_invariant():
     assert abs(orc.gauss_violation(L, links, e)[1] - orc.gauss_violation(L, lg, eg)[1]) < 1e-12
 
 
+@pytest.mark.parametrize("nc", [2, 3])
+def test_gauss_restore_step_is_gradient_descent(nc):
+    # the step E -= gamma (G(x) - U G(x+mu) U^+) is gamma times the gradient of F = (1/2) sum G^2;
+    # a transposed transport (U^+ G U) or a wrong sign fails the finite differences
+    g = cf.rng(51 + nc)
+    n = (3, 4, 3)
+    V = int(np.prod(n))
+    ng = nc * nc - 1
+    L = orc.lat(nc, n)
+    links = cf.su2_links(n, 0.7, g) if nc == 2 else cf.su3_links(n, 0.7, g)
+    e = cf.efield(n, nc, 0.4, g)
+    F = lambda ee: 0.5 * orc.gauss_violation(L, links, ee)[1] ** 2 * ng * V  # noqa: E731
+    e1, gmax = orc.gauss_restore_step(L, links, e, 1.0)
+    assert gmax == orc.gauss_violation(L, links, e)[0]
+    grad = e - e1
+    h = 1e-6
+    for i in (0, 7, ng * 3 * 5 + 2, ng * 3 * V - 1):
+        ep, em = e.copy(), e.copy()
+        ep[i] += h
+        em[i] -= h
+        assert abs((F(ep) - F(em)) / (2 * h) - grad[i]) < 1e-7 * max(1.0, abs(grad[i]))
+
+
+@pytest.mark.parametrize("nc", [2, 3])
+def test_gauss_restore_converges_and_keeps_links(nc):
+    g = cf.rng(61 + nc)
+    n = (4, 3, 3)
+    L = orc.lat(nc, n)
+    links = cf.su2_links(n, 0.5, g) if nc == 2 else cf.su3_links(n, 0.5, g)
+    e = cf.efield(n, nc, 0.3, g)
+    g0 = orc.gauss_violation(L, links, e)[0]
+    e1, it, vmax = orc.gauss_restore(L, links, e, 1e-11, 5000)
+    assert g0 > 0.1 and vmax <= 1e-11 and 0 < it < 5000
+    assert vmax == orc.gauss_violation(L, links, e1)[0]
+    # a field that already satisfies the law is left alone
+    e2, it2, _ = orc.gauss_restore(L, links, e1, 1e-10, 100)
+    assert it2 == 0 and np.array_equal(e2, e1)
+
+
 def test_currents_against_direct_bilinears():
     g = cf.rng(41)
     n = (3, 3, 4)
