@@ -59,6 +59,7 @@ struct ovl_ctx {
     int *slab_order[2] = {nullptr, nullptr};  // per precision: site order of TRAV_MODES (y slabs) or null
     int trav[2] = {ovl::TRAV_LEX, ovl::TRAV_LEX};  // stencil traversal per input precision
     int slabs[2] = {1, 1};                      // y slabs of TRAV_MODES per input precision
+    int nt_out[2] = {0, 0};                     // streaming output stores per input precision
     double *dout = nullptr;  // 16 doubles
     double *gam = nullptr;   // current coupling matrices
     double *jv = nullptr, *ja = nullptr;
@@ -274,6 +275,7 @@ HopArgs hop_args(const ovl_ctx *c, int prec)
     a.amass = amass(c);
     a.nblk = c->g.nblk;
     a.trav = c->trav[prec];
+    a.nt_out = c->nt_out[prec];
     a.order = a.trav == ovl::TRAV_BOXES ? c->xcd_order : (a.trav == ovl::TRAV_MODES ? c->slab_order[prec] : nullptr);
     if (c->ensemble) {
         a.trav = ovl::TRAV_LEX;
@@ -356,7 +358,6 @@ int set_traversal(ovl_ctx *c, int prec, int trav, int slabs)
 {
     if (c->ensemble) return OVL_OK;
     if (trav == ovl::TRAV_BOXES && !c->xcd_order) trav = ovl::TRAV_LEX;
-    if (trav == ovl::TRAV_MODES && c->g.nblk % ovl::NUM_XCD != 0) trav = ovl::TRAV_LEX;
     c->trav[prec] = trav;
     if (trav != ovl::TRAV_MODES) return OVL_OK;
     slabs = std::max(1, std::min(slabs, c->g.n1));
@@ -2095,67 +2096,67 @@ int ovl_autotune(ovl_ctx *c, char *report, int len)
             cand.push_back({c->trav[prec], c->slabs[prec]});
         } else {
             if (c->xcd_order) cand.push_back({ovl::TRAV_BOXES, 1});
-            if (c->g.nblk % ovl::NUM_XCD == 0) {
-                const int t0 = default_slabs(c, prec);
-                for (int t : {t0 / 2, t0, 2 * t0})
-                    if (t >= 1 && t <= c->g.n1 && (cand.empty() || cand.back() != std::make_pair((int)ovl::TRAV_MODES, t)))
-                        cand.push_back({ovl::TRAV_MODES, t});
-            }
-            if (cand.empty()) cand.push_back({ovl::TRAV_LEX, 1});
+            for (int t : {1, 2, 3, 4, 6})
+                if (t <= c->g.n1) cand.push_back({ovl::TRAV_MODES, t});
         }
         void *x = prec == P64 ? c->field[OVL_PSI] : c->x32;
         void *b1 = prec == P64 ? c->a64 : c->a32, *b2 = prec == P64 ? c->b64 : c->b32;
         void *w = prec == P64 ? c->w64 : c->w32;
         double best = 1e30;
-        int best_v = 1;
+        int best_v = 1, best_nt = 0;
         std::pair<int, int> best_t = cand[0];
         for (const auto &tr : cand) {
             RETCHK(set_traversal(c, prec, tr.first, tr.second));
-            for (int v = 1; v <= 4; ++v) {
-                c->wpe[prec] = v;
-                float ms = 0;
-                for (int rep_i = 0; rep_i < 2; ++rep_i) {  // first pass warms up
-                    HIPCHK(c, hipEventRecord(c->t0, c->st));
-                    for (int k = 0; k < 3; ++k) {
-                        HopArgs a = hop_args(c, prec);
-                        a.in = b1;
-                        a.out = w;
-                        a.sh_r = 1.0;
-                        RETCHK(hop(c, c->g, prec, prec, a));
-                        HopArgs q = hop_args(c, prec);
-                        q.in = w;
-                        q.out = b2;
-                        q.sh_r = 0.1;
-                        q.x = x;
-                        q.sx_r = 0.01;
-                        q.b1 = b1;
-                        q.s1_r = 0.5;
-                        q.b2 = b2;
-                        q.s2_r = -0.5;
-                        RETCHK(hop(c, c->g, prec, prec, q));
+            for (int nt = 0; nt <= 1; ++nt)
+                for (int v = 1; v <= 4; ++v) {
+                    c->wpe[prec] = v;
+                    c->nt_out[prec] = nt;
+                    float ms = 0;
+                    for (int rep_i = 0; rep_i < 2; ++rep_i) {  // first pass warms up
+                        HIPCHK(c, hipEventRecord(c->t0, c->st));
+                        for (int k = 0; k < 3; ++k) {
+                            HopArgs a = hop_args(c, prec);
+                            a.in = b1;
+                            a.out = w;
+                            a.sh_r = 1.0;
+                            RETCHK(hop(c, c->g, prec, prec, a));
+                            HopArgs q = hop_args(c, prec);
+                            q.in = w;
+                            q.out = b2;
+                            q.sh_r = 0.1;
+                            q.x = x;
+                            q.sx_r = 0.01;
+                            q.b1 = b1;
+                            q.s1_r = 0.5;
+                            q.b2 = b2;
+                            q.s2_r = -0.5;
+                            RETCHK(hop(c, c->g, prec, prec, q));
+                        }
+                        HIPCHK(c, hipEventRecord(c->t1, c->st));
+                        HIPCHK(c, hipEventSynchronize(c->t1));
+                        HIPCHK(c, hipEventElapsedTime(&ms, c->t0, c->t1));
                     }
-                    HIPCHK(c, hipEventRecord(c->t1, c->st));
-                    HIPCHK(c, hipEventSynchronize(c->t1));
-                    HIPCHK(c, hipEventElapsedTime(&ms, c->t0, c->t1));
+                    char buf[96];
+                    std::snprintf(buf, sizeof buf, "%s %s/%d%s wpe=%d %.1f us; ", prec == P64 ? "fp64" : "fp32",
+                                  tname[c->trav[prec]], c->slabs[prec], nt ? " nt" : "", v, 1e3 * ms / 3);
+                    rep += buf;
+                    if (ms < best) {
+                        best = ms;
+                        best_v = v;
+                        best_nt = nt;
+                        best_t = {c->trav[prec], c->slabs[prec]};
+                    }
                 }
-                char buf[96];
-                std::snprintf(buf, sizeof buf, "%s %s/%d wpe=%d %.1f us; ", prec == P64 ? "fp64" : "fp32",
-                              tname[c->trav[prec]], c->slabs[prec], v, 1e3 * ms / 3);
-                rep += buf;
-                if (ms < best) {
-                    best = ms;
-                    best_v = v;
-                    best_t = {c->trav[prec], c->slabs[prec]};
-                }
-            }
         }
         RETCHK(set_traversal(c, prec, best_t.first, best_t.second));
         c->wpe[prec] = best_v;
+        c->nt_out[prec] = best_nt;
     }
     c->profiling = prof;
     char buf[160];
-    std::snprintf(buf, sizeof buf, "chosen fp64 %s/%d wpe=%d, fp32 %s/%d wpe=%d", tname[c->trav[P64]], c->slabs[P64],
-                  c->wpe[P64], tname[c->trav[P32]], c->slabs[P32], c->wpe[P32]);
+    std::snprintf(buf, sizeof buf, "chosen fp64 %s/%d%s wpe=%d, fp32 %s/%d%s wpe=%d", tname[c->trav[P64]],
+                  c->slabs[P64], c->nt_out[P64] ? " nt" : "", c->wpe[P64], tname[c->trav[P32]], c->slabs[P32],
+                  c->nt_out[P32] ? " nt" : "", c->wpe[P32]);
     rep += buf;
     if (report && len > 0) std::snprintf(report, len, "%s", rep.c_str());
     return OVL_OK;

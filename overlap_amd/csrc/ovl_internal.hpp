@@ -48,6 +48,7 @@ struct HopArgs {
     double sh_r, sh_i, sx_r, sx_i, s1_r, s1_i, s2_r, s2_i, st_r, st_i;
     double amass;     // A = 6r + 2m - 2M of h_W = (1/2) gamma0 [A - hopping]
     int x_gamma0;     // 1: G = gamma0
+    int nt_out;       // 1: non-temporal (streaming) stores of `out`
     int nblk;         // blocks covered by this launch
     int trav;         // lane traversal, TRAV_* (falls back to TRAV_LEX where the launch does not split evenly)
     const int *order; // TRAV_BOXES: XCD k's box is order[k V/8 .. (k+1) V/8); TRAV_MODES: site order

@@ -94,14 +94,39 @@ __device__ __forceinline__ void store_sp(void *__restrict__ base, int V, int blk
     }
 }
 
+// streaming store of an output that the launch never reads back (keeps L2 for the stencil input)
+template <int NC, int MB, typename T>
+__device__ __forceinline__ void store_sp_nt(void *__restrict__ base, int V, int blk, int s, int m, const T *re,
+                                            const T *im)
+{
+    constexpr int NCOMP = 4 * NC;
+    if constexpr (sizeof(T) == 8) {
+        d2v *p = reinterpret_cast<d2v *>(base) + ((size_t)(blk * V + s) * NCOMP) * MB + m;
+#pragma unroll
+        for (int c = 0; c < NCOMP; ++c) {
+            d2v v = {re[c], im[c]};
+            __builtin_nontemporal_store(v, p + (size_t)c * MB);
+        }
+    } else {
+        f4v *p = reinterpret_cast<f4v *>(base) + ((size_t)(blk * V + s) * (NCOMP / 2)) * MB + m;
+#pragma unroll
+        for (int c = 0; c < NCOMP / 2; ++c) {
+            f4v v = {re[2 * c], im[2 * c], re[2 * c + 1], im[2 * c + 1]};
+            __builtin_nontemporal_store(v, p + (size_t)c * MB);
+        }
+    }
+}
+
 // Lane -> (mode m, site s, block blk) for the three traversals (TRAV_*).  The hardware hands
 // workgroup b to XCD b % 8; the XCD traversals undo that round-robin so that XCD k runs the
 // contiguous k-th eighth of the logical lanes:
 //   TRAV_BOXES: the eighth is XCD k's spatial box of sites (order[k V/8 .. (k+1) V/8)), every
 //     mode block in turn, so x/y/z neighbours are re-read from that XCD's L2 except at the box
 //     faces;
-//   TRAV_MODES: the eighth is nblk/8 whole mode blocks over the whole lattice, sites in the order
-//     of `order` (y slabs, z-major) or lexicographic, so no site is fetched by two XCDs and the
+//   TRAV_MODES: the eighth is a contiguous range of (mode block, site-order position) in
+//     block-major order -- nblk/8 whole blocks when there are at least 8, else 1/(8/nblk) of one
+//     block's site order (a z range, or a range of y slabs) -- with sites in the order of `order`
+//     (y slabs, z-major) or lexicographic, so few sites are fetched by two XCDs and the
 //     z-neighbour planes of a slab stay in L2.
 template <int MB>
 __device__ __forceinline__ bool lane_coords(const HopArgs &a, int V, int &m, int &s, int &blk)
@@ -121,7 +146,7 @@ __device__ __forceinline__ bool lane_coords(const HopArgs &a, int V, int &m, int
         blk = (int)(rr / v8);
         s = a.order[xc * v8 + (int)(rr - (long)blk * v8)];
     } else if (a.trav == TRAV_MODES) {
-        blk = (int)(r / V);  // blocks are contiguous per XCD: XCD k owns [k nblk/8, (k+1) nblk/8)
+        blk = (int)(r / V);
         const int i = (int)(r - (long)blk * V);
         s = a.order ? a.order[i] : i;
     } else {
@@ -292,7 +317,8 @@ __device__ __forceinline__ void hop_site(const HopArgs &a, int n0, int n1, int n
 #pragma unroll
         for (int c = 0; c < NCOMP; ++c) { orr[c] += cr * tr[c] - ci * ti[c]; oi[c] += cr * ti[c] + ci * tr[c]; }
     }
-    store_sp<NC, MB, TO>(a.out, 0, 0, q, m, orr, oi);
+    if (a.nt_out) store_sp_nt<NC, MB, TO>(a.out, 0, 0, q, m, orr, oi);
+    else store_sp<NC, MB, TO>(a.out, 0, 0, q, m, orr, oi);
 }
 
 // WPE: minimum waves per SIMD the register allocation must allow (1 = compiler's choice); the
@@ -1219,7 +1245,7 @@ hipError_t hop_mb(const Geom &g, const HopArgs &a, int wpe, hipStream_t st)
     // the XCD traversals need whole workgroups spread evenly over the XCDs
     const bool even = total % ((long)TPB * NUM_XCD) == 0;
     if (b.trav == TRAV_BOXES && (!even || g.V % NUM_XCD != 0 || !b.order)) b.trav = TRAV_LEX;
-    if (b.trav == TRAV_MODES && (!even || a.nblk % NUM_XCD != 0)) b.trav = TRAV_LEX;
+    if (b.trav == TRAV_MODES && !even) b.trav = TRAV_LEX;
     switch (g.mb) {
     case 16: hop_wpe<NC, TI, TO, 16>(b, nb, wpe, g, st); break;
     case 32: hop_wpe<NC, TI, TO, 32>(b, nb, wpe, g, st); break;

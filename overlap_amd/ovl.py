@@ -283,8 +283,8 @@ class Context:
         self._chk(lib().ovl_set_profiling(self.h, int(on)))
 
     def autotune(self) -> str:
-        buf = C.create_string_buffer(4096)
-        self._chk(lib().ovl_autotune(self.h, buf, 4096))
+        buf = C.create_string_buffer(16384)
+        self._chk(lib().ovl_autotune(self.h, buf, 16384))
         return buf.value.decode()
 
     # per-system calls (system 0 of an ordinary context)
