@@ -152,12 +152,12 @@ def bench_ensemble(ec: cf.EnsembleConfig, steps: int, warmup: int, tol: float):
         wall = time.perf_counter() - t0
         ens.set_profiling(False)
         t = ens.timing()
-        var = int(np.argmax([t.var_ms[i] for i in range(3)]))
+        var = int(np.argmax([t.var_ms[i] for i in range(4)]))
         out[name] = {"ms_per_step": 1e3 * wall / steps, "value": sites * ec.nmodes * steps / wall,
-                     "kernel": ["k_hop_ens FP64", "k_hop_ens FP32", "k_hop_ens FP32->FP64"][var],
+                     "kernel": ["k_hop_ens FP64", "k_hop_ens FP32", "k_hop_ens FP32->FP64", "k_hop_ens FP16"][var],
                      "kernel_GBps": t.var_bytes[var] / max(t.var_ms[var], 1e-9) / 1e6,
                      "kernel_share_of_step": t.var_ms[var] / max(t.total_ms, 1e-9),
-                     "fp64_steps": t.fp64_steps, "fp32_steps": t.fp32_steps}
+                     "fp64_steps": t.fp64_steps, "fp32_steps": t.fp32_steps, "fp16_steps": t.fp16_steps}
         out[name]["psi"] = [ens.get_system_modes(k, ovl.PSI) for k in range(ec.n_systems)]
         log(f"ensemble {name}: {out[name]['ms_per_step']:.1f} ms/step {out[name]['value']:.4e} site*mode/s, "
             f"{out[name]['kernel']} {out[name]['kernel_GBps']:.0f} GB/s")
@@ -239,10 +239,12 @@ def main():
         mix = ctx.get_modes(ovl.SCRATCH, 0, min(16, c.nmodes))
         tm = ctx.timing()
         res["rel_err_mixed_vs_fp64"] = float(max(np.linalg.norm(a - b) / np.linalg.norm(b) for a, b in zip(mix, ref)))
-        res["fp64_head_steps"] = int(tm.fp64_steps // max(1, 1))
-        res["fp32_tail_steps"] = int(tm.fp32_steps)
+        res["fp64_steps"] = int(tm.fp64_steps)
+        res["fp32_steps"] = int(tm.fp32_steps)
+        res["fp16_steps"] = int(tm.fp16_steps)
         del ref, mix
-        log(f"mixed vs fp64 rel err {res['rel_err_mixed_vs_fp64']:.3e}  head {tm.fp64_steps} tail {tm.fp32_steps}")
+        log(f"mixed vs fp64 rel err {res['rel_err_mixed_vs_fp64']:.3e}  steps fp64 {tm.fp64_steps} "
+            f"fp32 {tm.fp32_steps} fp16 {tm.fp16_steps}")
 
     # --- occupancy autotune of the stencil kernel for this shape
     tune = ctx.autotune()
@@ -264,8 +266,8 @@ def main():
     log(f"mixed: {ms_per_step:.1f} ms/step  {value:.4e} site*mode/s  (events {t.total_ms / args.steps:.1f} ms/step)")
 
     # dominant stencil variant of the timed region
-    var = int(np.argmax([t.var_ms[i] for i in range(3)]))
-    names = ["k_hop FP64", "k_hop FP32", "k_hop FP32->FP64"]
+    var = int(np.argmax([t.var_ms[i] for i in range(4)]))
+    names = ["k_hop FP64", "k_hop FP32", "k_hop FP32->FP64", "k_hop FP16"]
     avg_ms = t.var_ms[var] / max(1, t.var_launches[var])
     bytes_per_launch = t.var_bytes[var] / max(1, t.var_launches[var])
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
@@ -288,7 +290,7 @@ def main():
     roofline["variants"] = {
         names[i]: {"launches": int(t.var_launches[i]), "ms": round(t.var_ms[i], 3),
                    "GBps": round(t.var_bytes[i] / max(t.var_ms[i], 1e-9) / 1e6, 1)}
-        for i in range(3) if t.var_launches[i]}
+        for i in range(4) if t.var_launches[i]}
     log("variants", json.dumps(roofline["variants"]))
 
     # --- graph-captured replay of the same step
@@ -321,7 +323,7 @@ def main():
     # HBM traffic of the dominant stencil variant, from two counter passes (child processes)
     if not args.no_traffic:
         prefix = ["k_hop<%d, double, double" % c.nc, "k_hop<%d, float, float" % c.nc,
-                  "k_hop<%d, float, double" % c.nc][var]
+                  "k_hop<%d, float, double" % c.nc, "k_hop<%d, _Float16, _Float16" % c.nc][var]
         tr = measure_traffic(args, c, prefix, int(round(t.var_launches[var] / args.steps)))
         if tr:
             traffic = tr["FETCH_SIZE"]["bytes_per_launch"] + tr["WRITE_SIZE"]["bytes_per_launch"]

@@ -74,6 +74,9 @@ typedef struct {
     int head;            /* MIXED: FP64 head length (>= 0 overrides tol) */
     double lmin, lmax;   /* ZOLOTAREV: interval of h_W^2; lmin = lmax = 0 -> Ritz estimate */
     int mode_chunk;      /* mode blocks per recurrence pass (0 = all) */
+    int fp16_tail;       /* MIXED: 1 = store the far tail of the recurrence in FP16 (scaled by powers
+                            of two, FP32 arithmetic), 0 = FP32 is the lowest tier */
+    int head16;          /* MIXED with fp16_tail: first FP16 step (> 0 overrides tol; 0 = from tol) */
 } ovl_sign_opts;
 
 typedef struct {
@@ -213,11 +216,12 @@ typedef struct {
     double hop_ms;            /* summed event time of the stencil kernels */
     long long hop_launches;   /* number of stencil kernel launches */
     double hop_bytes;         /* algorithmic bytes of those launches */
-    int fp64_steps, fp32_steps;
-    /* per stencil-kernel variant: [0] FP64 in/out, [1] FP32 in/out, [2] FP32 in / FP64 out */
-    double var_ms[3];
-    long long var_launches[3];
-    double var_bytes[3];
+    int fp64_steps, fp32_steps, fp16_steps;
+    /* per stencil-kernel variant: [0] FP64 in/out, [1] FP32 in/out, [2] FP32 in / FP64 out,
+     * [3] FP16 storage in/out */
+    double var_ms[4];
+    long long var_launches[4];
+    double var_bytes[4];
 } ovl_timing;
 int ovl_get_timing(ovl_ctx *ctx, ovl_timing *t);
 int ovl_set_profiling(ovl_ctx *ctx, int on);

@@ -20,6 +20,7 @@ using ovl::Geom;
 using ovl::HopArgs;
 using ovl::P32;
 using ovl::P64;
+using ovl::P16;
 using ovl::SysDesc;
 typedef std::complex<double> cplx;
 
@@ -34,6 +35,7 @@ struct ovl_ctx {
     int ncomp = 0;
     size_t field64_bytes = 0, field32_bytes = 0;  // full field (all blocks)
     size_t blk64_elems = 0, blk32_elems = 0;      // per mode block (double2 / float4 units)
+    size_t blk16_elems = 0;                       // per mode block (16-byte FP16 units)
     hipStream_t st = nullptr;
     std::string err;
 
@@ -51,15 +53,19 @@ struct ovl_ctx {
     int work_blocks = 0;
     void *w64 = nullptr, *a64 = nullptr, *b64 = nullptr;
     void *x32 = nullptr, *w32 = nullptr, *a32 = nullptr, *b32 = nullptr;
+    void *x16 = nullptr, *w16 = nullptr, *a16 = nullptr, *b16 = nullptr;
 
     // reductions / small scratch
     double *partial = nullptr;
     size_t partial_len = 0;
     int *xcd_order = nullptr;  // device XCD-box site order (TRAV_BOXES) or null
-    int *slab_order[2] = {nullptr, nullptr};  // per precision: site order of TRAV_MODES (y slabs) or null
-    int trav[2] = {ovl::TRAV_LEX, ovl::TRAV_LEX};  // stencil traversal per input precision
-    int slabs[2] = {1, 1};                      // y slabs of TRAV_MODES per input precision
-    int nt_out[2] = {0, 0};                     // streaming output stores per input precision
+    // per input precision (P64, P32, P16) of the stencil, set by ovl_autotune:
+    int *slab_order[3] = {nullptr, nullptr, nullptr};  // site order of TRAV_MODES (y slabs) or null
+    int trav[3] = {ovl::TRAV_LEX, ovl::TRAV_LEX, ovl::TRAV_LEX};  // lane traversal
+    int slabs[3] = {1, 1, 1};                   // y slabs of TRAV_MODES
+    int nt_out[3] = {0, 0, 0};                  // streaming output stores
+    int wpe[3] = {1, 1, 1};                     // occupancy variant
+    double step_xmax = 0.0;  // bound of the mode norms during ovl_fermion_update (FP16 scales)
     double *dout = nullptr;  // 16 doubles
     double *gam = nullptr;   // current coupling matrices
     double *jv = nullptr, *ja = nullptr;
@@ -80,15 +86,14 @@ struct ovl_ctx {
     std::vector<hipEvent_t> evpool;
     std::vector<int> evvar;  // stencil variant of each recorded event pair
     size_t evused = 0;
-    long long var_launches[3] = {0, 0, 0};
-    double var_bytes[3] = {0, 0, 0};
+    long long var_launches[4] = {0, 0, 0, 0};
+    double var_bytes[4] = {0, 0, 0, 0};
     long long hop_launches = 0;
     double hop_bytes = 0;
-    int fp64_steps = 0, fp32_steps = 0;
+    int fp64_steps = 0, fp32_steps = 0, fp16_steps = 0;
     ovl_timing last{};
     hipEvent_t t0 = nullptr, t1 = nullptr;
     bool capturing = false;
-    int wpe[2] = {1, 1};  // stencil occupancy variant per input precision (P64, P32), set by ovl_autotune
 
     // lattices held: one for an ordinary context, or the systems of an ensemble, stored one after
     // another in every field (site-blocks q0..), link and E-field array (link index l0..)
@@ -218,6 +223,7 @@ void *blk_ptr(void *base, int prec, size_t blk, const ovl_ctx *c)
 {
     if (!base) return nullptr;
     if (prec == P64) return (char *)base + blk * c->blk64_elems * 16;
+    if (prec == P16) return (char *)base + blk * c->blk16_elems * 16;
     return (char *)base + blk * c->blk32_elems * 16;
 }
 
@@ -236,20 +242,22 @@ hipEvent_t next_event(ovl_ctx *c)
 double hop_alg_bytes(const ovl_ctx *c, const HopArgs &a, int pin, int pout)
 {
     const double sites_modes = (double)a.nblk * c->g.V * c->g.mb;
-    const double Sin = c->ncomp * 2.0 * (pin == P64 ? 8 : 4);
-    const double Sout = c->ncomp * 2.0 * (pout == P64 ? 8 : 4);
+    auto real_bytes = [](int p) { return p == P64 ? 8.0 : (p == P32 ? 4.0 : 2.0); };
+    const double Sin = c->ncomp * 2.0 * real_bytes(pin);
+    const double Sout = c->ncomp * 2.0 * real_bytes(pout);
     double b = sites_modes * (Sin + Sout);
     if (a.x) b += sites_modes * Sout;
     if (a.b1) b += sites_modes * Sout;
     if (a.b2) b += sites_modes * Sout;
     if (a.tgt) b += sites_modes * Sout;
-    b += 3.0 * c->link_sites * c->g.nc * c->g.nc * 2.0 * (pin == P64 ? 8 : 4);
+    b += 3.0 * c->link_sites * c->g.nc * c->g.nc * 2.0 * (pin == P64 ? 8 : 4);  // FP16 uses the FP32 links
     return b;
 }
 
 int hop(ovl_ctx *c, const Geom &g, int pin, int pout, const HopArgs &a)
 {
-    const int var = (pin == P64) ? 0 : (pout == P32 ? 1 : 2);
+    // variants: [0] FP64, [1] FP32, [2] FP32 in / FP64 out, [3] FP16
+    const int var = (pin == P64) ? 0 : (pin == P16 ? 3 : (pout == P32 ? 1 : 2));
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (c->profiling && !c->capturing) {
         e0 = next_event(c);
@@ -347,7 +355,7 @@ std::vector<int> slab_site_order(int n0, int n1, int n2, int t)
 // of the 4 MB L2 of an XCD
 int default_slabs(const ovl_ctx *c, int prec)
 {
-    const double site_bytes = (double)c->g.mb * c->ncomp * (prec == P64 ? 16 : 8);
+    const double site_bytes = (double)c->g.mb * c->ncomp * (prec == P64 ? 16 : (prec == P32 ? 8 : 4));
     int t = 1;
     while (t < c->g.n1 && 3.0 * c->g.n0 * ((c->g.n1 + t - 1) / t) * site_bytes > 3.0 * (1 << 20)) ++t;
     return t;
@@ -379,8 +387,8 @@ void begin_timing(ovl_ctx *c)
     c->evused = 0;
     c->hop_launches = 0;
     c->hop_bytes = 0;
-    for (int v = 0; v < 3; ++v) { c->var_launches[v] = 0; c->var_bytes[v] = 0; }
-    c->fp64_steps = c->fp32_steps = 0;
+    for (int v = 0; v < 4; ++v) { c->var_launches[v] = 0; c->var_bytes[v] = 0; }
+    c->fp64_steps = c->fp32_steps = c->fp16_steps = 0;
     if (!c->capturing) hipEventRecord(c->t0, c->st);
 }
 
@@ -391,7 +399,7 @@ int end_timing(ovl_ctx *c)
     float ms = 0;
     hipEventElapsedTime(&ms, c->t0, c->t1);
     c->last.total_ms = ms;
-    double hop_ms = 0, var_ms[3] = {0, 0, 0};
+    double hop_ms = 0, var_ms[4] = {0, 0, 0, 0};
     for (size_t i = 0; i + 1 < c->evused; i += 2) {
         float t = 0;
         hipEventElapsedTime(&t, c->evpool[i], c->evpool[i + 1]);
@@ -399,7 +407,7 @@ int end_timing(ovl_ctx *c)
         var_ms[c->evvar[i / 2]] += t;
     }
     c->last.hop_ms = hop_ms;
-    for (int v = 0; v < 3; ++v) {
+    for (int v = 0; v < 4; ++v) {
         c->last.var_ms[v] = var_ms[v];
         c->last.var_launches[v] = c->var_launches[v];
         c->last.var_bytes[v] = c->var_bytes[v];
@@ -408,6 +416,7 @@ int end_timing(ovl_ctx *c)
     c->last.hop_bytes = c->hop_bytes;
     c->last.fp64_steps = c->fp64_steps;
     c->last.fp32_steps = c->fp32_steps;
+    c->last.fp16_steps = c->fp16_steps;
     return OVL_OK;
 }
 
@@ -433,22 +442,16 @@ std::vector<double> cheb_coeffs(int n, double cut)
     return c;
 }
 
-// FP64 head length of the mixed recurrence: steps k < head run in FP64, the rest in FP32.
-// Error model: the FP32 storage of b_k (k >= head) perturbs the result by about
-// kappa * u32 * B_head, where B_k = max over t in [-1,1] of |b_k(t)| (the scalar Clenshaw
-// intermediates of the polynomial; they decay geometrically with k).  kappa = 2 is 10-20x above
-// the errors measured with FP32 storage on random SU(3) fields (0.1-0.2 u32 B_head; 0.7 u32 B_head
-// on a 6^3 lattice with 8 flux quanta); the coherent worst case (errors aligned with the extreme
-// eigenvector and summed linearly over steps) would ask for 1e4x more and is not what FP32
-// rounding produces.
-int choose_head(const std::vector<double> &c, double tol, double kappa = 2.0)
+// B_k = max over t in [t0, 1] of |b_k(t)|: the scalar Clenshaw intermediates of the polynomial
+// (they decay geometrically with k).  t0 = -1 gives the bound on the approximation interval;
+// t0 = T(0) < -1 covers the whole spectrum of T = alpha h^2 + beta (h^2 >= 0).
+std::vector<double> clenshaw_sup(const std::vector<double> &c, double t0)
 {
     const int n = (int)c.size() - 1;
-    const double u32 = 5.96e-8;
     const int NT = 2049;
     std::vector<double> t(NT), b1(NT, 0.0), b2(NT, 0.0), B(n + 2, 0.0);
-    for (int i = 0; i < NT; ++i) t[i] = std::cos(M_PI * i / (NT - 1));
-    for (int k = n; k >= 1; --k) {
+    for (int i = 0; i < NT; ++i) t[i] = 0.5 * (1.0 + t0) + 0.5 * (1.0 - t0) * std::cos(M_PI * i / (NT - 1));
+    for (int k = n; k >= 0; --k) {
         double mx = 0;
         for (int i = 0; i < NT; ++i) {
             const double bk = c[k] + 2.0 * t[i] * b1[i] - b2[i];
@@ -458,42 +461,74 @@ int choose_head(const std::vector<double> &c, double tol, double kappa = 2.0)
         }
         B[k] = mx;
     }
-    int head = n + 1;
-    for (int k = n; k >= 1; --k) {
-        if (kappa * u32 * B[k] > tol) break;
-        head = k;
-    }
-    return head;
+    return B;
 }
 
-int ensure_work(ovl_ctx *c, int blocks, bool need32)
+// Precision ladder of the mixed recurrence: the storage of b_k (k >= first) in a format with unit
+// roundoff u perturbs the result by about kappa * u * B_first.  kappa = 2 is 10-20x above the
+// errors measured with FP32 storage on random SU(3) fields (0.1-0.2 u32 B; 0.7 u32 B on a 6^3
+// lattice with 8 flux quanta) and with FP16 storage (0.06 u16 B on the same lattice); the coherent
+// worst case (errors aligned with the extreme eigenvector and summed linearly over steps) would
+// ask for 1e4x more and is not what rounding produces.  Returns the smallest `first` >= 1 such
+// that every step k >= first satisfies the bound (n + 1: none).
+int tier_start(const std::vector<double> &B, int n, double u, double tol, double kappa = 2.0)
+{
+    int first = n + 1;
+    for (int k = n; k >= 1; --k) {
+        if (kappa * u * B[k] > tol) break;
+        first = k;
+    }
+    return first;
+}
+
+int ensure_work(ovl_ctx *c, int blocks, bool need32, bool need16 = false)
 {
     if (blocks > c->work_blocks) {
         dfree(c->w64); dfree(c->a64); dfree(c->b64);
         dfree(c->x32); dfree(c->w32); dfree(c->a32); dfree(c->b32);
+        dfree(c->x16); dfree(c->w16); dfree(c->a16); dfree(c->b16);
         c->work_blocks = blocks;
     }
     const size_t b64 = (size_t)c->work_blocks * c->blk64_elems * 16;
     const size_t b32 = (size_t)c->work_blocks * c->blk32_elems * 16;
+    const size_t b16 = (size_t)c->work_blocks * c->blk16_elems * 16;
     RETCHK(dalloc(c, &c->w64, b64));
     RETCHK(dalloc(c, &c->a64, b64));
     RETCHK(dalloc(c, &c->b64, b64));
-    if (need32) {
+    if (need32 || need16) {
         RETCHK(dalloc(c, &c->x32, b32));
         RETCHK(dalloc(c, &c->w32, b32));
         RETCHK(dalloc(c, &c->a32, b32));
         RETCHK(dalloc(c, &c->b32, b32));
+    }
+    if (need16) {
+        RETCHK(dalloc(c, &c->x16, b16));
+        RETCHK(dalloc(c, &c->w16, b16));
+        RETCHK(dalloc(c, &c->a16, b16));
+        RETCHK(dalloc(c, &c->b16, b16));
     }
     return OVL_OK;
 }
 
 struct SignPlan {
     std::vector<double> coef;
-    int n = 0, head = 0;
+    int n = 0, head = 0, head16 = 0;  // k < head: FP64, head <= k < head16: FP32, k >= head16: FP16
     double lam = 0, alpha = 0, beta = 0;
+    // FP16 tier: b_k is stored as sk[k] * b_k, w = h b_{k+1} as sw[k] * w, x as sx * x (powers of two)
+    std::vector<double> sk, sw;
+    double sx = 1.0;
 };
 
-int make_plan(ovl_ctx *c, const ovl_sign_opts *o, SignPlan &p)
+// power of two that maps values bounded by `bound` to at most 2^8 (FP16 keeps 2^15 of headroom
+// above that and its subnormal floor 2^-24 far below the unit roundoff of the stored values)
+double fp16_scale(double bound)
+{
+    if (!(bound > 0.0) || !std::isfinite(bound)) return 1.0;
+    return std::ldexp(1.0, 8 - (int)std::ceil(std::log2(bound)));
+}
+
+// xmax: an upper bound of the 2-norm of every mode of the input (needed for the FP16 scales)
+int make_plan(ovl_ctx *c, const ovl_sign_opts *o, SignPlan &p, double xmax = 0.0)
 {
     if (o->degree < 1 || o->degree > 100000) return fail(c, OVL_E_ARG, "Chebyshev degree must be in [1, 100000]");
     if (!(o->cut > 0.0 && o->cut < 1.0)) return fail(c, OVL_E_SPECTRUM, "cut must be in (0, 1)");
@@ -503,41 +538,82 @@ int make_plan(ovl_ctx *c, const ovl_sign_opts *o, SignPlan &p)
     const double d2 = o->cut * o->cut;
     p.alpha = 2.0 / (p.lam * p.lam * (1.0 - d2));
     p.beta = -(1.0 + d2) / (1.0 - d2);
-    if (o->precision == OVL_PREC_FP64) p.head = p.n + 1;
-    else if (o->precision == OVL_PREC_FP32) p.head = 0;
-    else if (o->head >= 0) p.head = std::min(o->head, p.n + 1);
-    else p.head = choose_head(p.coef, o->tol > 0 ? o->tol : 1e-10);
+    const double u32 = 5.96e-8, u16 = 4.89e-4;
+    const double tol = o->tol > 0 ? o->tol : 1e-10;
+    p.head16 = p.n + 1;
+    if (o->precision == OVL_PREC_FP64) {
+        p.head = p.n + 1;
+    } else if (o->precision == OVL_PREC_FP32) {
+        p.head = 0;
+    } else {
+        const std::vector<double> B = clenshaw_sup(p.coef, -1.0);
+        const bool fp16 = o->fp16_tail != 0;
+        // error budget: 3/4 to the FP32 tier, 1/4 to the FP16 tier
+        if (o->head >= 0) p.head = std::min(o->head, p.n + 1);
+        else p.head = tier_start(B, p.n, u32, fp16 ? 0.75 * tol : tol);
+        if (fp16) {
+            if (o->head16 > 0) p.head16 = std::min(o->head16, p.n + 1);
+            else p.head16 = tier_start(B, p.n, u16, 0.25 * tol);
+            // at least one FP32 step between the FP16 tail and the FP64 head
+            p.head16 = std::max(p.head16, p.head + 1);
+        }
+    }
+    if (p.head16 <= p.n) {
+        if (!(xmax > 0.0) || !std::isfinite(xmax)) return fail(c, OVL_E_ARG, "FP16 tier needs a bound of the input norms");
+        const std::vector<double> Ball = clenshaw_sup(p.coef, p.beta);
+        p.sk.assign(p.n + 2, 1.0);
+        p.sw.assign(p.n + 2, 1.0);
+        for (int k = 0; k <= p.n; ++k) {
+            p.sk[k] = fp16_scale(Ball[k] * xmax);
+            p.sw[k] = fp16_scale(p.lam * Ball[k + 1] * xmax);
+        }
+        p.sx = fp16_scale(xmax);
+    }
     return OVL_OK;
 }
 
 // Clenshaw recurrence for one chunk of mode blocks:  p = P(T) x,  T = alpha h_W^2 + beta.
-// Steps k >= head run in FP32, k < head in FP64.  Returns the buffer and precision of p.
+// Steps k >= head16 store their vectors in FP16 (scaled), head <= k < head16 in FP32, k < head
+// in FP64.  Returns the buffer and precision of p.
 int clenshaw_chunk(ovl_ctx *c, const Geom &gc, const SignPlan &pl, const void *x64, void **pout, int *pprec)
 {
     const int n = pl.n;
+    const bool any16 = pl.head16 <= n;
     const bool any32 = pl.head <= n;
-    RETCHK(ensure_work(c, gc.nblk, any32));
+    RETCHK(ensure_work(c, gc.nblk, any32, any16));
     if (any32) HIPCHK(c, ovl::launch_convert(gc, gc.nblk, x64, P64, c->x32, P32, c->st));
-    // buffer rotation: b1 = b_{k+1}, b2 = b_{k+2} (null = zero)
+    if (any16) HIPCHK(c, ovl::launch_convert(gc, gc.nblk, x64, P64, c->x16, P16, c->st, pl.sx));
+    // buffer rotation: b1 = b_{k+1}, b2 = b_{k+2} (null = zero), stored times sc1 / sc2
     void *b1 = nullptr, *b2 = nullptr;
+    double sc1 = 1.0, sc2 = 1.0;
     int prec_b = -1;  // precision of b1/b2
     for (int k = n; k >= 0; --k) {
-        const int pk = (k < pl.head) ? P64 : P32;
-        if (pk == P64) c->fp64_steps++; else c->fp32_steps++;
-        void *xk = (pk == P64) ? (void *)x64 : c->x32;
-        void *A = (pk == P64) ? c->a64 : c->a32, *B = (pk == P64) ? c->b64 : c->b32;
-        void *W = (pk == P64) ? c->w64 : c->w32;
+        const int pk = (k < pl.head) ? P64 : (k < pl.head16 ? P32 : P16);
+        if (pk == P64) c->fp64_steps++;
+        else if (pk == P32) c->fp32_steps++;
+        else c->fp16_steps++;
+        void *xk = (pk == P64) ? (void *)x64 : (pk == P32 ? c->x32 : c->x16);
+        void *A = (pk == P64) ? c->a64 : (pk == P32 ? c->a32 : c->a16);
+        void *B = (pk == P64) ? c->b64 : (pk == P32 ? c->b32 : c->b16);
+        void *W = (pk == P64) ? c->w64 : (pk == P32 ? c->w32 : c->w16);
+        const double sk = (pk == P16) ? pl.sk[k] : 1.0, sw = (pk == P16) ? pl.sw[k] : 1.0;
+        const double sx = (pk == P16) ? pl.sx : 1.0;
         if (b1 && prec_b != pk) {
-            // FP32 -> FP64 transition: promote b_{k+1}, b_{k+2}
-            HIPCHK(c, ovl::launch_convert(gc, gc.nblk, b1, P32, c->a64, P64, c->st));
-            if (b2) HIPCHK(c, ovl::launch_convert(gc, gc.nblk, b2, P32, c->b64, P64, c->st));
-            b1 = c->a64;
-            b2 = b2 ? c->b64 : nullptr;
+            // FP16 -> FP32 or FP32 -> FP64 transition: promote b_{k+1}, b_{k+2} (and drop the scale)
+            HIPCHK(c, ovl::launch_convert(gc, gc.nblk, b1, prec_b, A, pk, c->st, 1.0 / sc1));
+            if (b2) HIPCHK(c, ovl::launch_convert(gc, gc.nblk, b2, prec_b, B, pk, c->st, 1.0 / sc2));
+            b1 = A;
+            b2 = b2 ? B : nullptr;
+            sc1 = sc2 = 1.0;
             prec_b = pk;
         }
         if (!b1) {  // b_n = c_n x
-            HIPCHK(c, ovl::launch_lincomb(gc, gc.nblk, pk, A, xk, pl.coef[k], 0.0, nullptr, 0, 0, c->st));
+            if (pk == P16)
+                HIPCHK(c, ovl::launch_convert(gc, gc.nblk, x64, P64, A, P16, c->st, pl.coef[k] * sk));
+            else
+                HIPCHK(c, ovl::launch_lincomb(gc, gc.nblk, pk, A, xk, pl.coef[k], 0.0, nullptr, 0, 0, c->st));
             b1 = A;
+            sc1 = sk;
             prec_b = pk;
             continue;
         }
@@ -546,7 +622,7 @@ int clenshaw_chunk(ovl_ctx *c, const Geom &gc, const SignPlan &pl, const void *x
         a.nblk = gc.nblk;
         a.in = b1;
         a.out = W;
-        a.sh_r = 1.0;
+        a.sh_r = sw / sc1;
         RETCHK(hop(c, gc, pk, pk, a));
         // b_k = c_k x + 2 alpha h_W w + 2 beta b_{k+1} - b_{k+2}    (k >= 1)
         // p   = c_0 x +   alpha h_W w +   beta b_1     - b_2        (k == 0)
@@ -556,16 +632,18 @@ int clenshaw_chunk(ovl_ctx *c, const Geom &gc, const SignPlan &pl, const void *x
         q.nblk = gc.nblk;
         q.in = W;
         q.out = out;
-        q.sh_r = f * pl.alpha;
+        q.sh_r = f * pl.alpha * sk / sw;
         q.x = xk;
-        q.sx_r = pl.coef[k];
+        q.sx_r = pl.coef[k] * sk / sx;
         q.b1 = b1;
-        q.s1_r = f * pl.beta;
+        q.s1_r = f * pl.beta * sk / sc1;
         q.b2 = b2;
-        q.s2_r = -1.0;
+        q.s2_r = -sk / sc2;
         RETCHK(hop(c, gc, pk, pk, q));
         b2 = b1;
+        sc2 = sc1;
         b1 = out;
+        sc1 = sk;
     }
     *pout = b1;
     *pprec = prec_b;
@@ -575,11 +653,13 @@ int clenshaw_chunk(ovl_ctx *c, const Geom &gc, const SignPlan &pl, const void *x
 // result = sh * h_W(p) + sx * G(x) + st * tgt for every chunk; kind selects the combination
 enum FinalKind { FIN_SIGN = 0, FIN_OVERLAP = 1, FIN_OVERLAP_AXPY = 2 };
 
-int sign_driver(ovl_ctx *c, const ovl_sign_opts *o, const void *in64, void *out64, int kind, double dt)
+// xmax: bound of the 2-norm of every input mode (used by the FP16 tier only)
+int sign_driver(ovl_ctx *c, const ovl_sign_opts *o, const void *in64, void *out64, int kind, double dt,
+                double xmax)
 {
     RETCHK(ensure_stencil(c));
     SignPlan pl;
-    RETCHK(make_plan(c, o, pl));
+    RETCHK(make_plan(c, o, pl, xmax));
     const int chunk = (o->mode_chunk > 0) ? std::min(o->mode_chunk, c->g.nblk) : c->g.nblk;
     const double M = c->prm.m_dw, m = c->prm.m_fermion;
     for (int b0 = 0; b0 < c->g.nblk; b0 += chunk) {
@@ -722,6 +802,20 @@ int mode_dot_all(ovl_ctx *c, const void *a, const void *b, std::vector<cplx> &ou
         RETCHK(mode_dot(c, sys_geom(c, s), sys_field(c, a, s), sys_field(c, b, s), d));
         out.insert(out.end(), d.begin(), d.begin() + c->prm.n_modes);
     }
+    return OVL_OK;
+}
+
+// max over all modes of all systems of the 2-norm of an FP64 field (0 when the FP16 tier is off)
+int input_bound(ovl_ctx *c, const ovl_sign_opts *o, const void *f, double *xmax)
+{
+    *xmax = 0.0;
+    if (o->method != OVL_SIGN_CHEBYSHEV || o->precision != OVL_PREC_MIXED || !o->fp16_tail) return OVL_OK;
+    std::vector<cplx> d;
+    RETCHK(mode_dot_all(c, f, f, d));
+    double m = 0;
+    for (const cplx &v : d) m = std::max(m, v.real());
+    *xmax = std::sqrt(m);
+    if (!(*xmax > 0.0)) *xmax = 1.0;  // all-zero input: any scale works
     return OVL_OK;
 }
 
@@ -878,6 +972,8 @@ void ovl_sign_defaults(ovl_sign_opts *o)
     o->cut = 0.05;
     o->tol = 1e-10;
     o->head = -1;
+    o->fp16_tail = 1;
+    o->head16 = 0;
 }
 
 int ovl_device_name(char *buf, int len)
@@ -898,6 +994,7 @@ static int ctx_init(ovl_ctx *c)
     c->ncomp = 4 * nc;
     c->blk64_elems = (size_t)c->g.V * c->ncomp * mb;
     c->blk32_elems = (size_t)c->g.V * (c->ncomp / 2) * mb;
+    c->blk16_elems = (size_t)c->g.V * (c->ncomp / 4) * mb;
     c->field64_bytes = c->blk64_elems * 16 * c->g.nblk;
     c->field32_bytes = c->blk32_elems * 16 * c->g.nblk;
     c->sys_nb.assign(c->sys.size(), 0);
@@ -936,7 +1033,7 @@ static int ctx_init(ovl_ctx *c)
     if (!c->ensemble) {
         // traversal 0 (automatic): mode split where the mode blocks divide over the XCDs, else
         // boxes; ovl_autotune may change it
-        for (int p = P64; p <= P32; ++p) {
+        for (int p = P64; p <= P16; ++p) {
             int t = ovl::TRAV_LEX;
             if (c->prm.traversal == 0) t = (c->g.nblk % ovl::NUM_XCD == 0) ? ovl::TRAV_MODES : ovl::TRAV_BOXES;
             if (c->prm.traversal == 2) t = ovl::TRAV_BOXES;
@@ -1088,7 +1185,8 @@ int ovl_destroy(ovl_ctx *c)
     void *ps[] = {c->links, c->efield, c->ul64, c->ul32, c->phase, c->field[0], c->field[1], c->field[2], c->w64, c->a64,
                   c->b64, c->x32, c->w32, c->a32, c->b32, c->partial, c->dout, c->gam, c->jv, c->ja, c->staging,
                   c->ms_ps, c->ms_r, c->ms_q, c->ms_w, c->ms_chi, c->ms_coef, c->ms_active, c->rz[0], c->rz[1],
-                  c->rz[2], c->rz[3], c->rz[4], c->rz[5], c->xcd_order, c->d_sys, c->d_work, c->slab_order[0], c->slab_order[1]};
+                  c->rz[2], c->rz[3], c->rz[4], c->rz[5], c->xcd_order, c->d_sys, c->d_work, c->slab_order[0], c->slab_order[1],
+                  c->slab_order[2], c->x16, c->w16, c->a16, c->b16};
     for (void *p : ps)
         if (p) hipFree(p);
     for (auto e : c->evpool) hipEventDestroy(e);
@@ -1358,7 +1456,10 @@ int ovl_apply_overlap(ovl_ctx *c, int in, int out, const ovl_sign_opts *o)
         end_timing(c);
         return r;
     }
-    RETCHK(sign_driver(c, o, c->field[in], c->field[out], FIN_OVERLAP, 0.0));
+    double xmax;
+    RETCHK(input_bound(c, o, c->field[in], &xmax));
+    begin_timing(c);
+    RETCHK(sign_driver(c, o, c->field[in], c->field[out], FIN_OVERLAP, 0.0, xmax));
     return end_timing(c);
 }
 
@@ -1370,8 +1471,10 @@ int ovl_apply_sign(ovl_ctx *c, int in, int out, const ovl_sign_opts *o)
     RETCHK(check_sign_opts(c, o));
     if (o->method != OVL_SIGN_CHEBYSHEV) return fail(c, OVL_E_ARG, "ovl_apply_sign supports the Chebyshev sign");
     if (in == out) return fail(c, OVL_E_ARG, "in and out must be different fields");
+    double xmax;
+    RETCHK(input_bound(c, o, c->field[in], &xmax));
     begin_timing(c);
-    RETCHK(sign_driver(c, o, c->field[in], c->field[out], FIN_SIGN, 0.0));
+    RETCHK(sign_driver(c, o, c->field[in], c->field[out], FIN_SIGN, 0.0, xmax));
     return end_timing(c);
 }
 
@@ -1399,8 +1502,11 @@ int ovl_sign_residual(ovl_ctx *c, int in, const ovl_sign_opts *o, double *rel)
         hipFree(t1);
         return OVL_E_NOMEM;
     }
-    int r = sign_driver(c, o, c->field[in], t1, FIN_SIGN, 0.0);
-    if (r == OVL_OK) r = sign_driver(c, o, t1, t2, FIN_SIGN, 0.0);
+    double xmax = 0;
+    int r = input_bound(c, o, c->field[in], &xmax);
+    // |eps(x)| <= ~|x| (the polynomial sign is bounded by about 1 on the whole spectrum)
+    if (r == OVL_OK) r = sign_driver(c, o, c->field[in], t1, FIN_SIGN, 0.0, xmax);
+    if (r == OVL_OK) r = sign_driver(c, o, t1, t2, FIN_SIGN, 0.0, 2.0 * xmax);
     std::vector<cplx> dx, dd;
     if (r == OVL_OK) {
         ovl::launch_lincomb(c->g, c->g.nblk, P64, t2, t2, 1, 0, c->field[in], -1, 0, c->st);
@@ -1682,7 +1788,7 @@ static int fermion_half(ovl_ctx *c, const ovl_step_opts *o, int src, int tgt)
     }
     if (o->sign.method == OVL_SIGN_ZOLOTAREV)
         return zolo_overlap(c, c->g, c->field[src], c->field[tgt], o->sign.lmin, o->sign.lmax, 1, o->dt);
-    return sign_driver(c, &o->sign, c->field[src], c->field[tgt], FIN_OVERLAP_AXPY, o->dt);
+    return sign_driver(c, &o->sign, c->field[src], c->field[tgt], FIN_OVERLAP_AXPY, o->dt, c->step_xmax);
 }
 
 static int one_step(ovl_ctx *c, const ovl_step_opts *o)
@@ -1700,9 +1806,16 @@ int ovl_fermion_update(ovl_ctx *c, const ovl_step_opts *o, int nsteps)
     if (o->kind != OVL_WILSON && o->kind != OVL_OVERLAP) return fail(c, OVL_E_ARG, "unknown fermion kind");
     if (o->gauge < OVL_GAUGE_FROZEN || o->gauge > OVL_GAUGE_FULL) return fail(c, OVL_E_ARG, "unknown gauge mode");
     if (o->kind == OVL_OVERLAP) RETCHK(check_sign_opts(c, &o->sign));
+    c->step_xmax = 0.0;
     if (o->kind == OVL_OVERLAP && o->sign.method == OVL_SIGN_CHEBYSHEV) {
+        // bound of the mode norms for the FP16 scales: the staggered leapfrog conserves the norms
+        // for dt |omega| < 2, so twice the larger of the two fields at the start is a safe bound
+        double b0, b1;
+        RETCHK(input_bound(c, &o->sign, c->field[OVL_PSI], &b0));
+        RETCHK(input_bound(c, &o->sign, c->field[OVL_PSIMID], &b1));
+        c->step_xmax = 2.0 * std::max(b0, b1);
         SignPlan pl;
-        RETCHK(make_plan(c, &o->sign, pl));
+        RETCHK(make_plan(c, &o->sign, pl, c->step_xmax));
     }
     RETCHK(ensure_stencil(c));
     begin_timing(c);
@@ -1712,8 +1825,10 @@ int ovl_fermion_update(ovl_ctx *c, const ovl_step_opts *o, int nsteps)
         return end_timing(c);
     }
     // capture one full step (both half steps and both gauge half steps) and replay it
+    // every buffer the captured step touches is allocated before the capture
     RETCHK(ensure_work(c, (o->sign.mode_chunk > 0) ? std::min(o->sign.mode_chunk, c->g.nblk) : c->g.nblk,
-                       o->kind == OVL_OVERLAP && o->sign.precision != OVL_PREC_FP64));
+                       o->kind == OVL_OVERLAP && o->sign.precision != OVL_PREC_FP64,
+                       o->kind == OVL_OVERLAP && o->sign.precision == OVL_PREC_MIXED && o->sign.fp16_tail));
     hipGraph_t graph_h = nullptr;
     hipGraphExec_t exec = nullptr;
     HIPCHK(c, hipStreamSynchronize(c->st));
@@ -1726,10 +1841,10 @@ int ovl_fermion_update(ovl_ctx *c, const ovl_step_opts *o, int nsteps)
     }
     const long long l0 = c->hop_launches;
     const double b0 = c->hop_bytes;
-    long long vl0[3];
-    double vb0[3];
-    for (int v = 0; v < 3; ++v) { vl0[v] = c->var_launches[v]; vb0[v] = c->var_bytes[v]; }
-    const int s64 = c->fp64_steps, s32 = c->fp32_steps;
+    long long vl0[4];
+    double vb0[4];
+    for (int v = 0; v < 4; ++v) { vl0[v] = c->var_launches[v]; vb0[v] = c->var_bytes[v]; }
+    const int s64 = c->fp64_steps, s32 = c->fp32_steps, s16 = c->fp16_steps;
     int r = one_step(c, o);
     hipError_t ee = hipStreamEndCapture(c->st, &graph_h);
     c->capturing = false;
@@ -1740,7 +1855,7 @@ int ovl_fermion_update(ovl_ctx *c, const ovl_step_opts *o, int nsteps)
     if (ee != hipSuccess) return fail(c, OVL_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ee));
     const long long per_l = c->hop_launches - l0;
     const double per_b = c->hop_bytes - b0;
-    const int p64 = c->fp64_steps - s64, p32 = c->fp32_steps - s32;
+    const int p64 = c->fp64_steps - s64, p32 = c->fp32_steps - s32, p16 = c->fp16_steps - s16;
     e = hipGraphInstantiate(&exec, graph_h, nullptr, nullptr, 0);
     if (e != hipSuccess) {
         hipGraphDestroy(graph_h);
@@ -1752,12 +1867,13 @@ int ovl_fermion_update(ovl_ctx *c, const ovl_step_opts *o, int nsteps)
     if (e != hipSuccess) return fail(c, OVL_E_HIP, std::string("hipGraphLaunch: ") + hipGetErrorString(e));
     c->hop_launches = l0 + per_l * nsteps;
     c->hop_bytes = b0 + per_b * nsteps;
-    for (int v = 0; v < 3; ++v) {
+    for (int v = 0; v < 4; ++v) {
         c->var_launches[v] = vl0[v] + (c->var_launches[v] - vl0[v]) * nsteps;
         c->var_bytes[v] = vb0[v] + (c->var_bytes[v] - vb0[v]) * nsteps;
     }
     c->fp64_steps = s64 + p64 * nsteps;
     c->fp32_steps = s32 + p32 * nsteps;
+    c->fp16_steps = s16 + p16 * nsteps;
     return end_timing(c);
 }
 
@@ -1942,7 +2058,11 @@ int ovl_currents(ovl_ctx *c, int field, const ovl_sign_opts *o, int first_mode, 
     if (o->method == OVL_SIGN_ZOLOTAREV)
         RETCHK(zolo_overlap(c, c->g, c->field[field], c->field[OVL_SCRATCH], o->lmin, o->lmax, 0, 0.0));
     else
-        RETCHK(sign_driver(c, o, c->field[field], c->field[OVL_SCRATCH], FIN_OVERLAP, 0.0));
+    {
+        double xmax;
+        RETCHK(input_bound(c, o, c->field[field], &xmax));
+        RETCHK(sign_driver(c, o, c->field[field], c->field[OVL_SCRATCH], FIN_OVERLAP, 0.0, xmax));
+    }
     if (!c->gam) {
         std::vector<double> g;
         build_gammas(g);
@@ -2079,8 +2199,11 @@ int ovl_autotune(ovl_ctx *c, char *report, int len)
     // except the work buffers.
     if (!c) return OVL_E_ARG;
     RETCHK(ensure_stencil(c));
-    RETCHK(ensure_work(c, c->g.nblk, true));
+    RETCHK(ensure_work(c, c->g.nblk, true, true));
     HIPCHK(c, ovl::launch_convert(c->g, c->g.nblk, c->field[OVL_PSI], P64, c->x32, P32, c->st));
+    HIPCHK(c, ovl::launch_convert(c->g, c->g.nblk, c->field[OVL_PSI], P64, c->x16, P16, c->st, 64.0));
+    HIPCHK(c, ovl::launch_convert(c->g, c->g.nblk, c->field[OVL_PSIMID], P64, c->a16, P16, c->st, 64.0));
+    HIPCHK(c, ovl::launch_convert(c->g, c->g.nblk, c->field[OVL_PSI], P64, c->b16, P16, c->st, 64.0));
     HIPCHK(c, ovl::launch_convert(c->g, c->g.nblk, c->field[OVL_PSIMID], P64, c->a32, P32, c->st));
     HIPCHK(c, ovl::launch_convert(c->g, c->g.nblk, c->field[OVL_PSI], P64, c->b32, P32, c->st));
     HIPCHK(c, ovl::launch_lincomb(c->g, c->g.nblk, P64, c->a64, c->field[OVL_PSIMID], 1, 0, nullptr, 0, 0, c->st));
@@ -2088,8 +2211,12 @@ int ovl_autotune(ovl_ctx *c, char *report, int len)
     std::string rep;
     const bool prof = c->profiling;
     c->profiling = false;
-    static const char *tname[3] = {"lex", "boxes", "modes"};
-    for (int prec = P64; prec <= P32; ++prec) {
+    static const char *pname[3] = {"fp64", "fp32", "fp16"};
+    auto trav_name = [](const ovl_ctx *cc, int prec) {
+        if (cc->trav[prec] == ovl::TRAV_MODES) return std::string("modes/") + std::to_string(cc->slabs[prec]);
+        return std::string(cc->trav[prec] == ovl::TRAV_BOXES ? "boxes" : "lex");
+    };
+    for (int prec = P64; prec <= P16; ++prec) {
         // candidate traversals (trav, slabs)
         std::vector<std::pair<int, int>> cand;
         if (c->ensemble || c->prm.traversal != 0) {
@@ -2099,9 +2226,10 @@ int ovl_autotune(ovl_ctx *c, char *report, int len)
             for (int t : {1, 2, 3, 4, 6})
                 if (t <= c->g.n1) cand.push_back({ovl::TRAV_MODES, t});
         }
-        void *x = prec == P64 ? c->field[OVL_PSI] : c->x32;
-        void *b1 = prec == P64 ? c->a64 : c->a32, *b2 = prec == P64 ? c->b64 : c->b32;
-        void *w = prec == P64 ? c->w64 : c->w32;
+        void *x = prec == P64 ? c->field[OVL_PSI] : (prec == P32 ? c->x32 : c->x16);
+        void *b1 = prec == P64 ? c->a64 : (prec == P32 ? c->a32 : c->a16);
+        void *b2 = prec == P64 ? c->b64 : (prec == P32 ? c->b32 : c->b16);
+        void *w = prec == P64 ? c->w64 : (prec == P32 ? c->w32 : c->w16);
         double best = 1e30;
         int best_v = 1, best_nt = 0;
         std::pair<int, int> best_t = cand[0];
@@ -2137,8 +2265,8 @@ int ovl_autotune(ovl_ctx *c, char *report, int len)
                         HIPCHK(c, hipEventElapsedTime(&ms, c->t0, c->t1));
                     }
                     char buf[96];
-                    std::snprintf(buf, sizeof buf, "%s %s/%d%s wpe=%d %.1f us; ", prec == P64 ? "fp64" : "fp32",
-                                  tname[c->trav[prec]], c->slabs[prec], nt ? " nt" : "", v, 1e3 * ms / 3);
+                    std::snprintf(buf, sizeof buf, "%s %s%s wpe=%d %.1f us; ", pname[prec],
+                                  trav_name(c, prec).c_str(), nt ? " nt" : "", v, 1e3 * ms / 3);
                     rep += buf;
                     if (ms < best) {
                         best = ms;
@@ -2153,11 +2281,13 @@ int ovl_autotune(ovl_ctx *c, char *report, int len)
         c->nt_out[prec] = best_nt;
     }
     c->profiling = prof;
-    char buf[160];
-    std::snprintf(buf, sizeof buf, "chosen fp64 %s/%d%s wpe=%d, fp32 %s/%d%s wpe=%d", tname[c->trav[P64]],
-                  c->slabs[P64], c->nt_out[P64] ? " nt" : "", c->wpe[P64], tname[c->trav[P32]], c->slabs[P32],
-                  c->nt_out[P32] ? " nt" : "", c->wpe[P32]);
-    rep += buf;
+    rep += "chosen";
+    for (int prec = P64; prec <= P16; ++prec) {
+        char buf[96];
+        std::snprintf(buf, sizeof buf, "%s %s %s%s wpe=%d", prec == P64 ? "" : ",", pname[prec],
+                      trav_name(c, prec).c_str(), c->nt_out[prec] ? " nt" : "", c->wpe[prec]);
+        rep += buf;
+    }
     if (report && len > 0) std::snprintf(report, len, "%s", rep.c_str());
     return OVL_OK;
 }

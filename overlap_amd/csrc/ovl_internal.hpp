@@ -5,6 +5,7 @@
 //   fastest-running index so a wavefront reads MB consecutive modes of one site per load.
 //   FP64: double2 per complex, element (blk, site, comp, m) at ((blk*V + site)*NCOMP + comp)*MB + m
 //   FP32: float4 per two complex,      (blk, site, comp/2, m) at ((blk*V + site)*NCOMP/2 + comp/2)*MB + m
+//   FP16: 8 halves per four complex,   (blk, site, comp/4, m) at ((blk*V + site)*NCOMP/4 + comp/4)*MB + m
 //   comp = spin*Nc + colour, NCOMP = 4*Nc.
 // Stencil links: [mu][site][Nc][Nc] complex, row-major, FP64 (double2) and FP32 (float2) copies,
 // each already multiplied by its U(1) phase.
@@ -68,13 +69,17 @@ constexpr int NUM_XCD = 8;
 // threads per workgroup of the stencil kernels
 constexpr int STENCIL_TPB = 256;
 
-enum Prec { P64 = 0, P32 = 1 };
+// field storage precisions; FP16 fields hold values scaled by a power of two chosen by the host
+// (the stencil computes in FP32 and the scale is folded into the epilogue coefficients)
+enum Prec { P64 = 0, P32 = 1, P16 = 2 };
 
 // launchers (ovl_kernels.hip); all enqueue on `st` and return hipError_t
 // wpe: occupancy variant of the stencil kernel (1 = compiler default, 2/3/4 = at least that many
 // waves per SIMD)
 hipError_t launch_hop(const Geom &g, int pin, int pout, const HopArgs &a, int wpe, hipStream_t st);
-hipError_t launch_convert(const Geom &g, int nblk, const void *src, int psrc, void *dst, int pdst, hipStream_t st);
+// dst = scale * src between precisions: P64->P32, P32->P64, P64->P16, P16->P32
+hipError_t launch_convert(const Geom &g, int nblk, const void *src, int psrc, void *dst, int pdst, hipStream_t st,
+                          double scale = 1.0);
 // out = ca * a + cb * b (same precision p), b may be null
 hipError_t launch_lincomb(const Geom &g, int nblk, int p, void *out, const void *a, double ca_r, double ca_i,
                           const void *b, double cb_r, double cb_i, hipStream_t st);

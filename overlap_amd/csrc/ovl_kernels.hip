@@ -14,18 +14,36 @@ namespace {
 
 constexpr int TPB = STENCIL_TPB;  // threads per block for the streaming kernels
 
+// storage type -> arithmetic type (C) and stencil link type (L).  FP16 storage computes in FP32
+// with the FP32 links.
 template <typename T> struct Vec;
-template <> struct Vec<double> { using F = double2; using L = double2; static constexpr int PER = 1; };
-template <> struct Vec<float> { using F = float4; using L = float2; static constexpr int PER = 2; };
+template <> struct Vec<double> { using C = double; using L = double2; };
+template <> struct Vec<float> { using C = float; using L = float2; };
+template <> struct Vec<_Float16> { using C = float; using L = float2; };
+
+// 16 bytes of FP16 storage: 4 complex numbers
+typedef _Float16 h8v __attribute__((ext_vector_type(8)));
 
 // ---------------------------------------------------------------------------------------------
 // spinor load / store for one lane
 // ---------------------------------------------------------------------------------------------
-template <int NC, int MB, typename T>
-__device__ __forceinline__ void load_sp(const void *__restrict__ base, int V, int blk, int s, int m, T *re, T *im)
+// FP16 layout: 16-byte units of 4 complex, (blk, site, comp/4, m) at ((blk*V + site)*NCOMP/4 + comp/4)*MB + m
+template <int NC, int MB, typename T, typename C>
+__device__ __forceinline__ void load_sp(const void *__restrict__ base, int V, int blk, int s, int m, C *re, C *im)
 {
     constexpr int NCOMP = 4 * NC;
-    if constexpr (sizeof(T) == 8) {
+    if constexpr (sizeof(T) == 2) {
+        const h8v *p = reinterpret_cast<const h8v *>(base) + ((size_t)(blk * V + s) * (NCOMP / 4)) * MB + m;
+#pragma unroll
+        for (int u = 0; u < NCOMP / 4; ++u) {
+            const h8v t = p[(size_t)u * MB];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                re[4 * u + j] = (C)t[2 * j];
+                im[4 * u + j] = (C)t[2 * j + 1];
+            }
+        }
+    } else if constexpr (sizeof(T) == 8) {
         const double2 *p = reinterpret_cast<const double2 *>(base) + ((size_t)(blk * V + s) * NCOMP) * MB + m;
 #pragma unroll
         for (int c = 0; c < NCOMP; ++c) {
@@ -51,11 +69,22 @@ __device__ __forceinline__ void load_sp(const void *__restrict__ base, int V, in
 typedef double d2v __attribute__((ext_vector_type(2)));
 typedef float f4v __attribute__((ext_vector_type(4)));
 
-template <int NC, int MB, typename T>
-__device__ __forceinline__ void load_sp_nt(const void *__restrict__ base, int V, int blk, int s, int m, T *re, T *im)
+template <int NC, int MB, typename T, typename C>
+__device__ __forceinline__ void load_sp_nt(const void *__restrict__ base, int V, int blk, int s, int m, C *re, C *im)
 {
     constexpr int NCOMP = 4 * NC;
-    if constexpr (sizeof(T) == 8) {
+    if constexpr (sizeof(T) == 2) {
+        const h8v *p = reinterpret_cast<const h8v *>(base) + ((size_t)(blk * V + s) * (NCOMP / 4)) * MB + m;
+#pragma unroll
+        for (int u = 0; u < NCOMP / 4; ++u) {
+            const h8v t = __builtin_nontemporal_load(p + (size_t)u * MB);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                re[4 * u + j] = (C)t[2 * j];
+                im[4 * u + j] = (C)t[2 * j + 1];
+            }
+        }
+    } else if constexpr (sizeof(T) == 8) {
         const d2v *p = reinterpret_cast<const d2v *>(base) + ((size_t)(blk * V + s) * NCOMP) * MB + m;
 #pragma unroll
         for (int c = 0; c < NCOMP; ++c) {
@@ -77,12 +106,28 @@ __device__ __forceinline__ void load_sp_nt(const void *__restrict__ base, int V,
 }
 
 // the output is read by the very next launch, so it is stored normally (it may stay in the MALL)
-template <int NC, int MB, typename T>
-__device__ __forceinline__ void store_sp(void *__restrict__ base, int V, int blk, int s, int m, const T *re,
-                                         const T *im)
+template <int NC, int MB, typename T, typename C>
+__device__ __forceinline__ h8v pack_h8(const C *re, const C *im, int u)
+{
+    h8v v;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        v[2 * j] = (_Float16)re[4 * u + j];
+        v[2 * j + 1] = (_Float16)im[4 * u + j];
+    }
+    return v;
+}
+
+template <int NC, int MB, typename T, typename C>
+__device__ __forceinline__ void store_sp(void *__restrict__ base, int V, int blk, int s, int m, const C *re,
+                                         const C *im)
 {
     constexpr int NCOMP = 4 * NC;
-    if constexpr (sizeof(T) == 8) {
+    if constexpr (sizeof(T) == 2) {
+        h8v *p = reinterpret_cast<h8v *>(base) + ((size_t)(blk * V + s) * (NCOMP / 4)) * MB + m;
+#pragma unroll
+        for (int u = 0; u < NCOMP / 4; ++u) p[(size_t)u * MB] = pack_h8<NC, MB, T>(re, im, u);
+    } else if constexpr (sizeof(T) == 8) {
         double2 *p = reinterpret_cast<double2 *>(base) + ((size_t)(blk * V + s) * NCOMP) * MB + m;
 #pragma unroll
         for (int c = 0; c < NCOMP; ++c) p[(size_t)c * MB] = make_double2(re[c], im[c]);
@@ -95,12 +140,16 @@ __device__ __forceinline__ void store_sp(void *__restrict__ base, int V, int blk
 }
 
 // streaming store of an output that the launch never reads back (keeps L2 for the stencil input)
-template <int NC, int MB, typename T>
-__device__ __forceinline__ void store_sp_nt(void *__restrict__ base, int V, int blk, int s, int m, const T *re,
-                                            const T *im)
+template <int NC, int MB, typename T, typename C>
+__device__ __forceinline__ void store_sp_nt(void *__restrict__ base, int V, int blk, int s, int m, const C *re,
+                                            const C *im)
 {
     constexpr int NCOMP = 4 * NC;
-    if constexpr (sizeof(T) == 8) {
+    if constexpr (sizeof(T) == 2) {
+        h8v *p = reinterpret_cast<h8v *>(base) + ((size_t)(blk * V + s) * (NCOMP / 4)) * MB + m;
+#pragma unroll
+        for (int u = 0; u < NCOMP / 4; ++u) __builtin_nontemporal_store(pack_h8<NC, MB, T>(re, im, u), p + (size_t)u * MB);
+    } else if constexpr (sizeof(T) == 8) {
         d2v *p = reinterpret_cast<d2v *>(base) + ((size_t)(blk * V + s) * NCOMP) * MB + m;
 #pragma unroll
         for (int c = 0; c < NCOMP; ++c) {
@@ -235,6 +284,8 @@ __device__ __forceinline__ void hop_site(const HopArgs &a, int n0, int n1, int n
 {
     constexpr int NCOMP = 4 * NC;
     using LT = typename Vec<TI>::L;
+    using CI = typename Vec<TI>::C;  // arithmetic type of the stencil
+    using CO = typename Vec<TO>::C;  // arithmetic type of the epilogue
     const int V = n0 * n1 * n2;
     const int z = s / (n0 * n1);
     const int rem = s - z * n0 * n1;
@@ -244,10 +295,10 @@ __device__ __forceinline__ void hop_site(const HopArgs &a, int n0, int n1, int n
     const LT *__restrict__ L = reinterpret_cast<const LT *>(a.links) + (size_t)l0 * NC * NC;
     constexpr int NN = NC * NC;
 
-    TI xr[NCOMP], xi[NCOMP];
-    TI pr[NCOMP], pi[NCOMP];
+    CI xr[NCOMP], xi[NCOMP];
+    CI pr[NCOMP], pi[NCOMP];
     load_sp<NC, MB, TI>(a.in, 0, 0, q0 + s, m, pr, pi);
-    const TI am = (TI)a.amass;
+    const CI am = (CI)a.amass;
 #pragma unroll
     for (int c = 0; c < NCOMP; ++c) { xr[c] = am * pr[c]; xi[c] = am * pi[c]; }
 
@@ -277,43 +328,43 @@ __device__ __forceinline__ void hop_site(const HopArgs &a, int n0, int n1, int n
     }
 
     // h_W = (1/2) gamma0 X, then the epilogue in the output precision
-    TO orr[NCOMP], oi[NCOMP];
-    const TO shr = (TO)a.sh_r, shi = (TO)a.sh_i;
+    CO orr[NCOMP], oi[NCOMP];
+    const CO shr = (CO)a.sh_r, shi = (CO)a.sh_i;
 #pragma unroll
     for (int c = 0; c < NCOMP; ++c) {
-        const TO g = (c < 2 * NC) ? (TO)0.5 : (TO)-0.5;
-        const TO hr = g * (TO)xr[c], hi = g * (TO)xi[c];
+        const CO g = (c < 2 * NC) ? (CO)0.5 : (CO)-0.5;
+        const CO hr = g * (CO)xr[c], hi = g * (CO)xi[c];
         orr[c] = shr * hr - shi * hi;
         oi[c] = shr * hi + shi * hr;
     }
-    TO tr[NCOMP], ti[NCOMP];
+    CO tr[NCOMP], ti[NCOMP];
     const int q = q0 + s;
     if (a.x) {
         load_sp_nt<NC, MB, TO>(a.x, 0, 0, q, m, tr, ti);
-        const TO cr = (TO)a.sx_r, ci = (TO)a.sx_i;
+        const CO cr = (CO)a.sx_r, ci = (CO)a.sx_i;
 #pragma unroll
         for (int c = 0; c < NCOMP; ++c) {
-            const TO g = (a.x_gamma0 && c >= 2 * NC) ? (TO)-1 : (TO)1;
-            const TO vr = g * tr[c], vi = g * ti[c];
+            const CO g = (a.x_gamma0 && c >= 2 * NC) ? (CO)-1 : (CO)1;
+            const CO vr = g * tr[c], vi = g * ti[c];
             orr[c] += cr * vr - ci * vi;
             oi[c] += cr * vi + ci * vr;
         }
     }
     if (a.b1) {
         load_sp_nt<NC, MB, TO>(a.b1, 0, 0, q, m, tr, ti);
-        const TO cr = (TO)a.s1_r, ci = (TO)a.s1_i;
+        const CO cr = (CO)a.s1_r, ci = (CO)a.s1_i;
 #pragma unroll
         for (int c = 0; c < NCOMP; ++c) { orr[c] += cr * tr[c] - ci * ti[c]; oi[c] += cr * ti[c] + ci * tr[c]; }
     }
     if (a.b2) {
         load_sp_nt<NC, MB, TO>(a.b2, 0, 0, q, m, tr, ti);
-        const TO cr = (TO)a.s2_r, ci = (TO)a.s2_i;
+        const CO cr = (CO)a.s2_r, ci = (CO)a.s2_i;
 #pragma unroll
         for (int c = 0; c < NCOMP; ++c) { orr[c] += cr * tr[c] - ci * ti[c]; oi[c] += cr * ti[c] + ci * tr[c]; }
     }
     if (a.tgt) {
         load_sp_nt<NC, MB, TO>(a.tgt, 0, 0, q, m, tr, ti);
-        const TO cr = (TO)a.st_r, ci = (TO)a.st_i;
+        const CO cr = (CO)a.st_r, ci = (CO)a.st_i;
 #pragma unroll
         for (int c = 0; c < NCOMP; ++c) { orr[c] += cr * tr[c] - ci * ti[c]; oi[c] += cr * ti[c] + ci * tr[c]; }
     }
@@ -366,25 +417,53 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(WPE))) void
 // pointwise kernels
 // ---------------------------------------------------------------------------------------------
 // precision conversion between the FP64 (double2) and FP32 (float4) layouts
+// precision conversions between the field layouts, dst = scale * src.  A FP32 unit (float4) holds
+// complex pairs 2r, 2r+1 and a FP16 unit (8 halves) complex 4r..4r+3 of the FP64 layout, where
+// r = unit index / MB; the mode index is unit index % MB in all three.
 __global__ __launch_bounds__(TPB) void k_d2f(const double2 *__restrict__ src, float4 *__restrict__ dst, long npairs,
-                                             int mb)
+                                             int mb, double scale)
 {
     long i = (long)blockIdx.x * TPB + threadIdx.x;
     if (i >= npairs) return;
-    const long m = i % mb, q = i / mb;  // q = pair index (blk,site,comp/2)
+    const long m = i % mb, q = i / mb;
     const double2 a = src[(2 * q) * mb + m], b = src[(2 * q + 1) * mb + m];
-    dst[i] = make_float4((float)a.x, (float)a.y, (float)b.x, (float)b.y);
+    dst[i] = make_float4((float)(scale * a.x), (float)(scale * a.y), (float)(scale * b.x), (float)(scale * b.y));
 }
-
 __global__ __launch_bounds__(TPB) void k_f2d(const float4 *__restrict__ src, double2 *__restrict__ dst, long npairs,
-                                             int mb)
+                                             int mb, double scale)
 {
     long i = (long)blockIdx.x * TPB + threadIdx.x;
     if (i >= npairs) return;
     const long m = i % mb, q = i / mb;
     const float4 t = src[i];
-    dst[(2 * q) * mb + m] = make_double2(t.x, t.y);
-    dst[(2 * q + 1) * mb + m] = make_double2(t.z, t.w);
+    dst[(2 * q) * mb + m] = make_double2(scale * t.x, scale * t.y);
+    dst[(2 * q + 1) * mb + m] = make_double2(scale * t.z, scale * t.w);
+}
+__global__ __launch_bounds__(TPB) void k_d2h(const double2 *__restrict__ src, h8v *__restrict__ dst, long nunits,
+                                             int mb, double scale)
+{
+    long i = (long)blockIdx.x * TPB + threadIdx.x;
+    if (i >= nunits) return;
+    const long m = i % mb, q = i / mb;
+    h8v v;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const double2 a = src[(4 * q + j) * mb + m];
+        v[2 * j] = (_Float16)(float)(scale * a.x);
+        v[2 * j + 1] = (_Float16)(float)(scale * a.y);
+    }
+    dst[i] = v;
+}
+__global__ __launch_bounds__(TPB) void k_h2f(const h8v *__restrict__ src, float4 *__restrict__ dst, long nunits,
+                                             int mb, float scale)
+{
+    long i = (long)blockIdx.x * TPB + threadIdx.x;
+    if (i >= nunits) return;
+    const long m = i % mb, q = i / mb;
+    const h8v t = src[i];
+    dst[(2 * q) * mb + m] = make_float4(scale * (float)t[0], scale * (float)t[1], scale * (float)t[2], scale * (float)t[3]);
+    dst[(2 * q + 1) * mb + m] =
+        make_float4(scale * (float)t[4], scale * (float)t[5], scale * (float)t[6], scale * (float)t[7]);
 }
 
 template <typename T>
@@ -1266,21 +1345,29 @@ hipError_t launch_hop(const Geom &g, int pin, int pout, const HopArgs &a, int wp
         if (pin == P64 && pout == P64) return hop_mb<2, double, double>(g, a, wpe, st);
         if (pin == P32 && pout == P32) return hop_mb<2, float, float>(g, a, wpe, st);
         if (pin == P32 && pout == P64) return hop_mb<2, float, double>(g, a, wpe, st);
+        if (pin == P16 && pout == P16) return hop_mb<2, _Float16, _Float16>(g, a, wpe, st);
     } else if (g.nc == 3) {
         if (pin == P64 && pout == P64) return hop_mb<3, double, double>(g, a, wpe, st);
         if (pin == P32 && pout == P32) return hop_mb<3, float, float>(g, a, wpe, st);
         if (pin == P32 && pout == P64) return hop_mb<3, float, double>(g, a, wpe, st);
+        if (pin == P16 && pout == P16) return hop_mb<3, _Float16, _Float16>(g, a, wpe, st);
     }
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_convert(const Geom &g, int nblk, const void *src, int psrc, void *dst, int pdst, hipStream_t st)
+hipError_t launch_convert(const Geom &g, int nblk, const void *src, int psrc, void *dst, int pdst, hipStream_t st,
+                          double scale)
 {
     const long npairs = (long)nblk * g.V * (4 * g.nc / 2) * g.mb;
+    const long nunits = npairs / 2;
     if (psrc == P64 && pdst == P32)
-        k_d2f<<<nblocks(npairs), TPB, 0, st>>>((const double2 *)src, (float4 *)dst, npairs, g.mb);
+        k_d2f<<<nblocks(npairs), TPB, 0, st>>>((const double2 *)src, (float4 *)dst, npairs, g.mb, scale);
     else if (psrc == P32 && pdst == P64)
-        k_f2d<<<nblocks(npairs), TPB, 0, st>>>((const float4 *)src, (double2 *)dst, npairs, g.mb);
+        k_f2d<<<nblocks(npairs), TPB, 0, st>>>((const float4 *)src, (double2 *)dst, npairs, g.mb, scale);
+    else if (psrc == P64 && pdst == P16)
+        k_d2h<<<nblocks(nunits), TPB, 0, st>>>((const double2 *)src, (h8v *)dst, nunits, g.mb, scale);
+    else if (psrc == P16 && pdst == P32)
+        k_h2f<<<nblocks(nunits), TPB, 0, st>>>((const h8v *)src, (float4 *)dst, nunits, g.mb, (float)scale);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();

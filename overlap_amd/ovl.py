@@ -53,7 +53,7 @@ class EnsembleParams(C.Structure):
 class SignOpts(C.Structure):
     _fields_ = [("method", C.c_int), ("precision", C.c_int), ("degree", C.c_int), ("cut", C.c_double),
                 ("tol", C.c_double), ("head", C.c_int), ("lmin", C.c_double), ("lmax", C.c_double),
-                ("mode_chunk", C.c_int)]
+                ("mode_chunk", C.c_int), ("fp16_tail", C.c_int), ("head16", C.c_int)]
 
 
 class StepOpts(C.Structure):
@@ -64,7 +64,8 @@ class StepOpts(C.Structure):
 class Timing(C.Structure):
     _fields_ = [("total_ms", C.c_double), ("hop_ms", C.c_double), ("hop_launches", C.c_longlong),
                 ("hop_bytes", C.c_double), ("fp64_steps", C.c_int), ("fp32_steps", C.c_int),
-                ("var_ms", C.c_double * 3), ("var_launches", C.c_longlong * 3), ("var_bytes", C.c_double * 3)]
+                ("fp16_steps", C.c_int), ("var_ms", C.c_double * 4), ("var_launches", C.c_longlong * 4),
+                ("var_bytes", C.c_double * 4)]
 
 
 _lib = None

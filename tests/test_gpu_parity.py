@@ -136,9 +136,42 @@ def test_mixed_precision_sign_reaches_1e10(gpu, nc, nb):
     t = ctx.timing()
     mix = ctx.get_modes(ovl.SCRATCH)
     err = max(np.linalg.norm(mix[k] - ref[k]) / np.linalg.norm(psi[k]) for k in range(64))
-    print(f"[mixed] nc={nc} fp64 steps {t.fp64_steps} fp32 steps {t.fp32_steps} err {err:.3e}", flush=True)
-    assert t.fp32_steps > 0 and t.fp64_steps < deg + 1
+    print(f"[mixed] nc={nc} steps fp64 {t.fp64_steps} fp32 {t.fp32_steps} fp16 {t.fp16_steps} err {err:.3e}",
+          flush=True)
+    assert t.fp32_steps > 0 and t.fp16_steps > 0 and t.fp64_steps < deg + 1
     assert err <= 1e-10
+    # without the FP16 tail: FP64 head + FP32 rest
+    om.fp16_tail = 0
+    ctx.apply_sign(ovl.PSI, ovl.SCRATCH, om)
+    t = ctx.timing()
+    mix = ctx.get_modes(ovl.SCRATCH)
+    err = max(np.linalg.norm(mix[k] - ref[k]) / np.linalg.norm(psi[k]) for k in range(64))
+    assert t.fp16_steps == 0 and t.fp32_steps > 0 and err <= 1e-10
+
+
+@pytest.mark.parametrize("nc,nb", [(2, 1), (3, 3)])
+def test_fp16_tier_vs_oracle(gpu, nc, nb):
+    # nearly the whole recurrence in FP16 storage (steps k >= 9 of 48): the result carries the FP16
+    # rounding (u16 = 4.9e-4 times the size of the intermediates), the FP64 form does not
+    print(f"[case] fp16 tier nc={nc} nb={nb}", flush=True)
+    n = (4, 3, 4)
+    ctx, L, ucc, *_, psi = make(nc, n, 20, 60 + nc, M=1.2, nb=nb, mb=16)
+    psi[3] *= 20.0  # modes of different norms share the scale of the largest
+    ctx.set_modes(ovl.PSI, psi)
+    deg, cut = 48, 0.1
+    o = ovl.sign_opts(degree=deg, cut=cut, precision=ovl.PREC_MIXED, head=4, head16=9)
+    ctx.apply_sign(ovl.PSI, ovl.PSIMID, o)
+    t = ctx.timing()
+    assert (t.fp64_steps, t.fp32_steps, t.fp16_steps) == (4, 5, deg + 1 - 9)
+    out = ctx.get_modes(ovl.PSIMID)
+    coef = orc.cheb_coeffs(deg, cut)
+    lam = orc.norm_bound(L)
+    errs = [np.linalg.norm(out[k] - orc.sign_cheb(L, ucc, lam, cut, deg, coef, psi[k])) / np.linalg.norm(psi[k])
+            for k in range(20)]
+    print(f"  fp16-tier error: max {max(errs):.2e} min {min(errs):.2e}", flush=True)
+    assert max(errs) < 2e-2 and np.isfinite(out).all()
+    # the FP16 variant is timed on its own
+    assert t.var_launches[3] == 2 * (deg - 9) and t.var_launches[1] > 0  # b_n = c_n x needs no stencil
 
 
 def test_sign_residual_and_mode_chunks(gpu):
@@ -296,7 +329,8 @@ def test_traversals_are_permutations(gpu, nc, n, nm, mb):
             ctx.set_modes(ovl.PSI, psi)
             if trav == 0:
                 print("  autotune:", ctx.autotune(), flush=True)
-            ctx.apply_sign(ovl.PSI, ovl.PSIMID, ovl.sign_opts(degree=32, cut=0.1, precision=ovl.PREC_MIXED, head=8))
+            ctx.apply_sign(ovl.PSI, ovl.PSIMID,
+                           ovl.sign_opts(degree=32, cut=0.1, precision=ovl.PREC_MIXED, head=8, head16=20))
             outs.append(ctx.get_modes(ovl.PSIMID))
     for o in outs[1:]:
         assert np.array_equal(o, outs[0])
