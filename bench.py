@@ -142,6 +142,15 @@ def bench_ensemble(ec: cf.EnsembleConfig, steps: int, warmup: int, tol: float):
 
     load()
     tune = ens.autotune()
+    # per-system error of one overlap apply (-i h_ov psi), mixed against FP64
+    apply_err = []
+    ens.apply_overlap(ovl.PSI, ovl.SCRATCH, opts(ovl.PREC_FP64).sign)
+    ref = [ens.get_system_modes(k, ovl.SCRATCH) for k in range(ec.n_systems)]
+    ens.apply_overlap(ovl.PSI, ovl.SCRATCH, opts(ovl.PREC_MIXED).sign)
+    for k in range(ec.n_systems):
+        mix = ens.get_system_modes(k, ovl.SCRATCH)
+        apply_err.append(float(max(np.linalg.norm(a - b) / np.linalg.norm(b) for a, b in zip(mix, ref[k]))))
+    del ref
     ens.fermion_update(opts(ovl.PREC_MIXED), warmup)
     out = {}
     for name, prec in (("mixed", ovl.PREC_MIXED), ("fp64", ovl.PREC_FP64)):
@@ -168,9 +177,11 @@ def bench_ensemble(ec: cf.EnsembleConfig, steps: int, warmup: int, tol: float):
            "autotune": tune, "mixed": out["mixed"], "fp64": out["fp64"],
            "speedup_vs_fp64": out["mixed"]["value"] / out["fp64"]["value"],
            "per_system_rel_err_vs_fp64": [float(f"{e:.3e}") for e in errs],
+           "per_system_overlap_apply_rel_err_vs_fp64": [float(f"{e:.3e}") for e in apply_err],
            "systems_layout": [[d[0], nb] for (d, nb, *_r) in sy]}
     ens.close()
-    log(f"ensemble per-system err vs fp64: max {max(errs):.3e}")
+    log(f"ensemble per-system err vs fp64: evolved modes max {max(errs):.3e}, one overlap apply max "
+        f"{max(apply_err):.3e}")
     return res
 
 
@@ -354,7 +365,7 @@ def main():
 
     out = {"metric": METRIC, "value": value, "unit": "site*mode updates/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
-           "vs_baseline": None, "dtype": "fp32+fp64 (mixed)", "data": "synthetic",
+           "vs_baseline": None, "dtype": "fp64+fp32+fp16-storage (mixed)", "data": "synthetic",
            "config": {"workload": c.name, "nc": c.nc, "lattice": list(c.n), "modes": c.nmodes,
                       "chebyshev_degree": c.cheb_n, "cut": c.cheb_cut, "rho": c.m_dw, "flux_quanta": c.nb,
                       "dt": c.dt, "tol": args.tol, "mode_block": args.mb, "mode_chunk": args.chunk,
