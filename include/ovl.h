@@ -152,6 +152,23 @@ int ovl_ncs_dot(ovl_ctx *ctx, double *ncsdot);
  * (host arrays or NULL), axial charge = sum_x ja[4x]; first_mode = global index of mode 0 */
 int ovl_currents(ovl_ctx *ctx, int field, const ovl_sign_opts *o, int first_mode, double *jv,
                  double *ja, double *axial_charge);
+/* Dynamics::Fermions::Update x nsteps with Fermions::Observables::ComputeCurrents after every
+ * `every`-th step (the reference's driver does both, Simulation.cpp:466-480).  The currents reuse
+ * the overlap apply of the leapfrog's second half step, -i h_ov[U(t + dt/2)] Psi(t + dt): the same
+ * operator the reference's ComputeCurrents applies after Update, since it reads the stencil links
+ * of the last EvolutionOperator::Setup (OverlapUpdate.cpp:116, OverlapCurrents.cpp:134).  So no
+ * extra overlap apply is made.  Chebyshev overlap, one lattice.  Outputs per observation (host
+ * arrays of nsteps / every entries, or NULL): the axial charge sum_x ja[4x] and dN_CS/dt
+ * (ovl_ncs_dot); jv, ja (4V, or NULL) hold the per-site currents of the last observation. */
+typedef struct {
+    int every;
+    int first_mode;        /* global index of mode 0 (commutator weights) */
+    int count;             /* out: observations written */
+    double *axial_charge;
+    double *ncs_dot;
+    double *jv, *ja;
+} ovl_observe;
+int ovl_fermion_update_observed(ovl_ctx *ctx, const ovl_step_opts *o, int nsteps, ovl_observe *obs);
 /* per-mode squared norms of a field (host array of n_modes doubles) */
 int ovl_mode_norms(ovl_ctx *ctx, int field, double *norm2);
 

@@ -58,11 +58,12 @@ struct ovl_ctx {
     // reductions / small scratch
     double *partial = nullptr;
     size_t partial_len = 0;
-    int *xcd_order = nullptr;  // device XCD-box site order (TRAV_BOXES) or null
+    bool boxes_ok = false;     // the lattice splits into 8 equal boxes (TRAV_BOXES possible)
     // per input precision (P64, P32, P16) of the stencil, set by ovl_autotune:
-    int *slab_order[3] = {nullptr, nullptr, nullptr};  // site order of TRAV_MODES (y slabs) or null
+    int *order[3] = {nullptr, nullptr, nullptr};  // device site order of the traversal (null: lexicographic)
+    bool order_set[3] = {false, false, false};
     int trav[3] = {ovl::TRAV_LEX, ovl::TRAV_LEX, ovl::TRAV_LEX};  // lane traversal
-    int slabs[3] = {1, 1, 1};                   // y slabs of TRAV_MODES
+    int slabs[3] = {1, 1, 1};                   // y slabs (TRAV_MODES) or sub-slabs per box (TRAV_BOXES)
     int nt_out[3] = {0, 0, 0};                  // streaming output stores
     int wpe[3] = {1, 1, 1};                     // occupancy variant
     double step_xmax = 0.0;  // bound of the mode norms during ovl_fermion_update (FP16 scales)
@@ -284,7 +285,7 @@ HopArgs hop_args(const ovl_ctx *c, int prec)
     a.nblk = c->g.nblk;
     a.trav = c->trav[prec];
     a.nt_out = c->nt_out[prec];
-    a.order = a.trav == ovl::TRAV_BOXES ? c->xcd_order : (a.trav == ovl::TRAV_MODES ? c->slab_order[prec] : nullptr);
+    a.order = c->order[prec];
     if (c->ensemble) {
         a.trav = ovl::TRAV_LEX;
         a.order = nullptr;
@@ -296,9 +297,10 @@ HopArgs hop_args(const ovl_ctx *c, int prec)
 }
 
 // Site traversal for the XCD-aware stencil launches: the lattice is cut into 8 boxes (one per XCD,
-// px*py*pz = 8 dividing the extents); inside a box x runs fastest, then the shorter of the two
-// remaining box extents, so all three neighbour reuse distances stay short.  Empty when no split fits.
-std::vector<int> xcd_site_order(int n0, int n1, int n2)
+// px*py*pz = 8 dividing the extents); inside a box the shorter of the y/z box extents is cut into
+// `sub` slabs, each walked with x fastest, then that extent, then the other one, so the three
+// planes a site's neighbours span (b_x * slab sites each) fit the XCD's L2.  Empty when no split fits.
+std::vector<int> xcd_site_order(int n0, int n1, int n2, int sub = 1)
 {
     const int n[3] = {n0, n1, n2};
     int best[3] = {0, 0, 0};
@@ -319,19 +321,20 @@ std::vector<int> xcd_site_order(int n0, int n1, int n2)
     if (best_cost < 0) return order;
     const int b[3] = {n[0] / best[0], n[1] / best[1], n[2] / best[2]};
     const bool y_mid = b[1] <= b[2];
+    const int slow = y_mid ? b[2] : b[1], mid = y_mid ? b[1] : b[2];
+    sub = std::max(1, std::min(sub, mid));
     order.reserve((size_t)n0 * n1 * n2);
     for (int kz = 0; kz < best[2]; ++kz)
         for (int ky = 0; ky < best[1]; ++ky)
-            for (int kx = 0; kx < best[0]; ++kx) {
-                const int slow = y_mid ? b[2] : b[1], mid = y_mid ? b[1] : b[2];
-                for (int i = 0; i < slow; ++i)
-                    for (int j = 0; j < mid; ++j)
-                        for (int x = 0; x < b[0]; ++x) {
-                            const int y = y_mid ? j : i, z = y_mid ? i : j;
-                            const int gx = kx * b[0] + x, gy = ky * b[1] + y, gz = kz * b[2] + z;
-                            order.push_back(gx + n0 * (gy + n1 * gz));
-                        }
-            }
+            for (int kx = 0; kx < best[0]; ++kx)
+                for (int t = 0; t < sub; ++t)
+                    for (int i = 0; i < slow; ++i)
+                        for (int j = t * mid / sub; j < (t + 1) * mid / sub; ++j)
+                            for (int x = 0; x < b[0]; ++x) {
+                                const int y = y_mid ? j : i, z = y_mid ? i : j;
+                                const int gx = kx * b[0] + x, gy = ky * b[1] + y, gz = kz * b[2] + z;
+                                order.push_back(gx + n0 * (gy + n1 * gz));
+                            }
     return order;
 }
 
@@ -365,20 +368,24 @@ int default_slabs(const ovl_ctx *c, int prec)
 int set_traversal(ovl_ctx *c, int prec, int trav, int slabs)
 {
     if (c->ensemble) return OVL_OK;
-    if (trav == ovl::TRAV_BOXES && !c->xcd_order) trav = ovl::TRAV_LEX;
-    c->trav[prec] = trav;
-    if (trav != ovl::TRAV_MODES) return OVL_OK;
+    if (trav == ovl::TRAV_BOXES && !c->boxes_ok) trav = ovl::TRAV_LEX;
     slabs = std::max(1, std::min(slabs, c->g.n1));
-    if (slabs == c->slabs[prec] && (slabs == 1 || c->slab_order[prec])) return OVL_OK;
-    c->slabs[prec] = slabs;
-    if (slabs == 1) {
-        if (c->slab_order[prec]) hipFree(c->slab_order[prec]);
-        c->slab_order[prec] = nullptr;
-        return OVL_OK;
+    if (trav == ovl::TRAV_LEX) slabs = 1;
+    if (c->order_set[prec] && trav == c->trav[prec] && slabs == c->slabs[prec]) return OVL_OK;
+    if (c->order[prec]) {
+        HIPCHK(c, hipStreamSynchronize(c->st));
+        hipFree(c->order[prec]);
+        c->order[prec] = nullptr;
     }
-    const std::vector<int> ord = slab_site_order(c->g.n0, c->g.n1, c->g.n2, slabs);
-    RETCHK(dalloc(c, (void **)&c->slab_order[prec], sizeof(int) * ord.size()));
-    HIPCHK(c, hipMemcpy(c->slab_order[prec], ord.data(), sizeof(int) * ord.size(), hipMemcpyHostToDevice));
+    c->trav[prec] = trav;
+    c->slabs[prec] = slabs;
+    c->order_set[prec] = true;
+    std::vector<int> ord;
+    if (trav == ovl::TRAV_BOXES) ord = xcd_site_order(c->g.n0, c->g.n1, c->g.n2, slabs);
+    else if (trav == ovl::TRAV_MODES && slabs > 1) ord = slab_site_order(c->g.n0, c->g.n1, c->g.n2, slabs);
+    if (ord.empty()) return OVL_OK;
+    RETCHK(dalloc(c, (void **)&c->order[prec], sizeof(int) * ord.size()));
+    HIPCHK(c, hipMemcpy(c->order[prec], ord.data(), sizeof(int) * ord.size(), hipMemcpyHostToDevice));
     return OVL_OK;
 }
 
@@ -654,8 +661,9 @@ int clenshaw_chunk(ovl_ctx *c, const Geom &gc, const SignPlan &pl, const void *x
 enum FinalKind { FIN_SIGN = 0, FIN_OVERLAP = 1, FIN_OVERLAP_AXPY = 2 };
 
 // xmax: bound of the 2-norm of every input mode (used by the FP16 tier only)
+// hpsi (FIN_OVERLAP_AXPY only, optional): also store -i h_ov in (the term added to out times dt)
 int sign_driver(ovl_ctx *c, const ovl_sign_opts *o, const void *in64, void *out64, int kind, double dt,
-                double xmax)
+                double xmax, void *hpsi = nullptr)
 {
     RETCHK(ensure_stencil(c));
     SignPlan pl;
@@ -686,6 +694,10 @@ int sign_driver(ovl_ctx *c, const ovl_sign_opts *o, const void *in64, void *out6
             if (kind == FIN_OVERLAP_AXPY) {
                 a.tgt = y;
                 a.st_r = 1.0;
+                if (hpsi) {
+                    a.out2 = blk_ptr(hpsi, P64, b0, c);
+                    a.out2_scale = 1.0 / dt;
+                }
             }
         }
         RETCHK(hop(c, gc, pp, P64, a));
@@ -1023,13 +1035,8 @@ static int ctx_init(ovl_ctx *c)
     c->partial_len = plen;
     RETCHK(dalloc(c, (void **)&c->partial, sizeof(double) * c->partial_len));
     RETCHK(dalloc(c, (void **)&c->dout, sizeof(double) * (16 + 2 * (size_t)maxblk * mb)));
-    if (!c->ensemble && (c->prm.traversal == 0 || c->prm.traversal == 2)) {
-        const std::vector<int> ord = xcd_site_order(c->g.n0, c->g.n1, c->g.n2);
-        if (!ord.empty()) {
-            RETCHK(dalloc(c, (void **)&c->xcd_order, sizeof(int) * ord.size()));
-            HIPCHK(c, hipMemcpy(c->xcd_order, ord.data(), sizeof(int) * ord.size(), hipMemcpyHostToDevice));
-        }
-    }
+    if (!c->ensemble && (c->prm.traversal == 0 || c->prm.traversal == 2))
+        c->boxes_ok = !xcd_site_order(c->g.n0, c->g.n1, c->g.n2).empty();
     if (!c->ensemble) {
         // traversal 0 (automatic): mode split where the mode blocks divide over the XCDs, else
         // boxes; ovl_autotune may change it
@@ -1038,7 +1045,7 @@ static int ctx_init(ovl_ctx *c)
             if (c->prm.traversal == 0) t = (c->g.nblk % ovl::NUM_XCD == 0) ? ovl::TRAV_MODES : ovl::TRAV_BOXES;
             if (c->prm.traversal == 2) t = ovl::TRAV_BOXES;
             if (c->prm.traversal == 3) t = ovl::TRAV_MODES;
-            RETCHK(set_traversal(c, p, t, default_slabs(c, p)));
+            RETCHK(set_traversal(c, p, t, t == ovl::TRAV_MODES ? default_slabs(c, p) : 1));
         }
     }
     if (c->ensemble) {
@@ -1185,8 +1192,8 @@ int ovl_destroy(ovl_ctx *c)
     void *ps[] = {c->links, c->efield, c->ul64, c->ul32, c->phase, c->field[0], c->field[1], c->field[2], c->w64, c->a64,
                   c->b64, c->x32, c->w32, c->a32, c->b32, c->partial, c->dout, c->gam, c->jv, c->ja, c->staging,
                   c->ms_ps, c->ms_r, c->ms_q, c->ms_w, c->ms_chi, c->ms_coef, c->ms_active, c->rz[0], c->rz[1],
-                  c->rz[2], c->rz[3], c->rz[4], c->rz[5], c->xcd_order, c->d_sys, c->d_work, c->slab_order[0], c->slab_order[1],
-                  c->slab_order[2], c->x16, c->w16, c->a16, c->b16};
+                  c->rz[2], c->rz[3], c->rz[4], c->rz[5], c->d_sys, c->d_work, c->order[0], c->order[1],
+                  c->order[2], c->x16, c->w16, c->a16, c->b16};
     for (void *p : ps)
         if (p) hipFree(p);
     for (auto e : c->evpool) hipEventDestroy(e);
@@ -1775,7 +1782,7 @@ static int gauge_half(ovl_ctx *c, int mode, double h)
 }
 
 // target += dt * (-i h src)   (OverlapUpdate.cpp:85-124, WILSON/Update.cpp:6-45)
-static int fermion_half(ovl_ctx *c, const ovl_step_opts *o, int src, int tgt)
+static int fermion_half(ovl_ctx *c, const ovl_step_opts *o, int src, int tgt, void *hpsi = nullptr)
 {
     if (o->kind == OVL_WILSON) {
         HopArgs a = hop_args(c, P64);
@@ -1788,19 +1795,29 @@ static int fermion_half(ovl_ctx *c, const ovl_step_opts *o, int src, int tgt)
     }
     if (o->sign.method == OVL_SIGN_ZOLOTAREV)
         return zolo_overlap(c, c->g, c->field[src], c->field[tgt], o->sign.lmin, o->sign.lmax, 1, o->dt);
-    return sign_driver(c, &o->sign, c->field[src], c->field[tgt], FIN_OVERLAP_AXPY, o->dt, c->step_xmax);
+    return sign_driver(c, &o->sign, c->field[src], c->field[tgt], FIN_OVERLAP_AXPY, o->dt, c->step_xmax, hpsi);
 }
 
-static int one_step(ovl_ctx *c, const ovl_step_opts *o)
+// observe: the second half step also leaves -i h_ov[U(t + dt/2)] Psi(t + dt) in the scratch field
+static int one_step(ovl_ctx *c, const ovl_step_opts *o, bool observe = false)
 {
     RETCHK(fermion_half(c, o, OVL_PSIMID, OVL_PSI));
     RETCHK(gauge_half(c, o->gauge, 0.5 * o->dt));
-    RETCHK(fermion_half(c, o, OVL_PSI, OVL_PSIMID));
+    RETCHK(fermion_half(c, o, OVL_PSI, OVL_PSIMID, observe ? c->field[OVL_SCRATCH] : nullptr));
     RETCHK(gauge_half(c, o->gauge, 0.5 * o->dt));
     return OVL_OK;
 }
 
+static int update_impl(ovl_ctx *c, const ovl_step_opts *o, int nsteps, bool observe_last);
+
 int ovl_fermion_update(ovl_ctx *c, const ovl_step_opts *o, int nsteps)
+{
+    return update_impl(c, o, nsteps, false);
+}
+
+// nsteps leapfrog steps; with observe_last the last step runs with stream launches and leaves
+// -i h_ov Psi in the scratch field
+static int update_impl(ovl_ctx *c, const ovl_step_opts *o, int nsteps, bool observe_last)
 {
     if (!c || !o || nsteps < 0 || !std::isfinite(o->dt)) return OVL_E_ARG;
     if (o->kind != OVL_WILSON && o->kind != OVL_OVERLAP) return fail(c, OVL_E_ARG, "unknown fermion kind");
@@ -1819,6 +1836,12 @@ int ovl_fermion_update(ovl_ctx *c, const ovl_step_opts *o, int nsteps)
     }
     RETCHK(ensure_stencil(c));
     begin_timing(c);
+    if (observe_last && nsteps > 0) {
+        RETCHK(update_impl(c, o, nsteps - 1, false));
+        begin_timing(c);
+        RETCHK(one_step(c, o, true));
+        return end_timing(c);
+    }
     const bool graph = o->use_graph && !(o->kind == OVL_OVERLAP && o->sign.method == OVL_SIGN_ZOLOTAREV) && nsteps > 0;
     if (!graph) {
         for (int s = 0; s < nsteps; ++s) RETCHK(one_step(c, o));
@@ -2047,6 +2070,8 @@ static void build_gammas(std::vector<double> &out)
             }
 }
 
+static int currents_from(ovl_ctx *c, const void *psi, int first_mode, double *jv, double *ja, double *axial_charge);
+
 int ovl_currents(ovl_ctx *c, int field, const ovl_sign_opts *o, int first_mode, double *jv, double *ja,
                  double *axial_charge)
 {
@@ -2063,6 +2088,12 @@ int ovl_currents(ovl_ctx *c, int field, const ovl_sign_opts *o, int first_mode, 
         RETCHK(input_bound(c, o, c->field[field], &xmax));
         RETCHK(sign_driver(c, o, c->field[field], c->field[OVL_SCRATCH], FIN_OVERLAP, 0.0, xmax));
     }
+    return currents_from(c, c->field[field], first_mode, jv, ja, axial_charge);
+}
+
+// per-site currents from a field and -i h_ov of it in the scratch field (OverlapCurrents.cpp:137-228)
+static int currents_from(ovl_ctx *c, const void *psi, int first_mode, double *jv, double *ja, double *axial_charge)
+{
     if (!c->gam) {
         std::vector<double> g;
         build_gammas(g);
@@ -2071,8 +2102,8 @@ int ovl_currents(ovl_ctx *c, int field, const ovl_sign_opts *o, int first_mode, 
     }
     RETCHK(dalloc(c, (void **)&c->jv, sizeof(double) * 4 * c->g.V));
     RETCHK(dalloc(c, (void **)&c->ja, sizeof(double) * 4 * c->g.V));
-    HIPCHK(c, ovl::launch_currents(c->g, c->field[field], c->field[OVL_SCRATCH], c->gam, first_mode, c->partial, c->jv,
-                                   c->ja, c->st));
+    HIPCHK(c, ovl::launch_currents(c->g, psi, c->field[OVL_SCRATCH], c->gam, first_mode, c->partial, c->jv, c->ja,
+                                   c->st));
     std::vector<double> hv(4 * (size_t)c->g.V), ha(4 * (size_t)c->g.V);
     HIPCHK(c, hipMemcpyAsync(hv.data(), c->jv, sizeof(double) * hv.size(), hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipMemcpyAsync(ha.data(), c->ja, sizeof(double) * ha.size(), hipMemcpyDeviceToHost, c->st));
@@ -2083,6 +2114,36 @@ int ovl_currents(ovl_ctx *c, int field, const ovl_sign_opts *o, int first_mode, 
         double s = 0;
         for (int i = 0; i < c->g.V; ++i) s += ha[4 * i];
         *axial_charge = s;
+    }
+    return OVL_OK;
+}
+
+// Dynamics::Fermions::Update with Fermions::Observables::ComputeCurrents every `every` steps
+// (Simulation.cpp:466-480), the currents taken from the overlap apply the leapfrog already does:
+// the second half step applies -i h_ov with the links of the middle of the step to Psi(t + dt),
+// which is what the reference's ComputeCurrents uses after Update (it reads EvolutionOperator::Ucc,
+// last set up before that half step, OverlapUpdate.cpp:116 and OverlapCurrents.cpp:134).
+int ovl_fermion_update_observed(ovl_ctx *c, const ovl_step_opts *o, int nsteps, ovl_observe *obs)
+{
+    if (!c || !o || !obs || nsteps < 0) return OVL_E_ARG;
+    RETCHK(single_lattice(c, "ovl_fermion_update_observed"));
+    obs->count = 0;
+    if (obs->every <= 0) return ovl_fermion_update(c, o, nsteps);
+    if (o->kind != OVL_OVERLAP || o->sign.method != OVL_SIGN_CHEBYSHEV)
+        return fail(c, OVL_E_ARG, "observed updates take the currents from the Chebyshev overlap apply");
+    int done = 0;
+    while (done < nsteps) {
+        const int chunk = std::min(obs->every, nsteps - done);
+        const bool observe = chunk == obs->every;
+        RETCHK(update_impl(c, o, chunk, observe));
+        done += chunk;
+        if (!observe) break;
+        double q5 = 0, ncs = 0;
+        RETCHK(currents_from(c, c->field[OVL_PSI], obs->first_mode, obs->jv, obs->ja, &q5));
+        RETCHK(obs_ncs_dot(c, 0, &ncs));
+        if (obs->axial_charge) obs->axial_charge[obs->count] = q5;
+        if (obs->ncs_dot) obs->ncs_dot[obs->count] = ncs;
+        obs->count++;
     }
     return OVL_OK;
 }
@@ -2214,7 +2275,8 @@ int ovl_autotune(ovl_ctx *c, char *report, int len)
     static const char *pname[3] = {"fp64", "fp32", "fp16"};
     auto trav_name = [](const ovl_ctx *cc, int prec) {
         if (cc->trav[prec] == ovl::TRAV_MODES) return std::string("modes/") + std::to_string(cc->slabs[prec]);
-        return std::string(cc->trav[prec] == ovl::TRAV_BOXES ? "boxes" : "lex");
+        if (cc->trav[prec] == ovl::TRAV_BOXES) return std::string("boxes/") + std::to_string(cc->slabs[prec]);
+        return std::string("lex");
     };
     for (int prec = P64; prec <= P16; ++prec) {
         // candidate traversals (trav, slabs)
@@ -2222,7 +2284,9 @@ int ovl_autotune(ovl_ctx *c, char *report, int len)
         if (c->ensemble || c->prm.traversal != 0) {
             cand.push_back({c->trav[prec], c->slabs[prec]});
         } else {
-            if (c->xcd_order) cand.push_back({ovl::TRAV_BOXES, 1});
+            if (c->boxes_ok)
+                for (int t : {1, 2, 3})
+                    cand.push_back({ovl::TRAV_BOXES, t});
             for (int t : {1, 2, 3, 4, 6})
                 if (t <= c->g.n1) cand.push_back({ovl::TRAV_MODES, t});
         }

@@ -50,6 +50,8 @@ struct HopArgs {
     double amass;     // A = 6r + 2m - 2M of h_W = (1/2) gamma0 [A - hopping]
     int x_gamma0;     // 1: G = gamma0
     int nt_out;       // 1: non-temporal (streaming) stores of `out`
+    void *out2;        // optional second output: out2_scale * (sh*h_W(in) + sx*G(x)), before b1/b2/tgt
+    double out2_scale;
     int nblk;         // blocks covered by this launch
     int trav;         // lane traversal, TRAV_* (falls back to TRAV_LEX where the launch does not split evenly)
     const int *order; // TRAV_BOXES: XCD k's box is order[k V/8 .. (k+1) V/8); TRAV_MODES: site order

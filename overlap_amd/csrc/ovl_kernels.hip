@@ -350,6 +350,13 @@ __device__ __forceinline__ void hop_site(const HopArgs &a, int n0, int n1, int n
             oi[c] += cr * vi + ci * vr;
         }
     }
+    if (a.out2) {  // sh*h_W(in) + sx*G(x) alone, times out2_scale (the overlap term of a leapfrog axpy)
+        CO ur[NCOMP], ui[NCOMP];
+        const CO f = (CO)a.out2_scale;
+#pragma unroll
+        for (int c = 0; c < NCOMP; ++c) { ur[c] = f * orr[c]; ui[c] = f * oi[c]; }
+        store_sp<NC, MB, TO>(a.out2, 0, 0, q, m, ur, ui);
+    }
     if (a.b1) {
         load_sp_nt<NC, MB, TO>(a.b1, 0, 0, q, m, tr, ti);
         const CO cr = (CO)a.s1_r, ci = (CO)a.s1_i;

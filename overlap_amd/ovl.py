@@ -28,7 +28,7 @@ SYMBOLS = [
     "ovl_wilson_evo_compute", "ovl_overlap_operator_compute", "ovl_zolotarev_coefficients",
     "ovl_update_links", "ovl_update_efield", "ovl_fermion_update",
     "ovl_energy", "ovl_gauss_violation", "ovl_unitarity_violation", "ovl_ncs_dot", "ovl_currents",
-    "ovl_mode_norms",
+    "ovl_fermion_update_observed", "ovl_mode_norms",
     "ovl_create_ensemble", "ovl_set_system_links", "ovl_get_system_links", "ovl_set_system_efield",
     "ovl_get_system_efield", "ovl_set_system_u1_flux", "ovl_set_system_modes", "ovl_get_system_modes",
     "ovl_system_observables", "ovl_ensemble_balance", "ovl_gauss_restore",
@@ -59,6 +59,12 @@ class SignOpts(C.Structure):
 class StepOpts(C.Structure):
     _fields_ = [("kind", C.c_int), ("dt", C.c_double), ("gauge", C.c_int), ("sign", SignOpts),
                 ("use_graph", C.c_int)]
+
+
+class Observe(C.Structure):
+    _fields_ = [("every", C.c_int), ("first_mode", C.c_int), ("count", C.c_int),
+                ("axial_charge", C.POINTER(C.c_double)), ("ncs_dot", C.POINTER(C.c_double)),
+                ("jv", C.POINTER(C.c_double)), ("ja", C.POINTER(C.c_double))]
 
 
 class Timing(C.Structure):
@@ -243,6 +249,15 @@ class Context:
 
     def fermion_update(self, opts: StepOpts, nsteps):
         return self._chk(lib().ovl_fermion_update(self.h, C.byref(opts), nsteps), allow=(E_CONVERGE,))
+
+    def fermion_update_observed(self, opts: StepOpts, nsteps, every, first_mode=0):
+        """Returns (axial charge history, dN_CS/dt history, jv, ja of the last observation)."""
+        n = nsteps // every if every > 0 else 0
+        q5, ncs = np.zeros(max(n, 1)), np.zeros(max(n, 1))
+        jv, ja = np.zeros(4 * self.V), np.zeros(4 * self.V)
+        ob = Observe(every, first_mode, 0, _d(q5), _d(ncs), _d(jv), _d(ja))
+        self._chk(lib().ovl_fermion_update_observed(self.h, C.byref(opts), nsteps, C.byref(ob)), allow=(E_CONVERGE,))
+        return q5[:ob.count], ncs[:ob.count], jv, ja
 
     # observables
     def _pair(self, fn):

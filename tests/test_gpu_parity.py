@@ -357,3 +357,39 @@ def test_gauss_restore_vs_oracle(gpu, nc):
     assert vmax <= 1e-11 and it < 5000
     assert np.array_equal(ctx.get_links(), links)
     assert ctx.gauss_violation()[0] == vmax
+
+
+def test_observed_update_currents(gpu):
+    # currents every 2 steps from the leapfrog's own overlap apply: the run is bit-identical to a
+    # plain one, and the currents are those of -i h_ov[U(t + dt/2)] Psi(t + dt) (the operator the
+    # reference's ComputeCurrents applies after Update)
+    print("[case] observed update", flush=True)
+    n, nc, nb, deg, cut, dt = (4, 4, 4), 3, 2, 64, 0.1, 0.02
+    so = ovl.StepOpts(kind=ovl.OVERLAP, dt=dt, gauge=ovl.GAUGE_LINKS,
+                      sign=ovl.sign_opts(degree=deg, cut=cut, precision=ovl.PREC_FP64), use_graph=1)
+    runs = []
+    for observed in (False, True):
+        ctx, L, ucc, links, e, psi = make(nc, n, 16, 700, sigma=0.3, M=1.2, nb=nb, mb=16)
+        ctx.set_modes(ovl.PSIMID, psi)
+        if observed:
+            q5, ncs, jv, ja = ctx.fermion_update_observed(so, 5, every=2, first_mode=3)
+        else:
+            ctx.fermion_update(so, 5)
+        runs.append((ctx, ctx.get_modes(ovl.PSI), ctx.get_modes(ovl.PSIMID), ctx.get_links()))
+    for a, b in zip(runs[0][1:], runs[1][1:]):
+        assert np.array_equal(a, b)
+    assert len(q5) == 2 and len(ncs) == 2
+    # the last observation came after step 4: undo step 5 on the CPU for the state of step 4
+    ctx4, L, ucc, links, e, psi = make(nc, n, 16, 700, sigma=0.3, M=1.2, nb=nb, mb=16)
+    ctx4.set_modes(ovl.PSIMID, psi)
+    ctx4.fermion_update(so, 4)
+    psi4, links4 = ctx4.get_modes(ovl.PSI), ctx4.get_links()
+    half = orc.update_links(L, links4, e, -0.5 * dt)  # U(t + dt/2) = exp(+i dt/2 E) U(t + dt)
+    uh = orc.links_to_ucc(L, half, orc.u1_phases(n, nb, 1.0))
+    coef = orc.cheb_coeffs(deg, cut)
+    h = np.stack([orc.overlap_evo_cheb(L, uh, orc.norm_bound(L), cut, deg, coef, p) for p in psi4])
+    jv0, ja0 = orc.currents(L, psi4, h, first_mode=3)
+    assert abs(q5[-1] - ja0[0::4].sum()) < 1e-12
+    assert abs(ncs[-1] - ctx4.ncs_dot()) <= 1e-15 * max(1.0, abs(ncs[-1]))
+    # jv/ja returned by the call are those of the last observation (after step 4)
+    assert np.abs(jv - jv0).max() < 1e-13 and np.abs(ja - ja0).max() < 1e-13
