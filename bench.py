@@ -313,6 +313,16 @@ def main():
         graph_ms = 1e3 * (time.perf_counter() - t0) / args.steps
         log(f"mixed (hipGraph): {graph_ms:.1f} ms/step")
 
+    # --- Chern-Simons rate and axial-charge history (currents from the leapfrog's own overlap apply)
+    observables = None
+    if c.kind == "overlap" and not args.quick:
+        so_obs = step_opts(c, prec_mixed, True, args.tol, args.chunk)
+        t0 = time.perf_counter()
+        q5, ncs, _, _ = ctx.fermion_update_observed(so_obs, 10, every=5)
+        observables = {"steps": 10, "every": 5, "axial_charge": [float(v) for v in q5],
+                       "ncs_dot": [float(v) for v in ncs], "seconds": round(time.perf_counter() - t0, 3)}
+        log(f"observed run: axial charge {observables['axial_charge']}, dN_CS/dt {observables['ncs_dot']}")
+
     # --- plain FP64 form
     fp64 = None
     if not args.no_fp64:
@@ -372,7 +382,7 @@ def main():
                       "traversal": args.traversal, "parallelism": "single GPU"},
            "roofline": roofline, "cpu_baseline": cpu, "fp64_gpu": fp64,
            "speedup_vs_fp64": (value / fp64["value"]) if fp64 else None,
-           "graph_ms_per_step": graph_ms, "accuracy": res, "ensemble": ens}
+           "graph_ms_per_step": graph_ms, "accuracy": res, "observables": observables, "ensemble": ens}
     print(json.dumps(out))
 
 
