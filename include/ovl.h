@@ -218,6 +218,19 @@ int ovl_ensemble_balance(ovl_ctx *ctx, double *imbalance);
  * max_iter iterations (the reference: 10000); iters and vmax (violation of the final field) may be NULL */
 int ovl_gauss_restore(ovl_ctx *ctx, int sys, double tol, int max_iter, int *iters, double *vmax);
 
+/* OverlapQEDEigenfunctions::Setup + InitialConditions::SetOverlapQEDEigenfunctions
+ * (INITIALCONDITIONS/FERMIONS/BASIS/OverlapQEDEigenfunctions.cpp:141-591,
+ * SetOverlapQEDEigenfunctions.cpp:5-56): the overlap Hamiltonian of the U(1) background alone (flux
+ * nb, charge q; colour 0 carries the phase and the other colours have no hopping, as the reference's
+ * ConvertU1GaugeLinks sets them) is assembled column by column from batched overlap applies and
+ * diagonalised on the device (rocSOLVER zheevd).  Mode s of Psi and PsiMid (global index
+ * g = first_mode + s: particle g % 2, level g / 2) becomes sqrt(V) times eigenvector D/2 - 1 - level
+ * (particle) or D/2 + level (antiparticle) of the descending spectrum, PsiMid = Psi exp(-i omega dt/2).
+ * D = 4 Nc V <= 16384.  Overwrites the scratch field; the SU(Nc) links are untouched.  eigenvalues
+ * (D doubles, descending) and hermiticity (max |H - H^+| of the assembled matrix) may be NULL. */
+int ovl_qed_eigenbasis(ovl_ctx *ctx, const ovl_sign_opts *o, int nb, double q, double dt, int first_mode,
+                       double *eigenvalues, double *hermiticity);
+
 /* ---- configuration files ----
  * IO::SaveConfiguration / IO::SaveGaugeLinks (IO/SaveConfiguration.cpp:3-63) and
  * IO::LoadConfiguration (IO/LoadConfiguration.cpp:3-144) for system `sys` (0 for an ordinary

@@ -14,7 +14,7 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
 
 
 def build(verbose: bool = False) -> str:
-    cmd = ["hipcc", *FLAGS, *SOURCES, "-o", LIB + ".tmp"]
+    cmd = ["hipcc", *FLAGS, *SOURCES, "-o", LIB + ".tmp", "-lrocsolver", "-lrocblas"]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

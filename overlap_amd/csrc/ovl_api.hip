@@ -5,6 +5,8 @@
 #include "ovl_internal.hpp"
 
 #include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
 
 #include <algorithm>
 #include <cmath>
@@ -46,6 +48,7 @@ struct ovl_ctx {
     int nb = 0;
     double q = 1.0;
     bool links_set = false, efield_set = false, stencil_dirty = true;
+    bool stencil_custom = false;  // the stencil links hold a special operator (QED eigenbasis)
 
     void *field[3] = {nullptr, nullptr, nullptr};  // Psi, PsiMid, scratch (FP64)
 
@@ -210,6 +213,7 @@ int check_field(ovl_ctx *c, int f)
 
 int ensure_stencil(ovl_ctx *c)
 {
+    if (c->stencil_custom) return OVL_OK;  // stencil links set directly for a special operator
     if (!c->links_set) return fail(c, OVL_E_STATE, "no gauge links uploaded (ovl_set_links)");
     if (c->stencil_dirty) {
         for (int s = 0; s < nsys(c); ++s)
@@ -2234,6 +2238,142 @@ int ovl_load_configuration(ovl_ctx *c, int sys, const char *ufile, const char *e
     if (efile) RETCHK(read_link_file(c, d, efile, n_adj(c), e));
     RETCHK(ovl_set_system_links(c, sys, u.data()));
     if (efile) RETCHK(ovl_set_system_efield(c, sys, e.data()));
+    return OVL_OK;
+}
+
+// ---- initial eigenbasis in a constant magnetic field ----
+// OverlapQEDEigenfunctions::Setup (INITIALCONDITIONS/FERMIONS/BASIS/OverlapQEDEigenfunctions.cpp:141-591)
+// and InitialConditions::SetOverlapQEDEigenfunctions (SetOverlapQEDEigenfunctions.cpp:5-56).  The
+// reference applies the overlap operator to the columns of a random orthonormal basis Q and forms
+// H = i (A Q) Q^+; the unit basis gives the same matrix without the O(D^3) product.
+int ovl_qed_eigenbasis(ovl_ctx *c, const ovl_sign_opts *o, int nb, double q, double dt, int first_mode,
+                       double *eigenvalues, double *hermiticity)
+{
+    if (!c || !o || !std::isfinite(q) || !std::isfinite(dt) || first_mode < 0) return OVL_E_ARG;
+    RETCHK(single_lattice(c, "ovl_qed_eigenbasis"));
+    RETCHK(check_sign_opts(c, o));
+    const int nc = c->g.nc, V = c->g.V, n0 = c->g.n0, n1 = c->g.n1;
+    const long D = 4L * nc * V;
+    if (D > 16384) return fail(c, OVL_E_ARG, "4 Nc V above 16384: the dense eigenproblem is not built for it");
+    const int nm = c->prm.n_modes;
+    // every requested mode must name an eigenvector
+    for (int s = 0; s < nm; ++s) {
+        const long g = (long)first_mode + s, level = (g / 2) % (2L * V * nc);
+        if (level >= D / 2) return fail(c, OVL_E_ARG, "mode index beyond the D/2 levels of the eigenbasis");
+    }
+    // U(1)-only stencil links (ConvertU1GaugeLinks, OverlapQEDEigenfunctions.cpp:20-48): colour (0,0)
+    // carries the phase of QED::SetConstantMagneticField (QED.cpp:136-143), every other entry is 0
+    std::vector<double> ucc((size_t)3 * V * nc * nc * 2, 0.0);
+    for (int st = 0; st < V; ++st) {
+        const int z = st / (n0 * n1), y = (st - z * n0 * n1) / n0, x = st - n0 * (y + n1 * z);
+        (void)z;
+        const double ay = -2.0 * M_PI * (double)(nb * x) / (double)(n0 * n1);
+        const double ax = (x == n0 - 1) ? 2.0 * M_PI * (double)(nb * y * n0) / (double)(n0 * n1) : 0.0;
+        const double ph[3] = {q * ax, q * ay, 0.0};
+        for (int mu = 0; mu < 3; ++mu) {
+            const size_t e = ((size_t)(mu * V + st) * nc + 0) * nc + 0;
+            ucc[2 * e] = std::cos(ph[mu]);
+            ucc[2 * e + 1] = std::sin(ph[mu]);
+        }
+    }
+    RETCHK(stage(c, ucc.size() * sizeof(double)));
+    HIPCHK(c, hipMemcpyAsync(c->staging, ucc.data(), ucc.size() * sizeof(double), hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, ovl::launch_ucc_to_links(c->g, c->staging, c->ul64, c->ul32, c->st));
+    c->stencil_custom = true;
+    c->stencil_dirty = true;  // afterwards the stencil is re-derived from the master links
+    struct Restore {
+        ovl_ctx *c;
+        ~Restore() { c->stencil_custom = false; }
+    } restore{c};
+
+    // H = i (-i h_ov) column by column: unit vectors in batches of the context's modes
+    std::vector<cplx> H((size_t)D * D);
+    const size_t per_mode = (size_t)V * 4 * nc;  // complex entries of one mode
+    std::vector<cplx> batch((size_t)nm * per_mode);
+    // flat index m = site + V (spin + 4 colour) <-> AoS (site Nc + colour) 4 + spin
+    auto aos_index = [&](long m) {
+        const long site = m % V, r = m / V, spin = r % 4, colour = r / 4;
+        return (size_t)((site * nc + colour) * 4 + spin);
+    };
+    for (long k0 = 0; k0 < D; k0 += nm) {
+        const int cnt = (int)std::min<long>(nm, D - k0);
+        std::fill(batch.begin(), batch.end(), cplx(0, 0));
+        for (int j = 0; j < cnt; ++j) batch[(size_t)j * per_mode + aos_index(k0 + j)] = 1.0;
+        RETCHK(modes_io(c, 0, OVL_SCRATCH, reinterpret_cast<double *>(batch.data()), 0, nm, true));
+        if (o->method == OVL_SIGN_ZOLOTAREV)
+            RETCHK(zolo_overlap(c, c->g, c->field[OVL_SCRATCH], c->field[OVL_PSIMID], o->lmin, o->lmax, 0, 0.0));
+        else
+            RETCHK(sign_driver(c, o, c->field[OVL_SCRATCH], c->field[OVL_PSIMID], FIN_OVERLAP, 0.0, 1.0));
+        RETCHK(modes_io(c, 0, OVL_PSIMID, reinterpret_cast<double *>(batch.data()), 0, nm, false));
+        for (int j = 0; j < cnt; ++j)
+            for (long m = 0; m < D; ++m) H[(size_t)(k0 + j) * D + m] = cplx(0, 1) * batch[(size_t)j * per_mode + aos_index(m)];
+    }
+    // Hermiticity (the reference prints entries with |H - H^+|^2 > 1e-12), then the Hermitian part
+    double asym = 0;
+    for (long i = 0; i < D; ++i)
+        for (long j = 0; j <= i; ++j) {
+            cplx &a = H[(size_t)j * D + i], &b = H[(size_t)i * D + j];
+            asym = std::max(asym, std::abs(a - std::conj(b)));
+            const cplx h = 0.5 * (a + std::conj(b));
+            a = h;
+            b = std::conj(h);
+        }
+    if (hermiticity) *hermiticity = asym;
+
+    // diagonalisation on the device (the reference: LAPACK zheev, MISC/LapackDiagonalization.cpp:68-120)
+    void *dA = nullptr, *dW = nullptr, *dE = nullptr, *dInfo = nullptr;
+    auto release = [&]() {
+        hipStreamSynchronize(c->st);
+        dfree(dA); dfree(dW); dfree(dE); dfree(dInfo);
+    };
+    int r = dalloc(c, &dA, sizeof(cplx) * (size_t)D * D);
+    if (r == OVL_OK) r = dalloc(c, &dW, sizeof(double) * D);
+    if (r == OVL_OK) r = dalloc(c, &dE, sizeof(double) * D);
+    if (r == OVL_OK) r = dalloc(c, &dInfo, sizeof(int));
+    if (r != OVL_OK) { release(); return r; }
+    std::vector<double> W(D);
+    int info = -1;
+    rocblas_handle handle = nullptr;
+    if (rocblas_create_handle(&handle) != rocblas_status_success) { release(); return fail(c, OVL_E_HIP, "rocblas_create_handle failed"); }
+    rocblas_set_stream(handle, c->st);
+    hipError_t he = hipMemcpyAsync(dA, H.data(), sizeof(cplx) * (size_t)D * D, hipMemcpyHostToDevice, c->st);
+    rocblas_status rs = rocblas_status_success;
+    if (he == hipSuccess)
+        rs = rocsolver_zheevd(handle, rocblas_evect_original, rocblas_fill_lower, (rocblas_int)D,
+                              (rocblas_double_complex *)dA, (rocblas_int)D, (double *)dW, (double *)dE,
+                              (rocblas_int *)dInfo);
+    if (he == hipSuccess && rs == rocblas_status_success) he = hipMemcpyAsync(W.data(), dW, sizeof(double) * D, hipMemcpyDeviceToHost, c->st);
+    if (he == hipSuccess && rs == rocblas_status_success)
+        he = hipMemcpyAsync(H.data(), dA, sizeof(cplx) * (size_t)D * D, hipMemcpyDeviceToHost, c->st);
+    if (he == hipSuccess && rs == rocblas_status_success) he = hipMemcpyAsync(&info, dInfo, sizeof(int), hipMemcpyDeviceToHost, c->st);
+    if (he == hipSuccess) he = hipStreamSynchronize(c->st);
+    rocblas_destroy_handle(handle);
+    release();
+    if (he != hipSuccess) return fail(c, OVL_E_HIP, std::string("eigensolver transfer: ") + hipGetErrorString(he));
+    if (rs != rocblas_status_success) return fail(c, OVL_E_HIP, "rocsolver_zheevd failed");
+    if (info != 0) return fail(c, OVL_E_CONVERGE, "rocsolver_zheevd did not converge");
+    // the reference stores the spectrum in descending order (LapackDiagonalization.cpp:103-112)
+    if (eigenvalues)
+        for (long b = 0; b < D; ++b) eigenvalues[b] = W[D - 1 - b];
+
+    // Psi(s) = sqrt(V) v_b, PsiMid(s) = Psi(s) exp(-i omega dt / 2), b from (particle, level) of the
+    // global mode index (FermionField.cpp:63-74, SetOverlapQEDEigenfunctions.cpp:14-44)
+    std::vector<cplx> psi((size_t)nm * per_mode), mid((size_t)nm * per_mode);
+    const double norm = std::sqrt((double)V);
+    for (int s = 0; s < nm; ++s) {
+        const long g = (long)first_mode + s, particle = g % 2, level = (g / 2) % (2L * V * nc);
+        const long b = particle == 0 ? D / 2 - 1 - level : D / 2 + level;
+        const long a = D - 1 - b;  // ascending index
+        const double w = W[a];
+        const cplx rot = std::exp(cplx(0, -0.5 * w * dt));
+        for (long m = 0; m < D; ++m) {
+            const cplx v = norm * H[(size_t)a * D + m];
+            psi[(size_t)s * per_mode + aos_index(m)] = v;
+            mid[(size_t)s * per_mode + aos_index(m)] = v * rot;
+        }
+    }
+    RETCHK(modes_io(c, 0, OVL_PSI, reinterpret_cast<double *>(psi.data()), 0, nm, true));
+    RETCHK(modes_io(c, 0, OVL_PSIMID, reinterpret_cast<double *>(mid.data()), 0, nm, true));
     return OVL_OK;
 }
 

@@ -31,7 +31,7 @@ SYMBOLS = [
     "ovl_fermion_update_observed", "ovl_mode_norms",
     "ovl_create_ensemble", "ovl_set_system_links", "ovl_get_system_links", "ovl_set_system_efield",
     "ovl_get_system_efield", "ovl_set_system_u1_flux", "ovl_set_system_modes", "ovl_get_system_modes",
-    "ovl_system_observables", "ovl_ensemble_balance", "ovl_gauss_restore",
+    "ovl_system_observables", "ovl_ensemble_balance", "ovl_gauss_restore", "ovl_qed_eigenbasis",
     "ovl_save_configuration", "ovl_load_configuration",
     "ovl_get_timing", "ovl_set_profiling", "ovl_autotune",
 ]
@@ -339,6 +339,16 @@ class Context:
         self._chk(lib().ovl_get_system_modes(self.h, s, field, out.ctypes.data_as(C.POINTER(C.c_double)), first,
                                              count))
         return out
+
+    def qed_eigenbasis(self, opts: SignOpts, nb, q=1.0, dt=0.02, first_mode=0):
+        """Sets Psi/PsiMid from the overlap eigenbasis of the U(1) background; returns (eigenvalues
+        in descending order, max |H - H^+| of the assembled matrix)."""
+        D = 4 * self.nc * self.V
+        ev = np.zeros(D)
+        herm = C.c_double()
+        self._chk(lib().ovl_qed_eigenbasis(self.h, C.byref(opts), nb, C.c_double(q), C.c_double(dt), first_mode,
+                                           _d(ev), C.byref(herm)), allow=(E_CONVERGE,))
+        return ev, herm.value
 
     def gauss_restore(self, tol=1e-12, max_iter=10000, s=0):
         """Returns (iterations, max |G^a| of the final field)."""

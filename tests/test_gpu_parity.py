@@ -393,3 +393,48 @@ def test_observed_update_currents(gpu):
     assert abs(ncs[-1] - ctx4.ncs_dot()) <= 1e-15 * max(1.0, abs(ncs[-1]))
     # jv/ja returned by the call are those of the last observation (after step 4)
     assert np.abs(jv - jv0).max() < 1e-13 and np.abs(ja - ja0).max() < 1e-13
+
+
+@pytest.mark.parametrize("nc,n,nb", [(2, (4, 4, 3), 2), (3, (3, 3, 4), 1)])
+def test_qed_eigenbasis_vs_dense_oracle(gpu, nc, n, nb):
+    # the overlap Hamiltonian of the U(1) background alone (colour 0 carries the phase), assembled
+    # on the CPU from the oracle's overlap apply and diagonalised by numpy, against the GPU's
+    print(f"[case] qed eigenbasis nc={nc} n={n} nb={nb}", flush=True)
+    V = int(np.prod(n))
+    D = 4 * nc * V
+    deg, cut, M, dt = 64, 0.1, 1.2, 0.02
+    nm = 16
+    ctx = ovl.Context(nc, n, nm, m_dw=M, mode_block=16)
+    ev, herm = ctx.qed_eigenbasis(ovl.sign_opts(degree=deg, cut=cut, precision=ovl.PREC_FP64), nb, 1.0, dt,
+                                  first_mode=2)
+    L = orc.lat(nc, n, 1.0, 0.0, M)
+    ph = orc.u1_phases(n, nb, 1.0)  # complex [mu][site]
+    ucc = np.zeros((3, V, nc, nc), complex)
+    ucc[:, :, 0, 0] = np.asarray(ph).reshape(3, V)
+    coef = orc.cheb_coeffs(deg, cut)
+    lam = orc.norm_bound(L)
+    H = np.empty((D, D), complex)
+    for k in range(D):
+        e = np.zeros(D, complex)
+        e[k] = 1.0
+        H[:, k] = 1j * orc.overlap_evo_cheb(L, ucc.reshape(-1), lam, cut, deg, coef, e)
+    assert np.abs(H - H.conj().T).max() < 1e-12 and herm < 1e-12
+    w, vecs = np.linalg.eigh(H)
+    assert np.abs(ev - w[::-1]).max() < 1e-11
+    # modes: eigenvectors of H with norm^2 V; PsiMid = Psi exp(-i omega dt / 2)
+    psi, mid = ctx.get_modes(ovl.PSI), ctx.get_modes(ovl.PSIMID)
+    for s in range(nm):
+        g = 2 + s
+        particle, level = g % 2, g // 2
+        b = D // 2 - 1 - level if particle == 0 else D // 2 + level
+        omega = ev[b]
+        v = psi[s]
+        assert abs(np.vdot(v, v).real - V) < 1e-9 * V
+        assert np.linalg.norm(H @ v - omega * v) < 1e-9 * np.sqrt(V)
+        assert np.allclose(mid[s], v * np.exp(-0.5j * omega * dt), rtol=0, atol=1e-14)
+    # the SU(Nc) stencil is back afterwards
+    ctx2, L2, ucc2, links, e2, psi2 = make(nc, n, nm, 5, M=M, mb=16)
+    ctx2.qed_eigenbasis(ovl.sign_opts(degree=16, cut=0.2, precision=ovl.PREC_FP64), nb, 1.0, dt)
+    ctx2.set_modes(ovl.PSI, psi2)
+    ctx2.apply_wilson(ovl.PSI, ovl.PSIMID)
+    assert rel(ctx2.get_modes(ovl.PSIMID)[0], orc.wilson_evo(L2, ucc2, psi2[0])) < 1e-13
