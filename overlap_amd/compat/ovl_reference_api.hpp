@@ -13,6 +13,9 @@
 //   Dynamics::Fermions::Update(U, E, Psi, PsiMid)                OverlapUpdate.cpp:175 / WILSON/Update.cpp:48
 //   ChernSimonsNumber::NCsDot(U, E)                              ChernSimonsDerivative.cpp:31
 //   Fermions::Observables::ComputeCurrents(U, Psi, jV, jA)       OverlapCurrents.cpp:292
+//   GaussLawRestoration::Restore(U, E)                            GaussRestore.cpp:129
+//   IO::SaveConfiguration / IO::LoadConfiguration(U, E, files)    SaveConfiguration.cpp:3, LoadConfiguration.cpp:3
+//   InitialConditions::SetOverlapQEDEigenfunctions(Psi, PsiMid)  SetOverlapQEDEigenfunctions.cpp:58
 //
 // Error behaviour: the reference exits or prints on bad input / non-convergence; here every
 // failure throws ovl_ref::Error carrying the C-ABI status code and message (non-convergence of
@@ -51,6 +54,9 @@ struct Settings {
     ovl_sign_opts sign{};                                  // default: reference Zolotarev + Ritz
     int mode_block = 0;
     bool resident = false;
+    int bflux = 1;                                         // BFlux (Simulation.cpp:196), QED flux quanta
+    double qcharge = 1.0;                                  // Qs (Simulation.cpp:197)
+    double gauss_tol = 1e-12;                              // 10^(1 - MAX_DIGITS_PRECISION), GaussRestore.cpp:145
     Settings()
     {
         ovl_sign_defaults(&sign);
@@ -264,5 +270,57 @@ template <class GL, class FF> double ComputeCurrents(GL *U, FF *Psi, double *jV,
 }
 }  // namespace Observables
 }  // namespace Fermions
+
+namespace GaussLawRestoration {
+// GaussRestore.cpp:129-160: relaxation of E until max |G^a| <= Settings::gauss_tol (10000 iterations
+// at most); returns the iteration count
+template <class GL, class EF> int Restore(GL *U, EF *E)
+{
+    ovl_ctx *c = context(U->N, 1);
+    check(ovl_set_links(c, reinterpret_cast<const double *>(U->U)));
+    check(ovl_set_efield(c, reinterpret_cast<const double *>(E->E)));
+    int it = 0;
+    double vmax = 0;
+    check(ovl_gauss_restore(c, 0, settings().gauss_tol, 10000, &it, &vmax));
+    check(ovl_get_efield(c, reinterpret_cast<double *>(E->E)));
+    return it;
+}
+}  // namespace GaussLawRestoration
+
+namespace IO {
+// SaveConfiguration.cpp:3-63 (E file optional: SaveGaugeLinks) and LoadConfiguration.cpp:3-144;
+// the reference composes the file names from its output directory and seed, here they are given
+template <class GL, class EF> void SaveConfiguration(GL *U, EF *E, const std::string &ufile, const std::string &efile)
+{
+    ovl_ctx *c = context(U->N, 1);
+    check(ovl_set_links(c, reinterpret_cast<const double *>(U->U)));
+    if (E) check(ovl_set_efield(c, reinterpret_cast<const double *>(E->E)));
+    check(ovl_save_configuration(c, 0, ufile.c_str(), E ? efile.c_str() : nullptr));
+}
+template <class GL, class EF> void LoadConfiguration(GL *U, EF *E, const std::string &ufile, const std::string &efile)
+{
+    ovl_ctx *c = context(U->N, 1);
+    check(ovl_load_configuration(c, 0, ufile.c_str(), E ? efile.c_str() : nullptr));
+    check(ovl_get_links(c, reinterpret_cast<double *>(U->U)));
+    if (E) check(ovl_get_efield(c, reinterpret_cast<double *>(E->E)));
+    state().bound = false;
+}
+}  // namespace IO
+
+namespace InitialConditions {
+// SetOverlapQEDEigenfunctions.cpp:58-86: eigenbasis of the overlap Hamiltonian in the U(1)
+// background (Settings::bflux, Settings::qcharge), Psi = sqrt(V) v, PsiMid = Psi exp(-i omega dTau/2)
+template <class FF> void SetOverlapQEDEigenfunctions(FF *Psi, FF *PsiMid, int firstMode = 0)
+{
+    Settings &cfg = settings();
+    ovl_ctx *c = context(Psi->N, Psi->NumberOfSamples);
+    check(ovl_qed_eigenbasis(c, &cfg.sign, cfg.bflux, cfg.qcharge, cfg.dTau, firstMode, nullptr, nullptr));
+    for (int s = 0; s < Psi->NumberOfSamples; ++s) {
+        check(ovl_get_modes(c, OVL_PSI, reinterpret_cast<double *>(Psi->GetMode(s)), s, 1));
+        check(ovl_get_modes(c, OVL_PSIMID, reinterpret_cast<double *>(PsiMid->GetMode(s)), s, 1));
+    }
+    state().bound = false;
+}
+}  // namespace InitialConditions
 
 }  // namespace ovl_ref

@@ -2319,6 +2319,9 @@ int ovl_qed_eigenbasis(ovl_ctx *c, const ovl_sign_opts *o, int nb, double q, dou
             b = std::conj(h);
         }
     if (hermiticity) *hermiticity = asym;
+    for (const cplx &v : H)
+        if (!std::isfinite(v.real()) || !std::isfinite(v.imag()))
+            return fail(c, OVL_E_SPECTRUM, "the assembled overlap operator has a non-finite entry");
 
     // diagonalisation on the device (the reference: LAPACK zheev, MISC/LapackDiagonalization.cpp:68-120)
     void *dA = nullptr, *dW = nullptr, *dE = nullptr, *dInfo = nullptr;
@@ -2332,26 +2335,48 @@ int ovl_qed_eigenbasis(ovl_ctx *c, const ovl_sign_opts *o, int nb, double q, dou
     if (r == OVL_OK) r = dalloc(c, &dInfo, sizeof(int));
     if (r != OVL_OK) { release(); return r; }
     std::vector<double> W(D);
-    int info = -1;
+    std::vector<cplx> Hin = H;  // kept for the QR fallback
     rocblas_handle handle = nullptr;
     if (rocblas_create_handle(&handle) != rocblas_status_success) { release(); return fail(c, OVL_E_HIP, "rocblas_create_handle failed"); }
     rocblas_set_stream(handle, c->st);
-    hipError_t he = hipMemcpyAsync(dA, H.data(), sizeof(cplx) * (size_t)D * D, hipMemcpyHostToDevice, c->st);
-    rocblas_status rs = rocblas_status_success;
-    if (he == hipSuccess)
-        rs = rocsolver_zheevd(handle, rocblas_evect_original, rocblas_fill_lower, (rocblas_int)D,
-                              (rocblas_double_complex *)dA, (rocblas_int)D, (double *)dW, (double *)dE,
-                              (rocblas_int *)dInfo);
-    if (he == hipSuccess && rs == rocblas_status_success) he = hipMemcpyAsync(W.data(), dW, sizeof(double) * D, hipMemcpyDeviceToHost, c->st);
-    if (he == hipSuccess && rs == rocblas_status_success)
-        he = hipMemcpyAsync(H.data(), dA, sizeof(cplx) * (size_t)D * D, hipMemcpyDeviceToHost, c->st);
-    if (he == hipSuccess && rs == rocblas_status_success) he = hipMemcpyAsync(&info, dInfo, sizeof(int), hipMemcpyDeviceToHost, c->st);
-    if (he == hipSuccess) he = hipStreamSynchronize(c->st);
+    // divide and conquer first; the QR iteration if it leaves non-finite values
+    std::string why;
+    bool ok = false;
+    for (int method = 0; method < 2 && !ok; ++method) {
+        int info = -1;
+        hipError_t he = hipMemcpyAsync(dA, Hin.data(), sizeof(cplx) * (size_t)D * D, hipMemcpyHostToDevice, c->st);
+        rocblas_status rs = rocblas_status_success;
+        if (he == hipSuccess)
+            rs = method == 0 ? rocsolver_zheevd(handle, rocblas_evect_original, rocblas_fill_lower, (rocblas_int)D,
+                                                (rocblas_double_complex *)dA, (rocblas_int)D, (double *)dW,
+                                                (double *)dE, (rocblas_int *)dInfo)
+                             : rocsolver_zheev(handle, rocblas_evect_original, rocblas_fill_lower, (rocblas_int)D,
+                                               (rocblas_double_complex *)dA, (rocblas_int)D, (double *)dW,
+                                               (double *)dE, (rocblas_int *)dInfo);
+        if (he == hipSuccess && rs == rocblas_status_success)
+            he = hipMemcpyAsync(W.data(), dW, sizeof(double) * D, hipMemcpyDeviceToHost, c->st);
+        if (he == hipSuccess && rs == rocblas_status_success)
+            he = hipMemcpyAsync(H.data(), dA, sizeof(cplx) * (size_t)D * D, hipMemcpyDeviceToHost, c->st);
+        if (he == hipSuccess && rs == rocblas_status_success)
+            he = hipMemcpyAsync(&info, dInfo, sizeof(int), hipMemcpyDeviceToHost, c->st);
+        if (he == hipSuccess) he = hipStreamSynchronize(c->st);
+        const char *name = method == 0 ? "zheevd" : "zheev";
+        if (he != hipSuccess) { why += std::string(name) + ": " + hipGetErrorString(he) + "; "; continue; }
+        if (rs != rocblas_status_success) { why += std::string(name) + ": status " + std::to_string((int)rs) + "; "; continue; }
+        if (info != 0) { why += std::string(name) + ": info " + std::to_string(info) + "; "; continue; }
+        long bad_w = 0, bad_v = 0;
+        for (double w : W) bad_w += !std::isfinite(w);
+        for (const cplx &v : H) bad_v += !(std::isfinite(v.real()) && std::isfinite(v.imag()));
+        if (bad_w || bad_v) {
+            why += std::string(name) + ": " + std::to_string(bad_w) + " non-finite eigenvalues, " + std::to_string(bad_v) +
+                   " non-finite vector entries; ";
+            continue;
+        }
+        ok = true;
+    }
     rocblas_destroy_handle(handle);
     release();
-    if (he != hipSuccess) return fail(c, OVL_E_HIP, std::string("eigensolver transfer: ") + hipGetErrorString(he));
-    if (rs != rocblas_status_success) return fail(c, OVL_E_HIP, "rocsolver_zheevd failed");
-    if (info != 0) return fail(c, OVL_E_CONVERGE, "rocsolver_zheevd did not converge");
+    if (!ok) return fail(c, OVL_E_CONVERGE, "eigensolver failed: " + why);
     // the reference stores the spectrum in descending order (LapackDiagonalization.cpp:103-112)
     if (eigenvalues)
         for (long b = 0; b < D; ++b) eigenvalues[b] = W[D - 1 - b];

@@ -347,7 +347,7 @@ class Context:
         ev = np.zeros(D)
         herm = C.c_double()
         self._chk(lib().ovl_qed_eigenbasis(self.h, C.byref(opts), nb, C.c_double(q), C.c_double(dt), first_mode,
-                                           _d(ev), C.byref(herm)), allow=(E_CONVERGE,))
+                                           _d(ev), C.byref(herm)))
         return ev, herm.value
 
     def gauss_restore(self, tol=1e-12, max_iter=10000, s=0):

@@ -4,6 +4,8 @@
 #include <cmath>
 #include <complex>
 #include <cstdio>
+#include <cstring>
+#include <string>
 #include <random>
 #include <vector>
 
@@ -113,6 +115,60 @@ int main()
     double ea = ovl_ref::OverlapOperations::ZolotarevCoefficients(0.1, 25.0, 1e-12, 25, bz, dz);
     double eb = orc_zolotarev(0.1, 25.0, 1e-12, 25, bz2, dz2);
     fails += !(ea == eb && std::memcmp(bz, bz2, sizeof bz) == 0 && std::memcmp(dz, dz2, sizeof dz) == 0);
+
+    // GaussLawRestoration::Restore: converges, links untouched, against the oracle's iteration
+    {
+        std::vector<double> lk0(U.store), ef0(E.store);
+        cfg.gauss_tol = 1e-11;
+        const int it = ovl_ref::GaussLawRestoration::Restore(&U, &E);
+        double vmax = 0;
+        const int it_ref = orc_gauss_restore(&L, lk0.data(), ef0.data(), 1e-11, 10000, &vmax);
+        const double eg = maxrel(E.E, ef0.data(), ef0.size());
+        double gmax = 0, grms = 0;
+        orc_gauss_violation(&L, U.U, E.E, &gmax, &grms);
+        std::printf("GaussLawRestoration::Restore %d iterations (oracle %d), E rel err %.3e, max |G| %.2e\n", it,
+                    it_ref, eg, gmax);
+        // the stopping test compares a maximum with 1e-11, so the count may differ by one step
+        fails += !(std::abs(it - it_ref) <= 1 && (it != it_ref || eg < 1e-10) && gmax <= 2e-11 &&
+                   std::memcmp(U.U, lk0.data(), lk0.size() * sizeof(double)) == 0);
+    }
+
+    // IO::SaveConfiguration / LoadConfiguration round trip
+    {
+        const std::string uf = "/tmp/ovl_compat_u.txt", ef = "/tmp/ovl_compat_e.txt";
+        ovl_ref::IO::SaveConfiguration(&U, &E, uf, ef);
+        GaugeLinks U2(4, 4, 4);
+        ElectricFields E2(4, 4, 4);
+        ovl_ref::IO::LoadConfiguration(&U2, &E2, uf, ef);
+        const bool same = U2.store == U.store && E2.store == E.store;
+        std::printf("IO::Save/LoadConfiguration round trip %s\n", same ? "exact" : "DIFFERS");
+        fails += !same;
+        std::remove(uf.c_str());
+        std::remove(ef.c_str());
+    }
+
+    // InitialConditions::SetOverlapQEDEigenfunctions: |Psi_s|^2 = V, PsiMid = Psi e^{-i w dt/2} with |.| kept
+    {
+        cfg.bflux = 1;
+        cfg.sign.method = OVL_SIGN_CHEBYSHEV;
+        cfg.sign.precision = OVL_PREC_FP64;
+        cfg.sign.degree = 48;
+        cfg.sign.cut = 0.1;
+        ovl_ref::InitialConditions::SetOverlapQEDEigenfunctions(&Psi, &PsiMid);
+        double worst = 0;
+        for (int s = 0; s < modes; ++s) {
+            double n2 = 0, m2 = 0;
+            for (int x = 0; x < V; ++x)
+                for (int c = 0; c < 2; ++c)
+                    for (int a = 0; a < 4; ++a) {
+                        n2 += std::norm(Psi.modes[s][x].c[c].f[a]);
+                        m2 += std::norm(PsiMid.modes[s][x].c[c].f[a]);
+                    }
+            worst = std::max(worst, std::max(std::fabs(n2 - V), std::fabs(m2 - V)) / V);
+        }
+        std::printf("SetOverlapQEDEigenfunctions norm deviation %.3e\n", worst);
+        fails += !(worst < 1e-10);
+    }
 
     // error behaviour: bad range throws with the C-ABI code
     try {
