@@ -93,13 +93,23 @@ def _kernel_name(n: str) -> str:
     return n.replace("void ", "").replace("ovl::(anonymous namespace)::", "")
 
 
-def measure_traffic(args, c: cf.Config, prefix: str, n_last: int):
+def tuning_string(ctx) -> str:
+    """The context's stencil tuning in the --tuning format."""
+    parts = []
+    for p, name in ((0, "64"), (1, "32"), (2, "16")):
+        t = ctx.get_tuning(p)
+        parts.append(f"{name}:{t.traversal}/{t.slabs}/{t.nt_store}/{t.waves_per_eu}")
+    return ",".join(parts)
+
+
+def measure_traffic(args, c: cf.Config, prefix: str, n_last: int, tuning: str):
     """HBM bytes per launch of the dominant stencil variant from FETCH_SIZE and WRITE_SIZE, each
     scaled by what the same counter reads for the calibration copy of one FP64 field (known bytes
     read and written).  The counted run is `bench.py --quick --steps 1` of the same workload; the
     last `n_last` dispatches whose name starts with `prefix` are its timed step."""
     child = ["--quick", "--steps", "1", "--warmup", "0", "--workload", c.name, "--mb", str(args.mb),
-             "--traversal", str(args.traversal), "--tol", str(args.tol), "--chunk", str(args.chunk)]
+             "--traversal", str(args.traversal), "--tol", str(args.tol), "--chunk", str(args.chunk),
+             "--tuning", tuning]
     if args.modes:
         child += ["--modes", str(args.modes)]
     outdir = tempfile.mkdtemp(prefix="ovl_pmc_", dir="/tmp")
@@ -212,6 +222,8 @@ def main():
     ap.add_argument("--cpu-modes", type=int, default=16)
     ap.add_argument("--no-ensemble", action="store_true", help="skip the ensemble workload")
     ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 counter passes")
+    ap.add_argument("--tuning", default="", help="fixed stencil tuning instead of the autotune sweep, e.g. "
+                    "'64:2/1/1/2,32:2/1/1/1,16:2/1/0/1' = precision:traversal/slabs/nt_store/waves_per_eu")
     ap.add_argument("--ensemble-only", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -266,10 +278,19 @@ def main():
         log(f"mixed vs fp64 rel err {res['rel_err_mixed_vs_fp64']:.3e}  steps fp64 {tm.fp64_steps} "
             f"fp32 {tm.fp32_steps} fp16 {tm.fp16_steps}")
 
-    # --- occupancy autotune of the stencil kernel for this shape
-    tune = ctx.autotune()
+    # --- stencil autotune for this shape (or a fixed tuning)
+    if args.tuning:
+        for item in args.tuning.split(","):
+            p, v = item.split(":")
+            prec = {"64": ovl.PREC_FP64, "32": 1, "16": 2}[p.strip()]
+            ctx.set_tuning(prec, *[int(x) for x in v.split("/")])
+        tune = "fixed " + args.tuning
+    else:
+        tune = ctx.autotune()
     log("autotune", tune)
+    tuning = tuning_string(ctx)
     res["autotune"] = tune.split(";")[-1].strip()  # the choice; every candidate's time is in the log
+    res["tuning"] = tuning
 
     # --- warmup + timed region: mixed form, per-launch HIP events on the library stream
     so = step_opts(c, prec_mixed, False, args.tol, args.chunk)
@@ -354,7 +375,7 @@ def main():
     if not args.no_traffic:
         prefix = ["k_hop<%d, double, double" % c.nc, "k_hop<%d, float, float" % c.nc,
                   "k_hop<%d, float, double" % c.nc, "k_hop<%d, _Float16, _Float16" % c.nc][var]
-        tr = measure_traffic(args, c, prefix, int(round(t.var_launches[var] / args.steps)))
+        tr = measure_traffic(args, c, prefix, int(round(t.var_launches[var] / args.steps)), tuning)
         if tr:
             traffic = tr["FETCH_SIZE"]["bytes_per_launch"] + tr["WRITE_SIZE"]["bytes_per_launch"]
             roofline["traffic"] = traffic

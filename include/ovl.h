@@ -260,6 +260,18 @@ int ovl_set_profiling(ovl_ctx *ctx, int on);
  * times every occupancy variant of the stencil kernel on this context's lattice and modes, for
  * FP64 and FP32, keeps the fastest for later calls and writes a one-line report */
 int ovl_autotune(ovl_ctx *ctx, char *report, int len);
+/* the stencil kernel's tuning per storage precision (0 = FP64, 1 = FP32, 2 = FP16 storage), as
+ * ovl_autotune leaves it: lane traversal (1 lexicographic, 2 XCD boxes, 3 XCD mode split; see
+ * ovl_params.traversal), y slabs of the traversal, streaming output stores (0/1), occupancy
+ * variant (minimum waves per SIMD, 1-4).  Setting one replays a saved tuning without the sweep. */
+typedef struct {
+    int traversal;
+    int slabs;
+    int nt_store;
+    int waves_per_eu;
+} ovl_tuning;
+int ovl_get_tuning(ovl_ctx *ctx, int precision, ovl_tuning *t);
+int ovl_set_tuning(ovl_ctx *ctx, int precision, const ovl_tuning *t);
 
 #ifdef __cplusplus
 }

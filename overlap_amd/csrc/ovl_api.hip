@@ -2416,6 +2416,32 @@ int ovl_set_profiling(ovl_ctx *c, int on)
     return OVL_OK;
 }
 
+// public traversal numbers (ovl_params.traversal) <-> TRAV_*
+static int trav_public(int t) { return t == ovl::TRAV_BOXES ? 2 : (t == ovl::TRAV_MODES ? 3 : 1); }
+
+int ovl_get_tuning(ovl_ctx *c, int precision, ovl_tuning *t)
+{
+    if (!c || !t || precision < P64 || precision > P16) return OVL_E_ARG;
+    t->traversal = trav_public(c->trav[precision]);
+    t->slabs = c->slabs[precision];
+    t->nt_store = c->nt_out[precision];
+    t->waves_per_eu = c->wpe[precision];
+    return OVL_OK;
+}
+
+int ovl_set_tuning(ovl_ctx *c, int precision, const ovl_tuning *t)
+{
+    if (!c || !t || precision < P64 || precision > P16) return OVL_E_ARG;
+    if (t->traversal < 1 || t->traversal > 3 || t->slabs < 1 || t->waves_per_eu < 1 || t->waves_per_eu > 4 ||
+        (t->nt_store != 0 && t->nt_store != 1))
+        return fail(c, OVL_E_ARG, "tuning out of range");
+    const int trav = t->traversal == 2 ? ovl::TRAV_BOXES : (t->traversal == 3 ? ovl::TRAV_MODES : ovl::TRAV_LEX);
+    RETCHK(set_traversal(c, precision, trav, t->slabs));
+    c->nt_out[precision] = t->nt_store;
+    c->wpe[precision] = t->waves_per_eu;
+    return OVL_OK;
+}
+
 int ovl_autotune(ovl_ctx *c, char *report, int len)
 {
     // Times the Clenshaw step (pure stencil + epilogue stencil) of every lane traversal (when the

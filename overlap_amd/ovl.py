@@ -33,7 +33,7 @@ SYMBOLS = [
     "ovl_get_system_efield", "ovl_set_system_u1_flux", "ovl_set_system_modes", "ovl_get_system_modes",
     "ovl_system_observables", "ovl_ensemble_balance", "ovl_gauss_restore", "ovl_qed_eigenbasis",
     "ovl_save_configuration", "ovl_load_configuration",
-    "ovl_get_timing", "ovl_set_profiling", "ovl_autotune",
+    "ovl_get_timing", "ovl_set_profiling", "ovl_autotune", "ovl_get_tuning", "ovl_set_tuning",
 ]
 OBS_NAMES = ("e_el", "e_mag", "gauss_max", "gauss_rms", "unit_max", "unit_rms", "ncs_dot")
 
@@ -59,6 +59,10 @@ class SignOpts(C.Structure):
 class StepOpts(C.Structure):
     _fields_ = [("kind", C.c_int), ("dt", C.c_double), ("gauge", C.c_int), ("sign", SignOpts),
                 ("use_graph", C.c_int)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("traversal", C.c_int), ("slabs", C.c_int), ("nt_store", C.c_int), ("waves_per_eu", C.c_int)]
 
 
 class Observe(C.Structure):
@@ -297,6 +301,14 @@ class Context:
 
     def set_profiling(self, on: bool):
         self._chk(lib().ovl_set_profiling(self.h, int(on)))
+
+    def get_tuning(self, precision) -> Tuning:
+        t = Tuning()
+        self._chk(lib().ovl_get_tuning(self.h, precision, C.byref(t)))
+        return t
+
+    def set_tuning(self, precision, traversal, slabs=1, nt_store=1, waves_per_eu=1):
+        self._chk(lib().ovl_set_tuning(self.h, precision, C.byref(Tuning(traversal, slabs, nt_store, waves_per_eu))))
 
     def autotune(self) -> str:
         buf = C.create_string_buffer(16384)
