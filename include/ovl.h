@@ -93,6 +93,9 @@ int ovl_destroy(ovl_ctx *ctx);
 const char *ovl_error(const ovl_ctx *ctx);
 int ovl_device_name(char *buf, int len);
 void ovl_sign_defaults(ovl_sign_opts *o);
+/* the precision ladder the Chebyshev sign uses for these options (host arithmetic, no device):
+ * steps k < *head in FP64, *head <= k < *head16 in FP32, k >= *head16 in FP16 storage */
+int ovl_sign_plan(const ovl_sign_opts *o, int *head, int *head16);
 
 /* ---- gauge field (GaugeLinks / ElectricFields objects, LATTICE/GaugeLinks.cpp:11-89,
  *      LATTICE/ElectricFields.cpp:9-88) ---- */

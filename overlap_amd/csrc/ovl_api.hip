@@ -530,6 +530,32 @@ struct SignPlan {
     double sx = 1.0;
 };
 
+// the precision ladder: k < head FP64, head <= k < head16 FP32, k >= head16 FP16 storage
+void plan_tiers(const ovl_sign_opts *o, const std::vector<double> &coef, int *head, int *head16)
+{
+    const int n = (int)coef.size() - 1;
+    const double u32 = 5.96e-8, u16 = 4.89e-4;
+    const double tol = o->tol > 0 ? o->tol : 1e-10;
+    *head16 = n + 1;
+    if (o->precision == OVL_PREC_FP64) {
+        *head = n + 1;
+    } else if (o->precision == OVL_PREC_FP32) {
+        *head = 0;
+    } else {
+        const std::vector<double> B = clenshaw_sup(coef, -1.0);
+        const bool fp16 = o->fp16_tail != 0;
+        // error budget: 3/4 to the FP32 tier, 1/4 to the FP16 tier
+        if (o->head >= 0) *head = std::min(o->head, n + 1);
+        else *head = tier_start(B, n, u32, fp16 ? 0.75 * tol : tol);
+        if (fp16) {
+            if (o->head16 > 0) *head16 = std::min(o->head16, n + 1);
+            else *head16 = tier_start(B, n, u16, 0.25 * tol);
+            // at least one FP32 step between the FP16 tail and the FP64 head
+            *head16 = std::max(*head16, *head + 1);
+        }
+    }
+}
+
 // power of two that maps values bounded by `bound` to at most 2^8 (FP16 keeps 2^15 of headroom
 // above that and its subnormal floor 2^-24 far below the unit roundoff of the stored values)
 double fp16_scale(double bound)
@@ -549,26 +575,7 @@ int make_plan(ovl_ctx *c, const ovl_sign_opts *o, SignPlan &p, double xmax = 0.0
     const double d2 = o->cut * o->cut;
     p.alpha = 2.0 / (p.lam * p.lam * (1.0 - d2));
     p.beta = -(1.0 + d2) / (1.0 - d2);
-    const double u32 = 5.96e-8, u16 = 4.89e-4;
-    const double tol = o->tol > 0 ? o->tol : 1e-10;
-    p.head16 = p.n + 1;
-    if (o->precision == OVL_PREC_FP64) {
-        p.head = p.n + 1;
-    } else if (o->precision == OVL_PREC_FP32) {
-        p.head = 0;
-    } else {
-        const std::vector<double> B = clenshaw_sup(p.coef, -1.0);
-        const bool fp16 = o->fp16_tail != 0;
-        // error budget: 3/4 to the FP32 tier, 1/4 to the FP16 tier
-        if (o->head >= 0) p.head = std::min(o->head, p.n + 1);
-        else p.head = tier_start(B, p.n, u32, fp16 ? 0.75 * tol : tol);
-        if (fp16) {
-            if (o->head16 > 0) p.head16 = std::min(o->head16, p.n + 1);
-            else p.head16 = tier_start(B, p.n, u16, 0.25 * tol);
-            // at least one FP32 step between the FP16 tail and the FP64 head
-            p.head16 = std::max(p.head16, p.head + 1);
-        }
-    }
+    plan_tiers(o, p.coef, &p.head, &p.head16);
     if (p.head16 <= p.n) {
         if (!(xmax > 0.0) || !std::isfinite(xmax)) return fail(c, OVL_E_ARG, "FP16 tier needs a bound of the input norms");
         const std::vector<double> Ball = clenshaw_sup(p.coef, p.beta);
@@ -990,6 +997,15 @@ void ovl_sign_defaults(ovl_sign_opts *o)
     o->head = -1;
     o->fp16_tail = 1;
     o->head16 = 0;
+}
+
+int ovl_sign_plan(const ovl_sign_opts *o, int *head, int *head16)
+{
+    if (!o || !head || !head16) return OVL_E_ARG;
+    if (o->method != OVL_SIGN_CHEBYSHEV || o->degree < 1 || o->degree > 100000 || !(o->cut > 0.0 && o->cut < 1.0))
+        return OVL_E_ARG;
+    plan_tiers(o, cheb_coeffs(o->degree, o->cut), head, head16);
+    return OVL_OK;
 }
 
 int ovl_device_name(char *buf, int len)

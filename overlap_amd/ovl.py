@@ -21,7 +21,7 @@ GAUGE_FROZEN, GAUGE_LINKS, GAUGE_FULL = 0, 1, 2
 
 # exported symbols, in the order of include/ovl.h
 SYMBOLS = [
-    "ovl_create", "ovl_destroy", "ovl_error", "ovl_device_name", "ovl_sign_defaults",
+    "ovl_create", "ovl_destroy", "ovl_error", "ovl_device_name", "ovl_sign_defaults", "ovl_sign_plan",
     "ovl_set_links", "ovl_get_links", "ovl_set_efield", "ovl_get_efield", "ovl_set_u1_flux",
     "ovl_set_modes", "ovl_get_modes", "ovl_copy_field",
     "ovl_apply_wilson", "ovl_apply_overlap", "ovl_apply_sign", "ovl_sign_residual", "ovl_ritz",
@@ -115,6 +115,15 @@ def sign_opts(**kw) -> SignOpts:
     for k, v in kw.items():
         setattr(o, k, v)
     return o
+
+
+def sign_plan(opts: SignOpts):
+    """(head, head16): steps k < head run in FP64, head <= k < head16 in FP32, the rest in FP16."""
+    h, h16 = C.c_int(), C.c_int()
+    rc = lib().ovl_sign_plan(C.byref(opts), C.byref(h), C.byref(h16))
+    if rc != OK:
+        raise OvlError(rc, "invalid sign options")
+    return h.value, h16.value
 
 
 def zolotarev_coefficients(amin, amax, errtol=1e-12, no=25):

@@ -81,3 +81,46 @@ def test_create_reports_missing_device():
     with pytest.raises(ovl.OvlError) as e:
         ovl.Context(2, (4, 4, 4), 4)
     assert e.value.code == ovl.E_HIP
+
+
+def _tier_starts_numpy(deg, cut, tol, fp16):
+    # the precision ladder restated in numpy: B_k = max_t |b_k(t)| of the scalar Clenshaw
+    # recurrence of the Chebyshev coefficients (as orc.cheb_coeffs computes them) on t in [-1, 1];
+    # a tier with unit roundoff u starts at the smallest k whose whole tail has 2 u B_k <= its share
+    import orc
+    c = orc.cheb_coeffs(deg, cut)
+    t = np.cos(np.pi * np.arange(2049) / 2048)
+    b1 = np.zeros_like(t)
+    b2 = np.zeros_like(t)
+    B = np.zeros(deg + 2)
+    for k in range(deg, 0, -1):
+        bk = c[k] + 2 * t * b1 - b2
+        b2, b1 = b1, bk
+        B[k] = np.abs(bk).max()
+
+    def start(u, share):
+        first = deg + 1
+        for k in range(deg, 0, -1):
+            if 2 * u * B[k] > share:
+                break
+            first = k
+        return first
+    head = start(5.96e-8, 0.75 * tol if fp16 else tol)
+    head16 = max(start(4.89e-4, 0.25 * tol), head + 1) if fp16 else deg + 1
+    return head, head16
+
+
+def test_sign_plan_host_only():
+    o = ovl.sign_opts(degree=384, cut=0.05, precision=ovl.PREC_MIXED, tol=1e-10)
+    assert ovl.sign_plan(o) == (113, 211)  # the ladder of the 24^3 SU(3) bench runs
+    assert ovl.sign_plan(o) == _tier_starts_numpy(384, 0.05, 1e-10, True)
+    o.fp16_tail = 0
+    assert ovl.sign_plan(o) == (110, 385) == _tier_starts_numpy(384, 0.05, 1e-10, False)
+    o = ovl.sign_opts(degree=256, cut=0.05, precision=ovl.PREC_MIXED, tol=1e-11)
+    assert ovl.sign_plan(o) == _tier_starts_numpy(256, 0.05, 1e-11, True)
+    assert ovl.sign_plan(ovl.sign_opts(degree=64, precision=ovl.PREC_FP64)) == (65, 65)
+    assert ovl.sign_plan(ovl.sign_opts(degree=64, precision=ovl.PREC_FP32)) == (0, 65)
+    o = ovl.sign_opts(degree=64, precision=ovl.PREC_MIXED, head=4, head16=9)
+    assert ovl.sign_plan(o) == (4, 9)
+    with pytest.raises(ovl.OvlError):
+        ovl.sign_plan(ovl.sign_opts(cut=1.5))
