@@ -1,0 +1,155 @@
+"""The five benchmark workloads at their full sizes on the GPU.  Sampled modes are checked against
+the CPU oracle run on the same inputs (FP64: 1e-12 relative; a few modes, few steps, so the oracle
+finishes in seconds); the whole state is checked through size-independent properties: the mixed
+FP64/FP32/FP16 form against the FP64 form (1e-10 relative per mode), norm conservation of the
+leapfrog, the Gauss law and the link unitarity, which the fermion step must not touch."""
+import numpy as np
+import pytest
+
+import orc
+from overlap_amd import configs as cf
+from overlap_amd import ovl
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def setup(c: cf.Config, nmodes=None):
+    links, e, psi = c.generate(nmodes)
+    ctx = ovl.Context(c.nc, c.n, psi.shape[0], r_wilson=c.r_wilson, m_fermion=c.m_fermion, m_dw=c.m_dw)
+    ctx.set_links(links)
+    ctx.set_efield(e)
+    if c.nb:
+        ctx.set_u1_flux(c.nb, c.q)
+    ctx.set_modes(ovl.PSI, psi)
+    ctx.set_modes(ovl.PSIMID, psi)
+    return ctx, links, e, psi
+
+
+def step_opts(c: cf.Config, prec, graph=1):
+    kind = ovl.OVERLAP if c.kind == "overlap" else ovl.WILSON
+    so = ovl.sign_opts(degree=max(c.cheb_n, 1), cut=c.cheb_cut, precision=prec, tol=1e-10)
+    return ovl.StepOpts(kind=kind, dt=c.dt, gauge=ovl.GAUGE_LINKS, sign=so, use_graph=graph)
+
+
+def oracle_steps(c: cf.Config, links, e, psi, steps, threads=8):
+    L = orc.lat(c.nc, c.n, c.r_wilson, c.m_fermion, c.m_dw)
+    kind = orc.OVERLAP_CHEB if c.kind == "overlap" else orc.WILSON
+    o = orc.StepOpts(kind, c.dt, 1, c.nb, c.q, c.cheb_n, c.cheb_cut, 0.0, 0.0, threads)
+    return orc.leapfrog(L, links, e, psi, psi, o, steps)
+
+
+def test_config0_su2_wilson_8_full_run(gpu):
+    # SU(2) 8^3, 16 modes, 100 steps at dt 0.05: every mode against the oracle
+    c = cf.CONFIGS["su2_wilson_8"]
+    print(f"[case] {c.name} {c.steps} steps", flush=True)
+    ctx, links, e, psi = setup(c)
+    g0 = ctx.gauss_violation()
+    ctx.fermion_update(step_opts(c, ovl.PREC_FP64), c.steps)
+    lk, p, pm, _ = oracle_steps(c, links, e, psi, c.steps)
+    assert rel(ctx.get_links(), lk) < 1e-12
+    assert rel(ctx.get_modes(ovl.PSI), p) < 1e-11 and rel(ctx.get_modes(ovl.PSIMID), pm) < 1e-11
+    norms = ctx.mode_norms(ovl.PSI)
+    assert np.abs(norms - 1).max() < 1e-2  # staggered leapfrog: |psi|^2 conserved to O(dt^2)
+    # links move with E fixed: the Gauss law is not conserved by that (E has no back-reaction),
+    # unitarity is
+    assert ctx.unitarity_violation()[0] < 1e-13
+    assert np.isfinite(g0).all()
+
+
+def test_config1_su2_overlap_16_full_size(gpu):
+    # SU(2) 16^3, 64 modes, degree 256: two modes against the oracle (FP64, 1 step); all modes
+    # mixed against FP64 (2 steps); sign residual of the mixed form
+    c = cf.CONFIGS["su2_overlap_16"]
+    print(f"[case] {c.name}", flush=True)
+    ctx, links, e, psi = setup(c)
+    ctx.fermion_update(step_opts(c, ovl.PREC_FP64), 1)
+    _, p, pm, _ = oracle_steps(c, links, e, psi[:2], 1)
+    assert rel(ctx.get_modes(ovl.PSI, 0, 2), p) < 1e-12 and rel(ctx.get_modes(ovl.PSIMID, 0, 2), pm) < 1e-12
+    f64 = [ctx.get_modes(ovl.PSI), ctx.get_modes(ovl.PSIMID)]
+    ctx2, *_ = setup(c)
+    ctx2.fermion_update(step_opts(c, ovl.PREC_MIXED), 1)
+    for f, ref in zip((ovl.PSI, ovl.PSIMID), f64):
+        mix = ctx2.get_modes(f)
+        err = max(np.linalg.norm(a - b) / np.linalg.norm(b) for a, b in zip(mix, ref))
+        print(f"  mixed vs fp64 field {f}: {err:.2e}", flush=True)
+        assert err < 1e-10
+    r = ctx2.sign_residual(ovl.PSI, step_opts(c, ovl.PREC_MIXED).sign)
+    print(f"  sign residual {r:.2e}", flush=True)
+    assert r < 1e-9
+
+
+def test_config2_su3_wilson_32_full_size(gpu):
+    # SU(3) 32^3 with 4 flux quanta, 512 modes in one batched launch, 3 steps: two modes against
+    # the oracle, norms of all 512
+    c = cf.CONFIGS["su3_wilson_32"]
+    print(f"[case] {c.name}", flush=True)
+    ctx, links, e, psi = setup(c)
+    steps = 3
+    ctx.fermion_update(step_opts(c, ovl.PREC_FP64), steps)
+    sample = [0, 511]
+    lk, p, pm, _ = oracle_steps(c, links, e, psi[sample], steps)
+    assert rel(ctx.get_links(), lk) < 1e-12
+    for i, s in enumerate(sample):
+        assert rel(ctx.get_modes(ovl.PSI, s, 1)[0], p[i]) < 1e-12
+        assert rel(ctx.get_modes(ovl.PSIMID, s, 1)[0], pm[i]) < 1e-12
+    norms = ctx.mode_norms(ovl.PSI)
+    assert norms.shape == (512,) and np.abs(norms - 1).max() < 1e-2
+
+
+def test_config3_su3_overlap_24_full_size(gpu):
+    # the bench workload: SU(3) 24^3, 8 flux quanta, 256 modes, degree 384: one overlap apply of
+    # one mode against the oracle (FP64), the mixed form against FP64 for every mode
+    c = cf.CONFIGS["su3_overlap_24"]
+    print(f"[case] {c.name}", flush=True)
+    ctx, links, e, psi = setup(c)
+    o64 = step_opts(c, ovl.PREC_FP64).sign
+    ctx.apply_overlap(ovl.PSI, ovl.SCRATCH, o64)
+    L = orc.lat(c.nc, c.n, c.r_wilson, c.m_fermion, c.m_dw)
+    ucc = orc.links_to_ucc(L, links, orc.u1_phases(c.n, c.nb, c.q))
+    coef = orc.cheb_coeffs(c.cheb_n, c.cheb_cut)
+    want = orc.overlap_evo_cheb(L, ucc, orc.norm_bound(L), c.cheb_cut, c.cheb_n, coef, psi[17])
+    got = ctx.get_modes(ovl.SCRATCH, 17, 1)[0]
+    assert rel(got, want) < 1e-12
+    ref = ctx.get_modes(ovl.SCRATCH)
+    ctx.apply_overlap(ovl.PSI, ovl.SCRATCH, step_opts(c, ovl.PREC_MIXED).sign)
+    t = ctx.timing()
+    mix = ctx.get_modes(ovl.SCRATCH)
+    err = max(np.linalg.norm(a - b) / np.linalg.norm(b) for a, b in zip(mix, ref))
+    print(f"  mixed vs fp64 over 256 modes: {err:.2e} (steps {t.fp64_steps}/{t.fp32_steps}/{t.fp16_steps})",
+          flush=True)
+    assert err < 1e-10 and t.fp16_steps > 0
+
+
+def test_config4_ensemble_full_size(gpu):
+    # 32 SU(2) systems of uneven sizes: one overlap apply of every system, mixed against FP64 per
+    # system, and the smallest system's first mode against the oracle
+    ec = cf.ENSEMBLES["su2_overlap_ensemble"]
+    print(f"[case] {ec.name}", flush=True)
+    sy = [ec.system(k) for k in range(ec.n_systems)]
+    ens = ovl.Ensemble(ec.nc, [s[0] for s in sy], ec.nmodes, m_dw=ec.m_dw)
+    for k, (n, nb, links, e, psi) in enumerate(sy):
+        ens.set_system_links(k, links)
+        ens.set_system_efield(k, e)
+        ens.set_system_u1_flux(k, nb, ec.q)
+        ens.set_system_modes(k, ovl.PSI, psi)
+    so = lambda prec: ovl.sign_opts(degree=ec.cheb_n, cut=ec.cheb_cut, precision=prec, tol=1e-10)  # noqa: E731
+    ens.apply_overlap(ovl.PSI, ovl.SCRATCH, so(ovl.PREC_FP64))
+    ref = [ens.get_system_modes(k, ovl.SCRATCH) for k in range(ec.n_systems)]
+    ens.apply_overlap(ovl.PSI, ovl.SCRATCH, so(ovl.PREC_MIXED))
+    worst = 0.0
+    for k in range(ec.n_systems):
+        mix = ens.get_system_modes(k, ovl.SCRATCH)
+        worst = max(worst, max(np.linalg.norm(a - b) / np.linalg.norm(b) for a, b in zip(mix, ref[k])))
+    print(f"  worst system mixed vs fp64: {worst:.2e}", flush=True)
+    assert worst < 1e-10
+    k = min(range(ec.n_systems), key=lambda i: np.prod(sy[i][0]))
+    n, nb, links, e, psi = sy[k]
+    L = orc.lat(ec.nc, n, ec.r_wilson, ec.m_fermion, ec.m_dw)
+    ucc = orc.links_to_ucc(L, links, orc.u1_phases(n, nb, ec.q) if nb else None)
+    coef = orc.cheb_coeffs(ec.cheb_n, ec.cheb_cut)
+    want = orc.overlap_evo_cheb(L, ucc, orc.norm_bound(L), ec.cheb_cut, ec.cheb_n, coef, psi[0])
+    assert rel(ref[k][0], want) < 1e-12
