@@ -419,3 +419,33 @@ def test_wilson_leapfrog_norm_and_free_amplification():
     _, p1, pm1, _ = orc.leapfrog(L, links, e, psi, psi, o, 40)
     nrm = np.linalg.norm(p1, axis=1)
     assert np.all(np.abs(nrm - 1) < 0.02)
+
+
+@pytest.mark.parametrize("kind", ["wilson", "overlap"])
+def test_leapfrog_conserves_re_psi_psimid(kind):
+    # a step is Psi' = Psi + dt A1 Mid, Mid' = Mid + dt A2 Psi' with A = -i h anti-Hermitian and A2
+    # built from the links half a step later (OverlapUpdate.cpp:9-58, WILSON/Update.cpp:6-47), so
+    # Re<Psi, Mid> changes only by Re of imaginary terms: it is conserved for every mode while the
+    # links move and |Psi|^2 is not.  Using Psi instead of Psi' in the second half, or a
+    # non-Hermitian h, breaks it at O(dt^2).
+    g = cf.rng(52)
+    n = (4, 4, 4) if kind == "wilson" else (4, 4, 2)
+    nc = 3
+    V = int(np.prod(n))
+    links = cf.su3_links(n, 0.3, g)
+    e = cf.efield(n, nc, 0.3, g)
+    psi = cf.spinors(n, nc, 3, g)
+    if kind == "wilson":
+        L = orc.lat(nc, n, 1.0, 0.1, 0.0)
+        o = orc.StepOpts(orc.WILSON, 0.05, 1, 2, 1.0, 0, 0.0, 0.0, 0.0, 1)
+        steps = 30
+    else:
+        L = orc.lat(nc, n, 1.0, 0.0, 1.2)
+        o = orc.StepOpts(orc.OVERLAP_CHEB, 0.02, 1, 1, 1.0, 48, 0.1, 0.0, 0.0, 1)
+        steps = 4
+    lk, p, pm, _ = orc.leapfrog(L, links, e, psi, psi, o, steps)
+    assert not np.array_equal(lk, links)
+    inv = np.einsum("ij,ij->i", p.conj(), pm).real
+    assert np.abs(inv - 1).max() < 1e-13
+    assert np.abs(np.linalg.norm(p, axis=1) ** 2 - 1).max() > 1e-4  # the norm itself moves
+    assert V * nc * 4 == p.shape[1]
