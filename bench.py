@@ -96,7 +96,7 @@ def _kernel_name(n: str) -> str:
 def tuning_string(ctx) -> str:
     """The context's stencil tuning in the --tuning format."""
     parts = []
-    for p, name in ((0, "64"), (1, "32"), (2, "16")):
+    for p, name in ((0, "64"), (1, "32"), (2, "16"), (3, "48")):
         t = ctx.get_tuning(p)
         parts.append(f"{name}:{t.traversal}/{t.slabs}/{t.nt_store}/{t.waves_per_eu}")
     return ",".join(parts)
@@ -180,12 +180,14 @@ def bench_ensemble(ec: cf.EnsembleConfig, steps: int, warmup: int, tol: float):
         wall = time.perf_counter() - t0
         ens.set_profiling(False)
         t = ens.timing()
-        var = int(np.argmax([t.var_ms[i] for i in range(4)]))
+        var = int(np.argmax([t.var_ms[i] for i in range(5)]))
         out[name] = {"ms_per_step": 1e3 * wall / steps, "value": sites * ec.nmodes * steps / wall,
-                     "kernel": ["k_hop_ens FP64", "k_hop_ens FP32", "k_hop_ens FP32->FP64", "k_hop_ens FP16"][var],
+                     "kernel": ["k_hop_ens FP64", "k_hop_ens FP32", "k_hop_ens FP32->FP64", "k_hop_ens FP16",
+                                "k_hop_ens 48-bit"][var],
                      "kernel_GBps": t.var_bytes[var] / max(t.var_ms[var], 1e-9) / 1e6,
                      "kernel_share_of_step": t.var_ms[var] / max(t.total_ms, 1e-9),
-                     "fp64_steps": t.fp64_steps, "fp32_steps": t.fp32_steps, "fp16_steps": t.fp16_steps}
+                     "fp64_steps": t.fp64_steps, "fp48_steps": t.fp48_steps, "fp32_steps": t.fp32_steps,
+                     "fp16_steps": t.fp16_steps}
         out[name]["psi"] = [ens.get_system_modes(k, ovl.PSI) for k in range(ec.n_systems)]
         log(f"ensemble {name}: {out[name]['ms_per_step']:.1f} ms/step {out[name]['value']:.4e} site*mode/s, "
             f"{out[name]['kernel']} {out[name]['kernel_GBps']:.0f} GB/s")
@@ -272,17 +274,18 @@ def main():
         tm = ctx.timing()
         res["rel_err_mixed_vs_fp64"] = float(max(np.linalg.norm(a - b) / np.linalg.norm(b) for a, b in zip(mix, ref)))
         res["fp64_steps"] = int(tm.fp64_steps)
+        res["fp48_steps"] = int(tm.fp48_steps)
         res["fp32_steps"] = int(tm.fp32_steps)
         res["fp16_steps"] = int(tm.fp16_steps)
         del ref, mix
         log(f"mixed vs fp64 rel err {res['rel_err_mixed_vs_fp64']:.3e}  steps fp64 {tm.fp64_steps} "
-            f"fp32 {tm.fp32_steps} fp16 {tm.fp16_steps}")
+            f"48-bit {tm.fp48_steps} fp32 {tm.fp32_steps} fp16 {tm.fp16_steps}")
 
     # --- stencil autotune for this shape (or a fixed tuning)
     if args.tuning:
         for item in args.tuning.split(","):
             p, v = item.split(":")
-            prec = {"64": ovl.PREC_FP64, "32": 1, "16": 2}[p.strip()]
+            prec = {"64": 0, "32": 1, "16": 2, "48": 3}[p.strip()]
             ctx.set_tuning(prec, *[int(x) for x in v.split("/")])
         tune = "fixed " + args.tuning
     else:
@@ -307,8 +310,8 @@ def main():
     log(f"mixed: {ms_per_step:.1f} ms/step  {value:.4e} site*mode/s  (events {t.total_ms / args.steps:.1f} ms/step)")
 
     # dominant stencil variant of the timed region
-    var = int(np.argmax([t.var_ms[i] for i in range(4)]))
-    names = ["k_hop FP64", "k_hop FP32", "k_hop FP32->FP64", "k_hop FP16"]
+    var = int(np.argmax([t.var_ms[i] for i in range(5)]))
+    names = ["k_hop FP64", "k_hop FP32", "k_hop FP32->FP64", "k_hop FP16", "k_hop 48-bit"]
     avg_ms = t.var_ms[var] / max(1, t.var_launches[var])
     bytes_per_launch = t.var_bytes[var] / max(1, t.var_launches[var])
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
@@ -331,7 +334,7 @@ def main():
     roofline["variants"] = {
         names[i]: {"launches": int(t.var_launches[i]), "ms": round(t.var_ms[i], 3),
                    "GBps": round(t.var_bytes[i] / max(t.var_ms[i], 1e-9) / 1e6, 1)}
-        for i in range(4) if t.var_launches[i]}
+        for i in range(5) if t.var_launches[i]}
     log("variants", json.dumps(roofline["variants"]))
 
     # --- graph-captured replay of the same step
@@ -374,7 +377,8 @@ def main():
     # HBM traffic of the dominant stencil variant, from two counter passes (child processes)
     if not args.no_traffic:
         prefix = ["k_hop<%d, double, double" % c.nc, "k_hop<%d, float, float" % c.nc,
-                  "k_hop<%d, float, double" % c.nc, "k_hop<%d, _Float16, _Float16" % c.nc][var]
+                  "k_hop<%d, float, double" % c.nc, "k_hop<%d, _Float16, _Float16" % c.nc,
+                  "k_hop<%d, f48, f48" % c.nc][var]
         tr = measure_traffic(args, c, prefix, int(round(t.var_launches[var] / args.steps)), tuning)
         if tr:
             traffic = tr["FETCH_SIZE"]["bytes_per_launch"] + tr["WRITE_SIZE"]["bytes_per_launch"]

@@ -77,6 +77,9 @@ typedef struct {
     int fp16_tail;       /* MIXED: 1 = store the far tail of the recurrence in FP16 (scaled by powers
                             of two, FP32 arithmetic), 0 = FP32 is the lowest tier */
     int head16;          /* MIXED with fp16_tail: first FP16 step (> 0 overrides tol; 0 = from tol) */
+    int fp48;            /* MIXED: 1 = store the FP64 steps whose error bound allows it in 48 bits (the
+                            top 48 bits of the double, rounded; FP64 arithmetic), 0 = plain FP64 */
+    int head48;          /* MIXED with fp48: first 48-bit step (> 0 overrides tol; 0 = from tol) */
 } ovl_sign_opts;
 
 typedef struct {
@@ -94,8 +97,9 @@ const char *ovl_error(const ovl_ctx *ctx);
 int ovl_device_name(char *buf, int len);
 void ovl_sign_defaults(ovl_sign_opts *o);
 /* the precision ladder the Chebyshev sign uses for these options (host arithmetic, no device):
- * steps k < *head in FP64, *head <= k < *head16 in FP32, k >= *head16 in FP16 storage */
-int ovl_sign_plan(const ovl_sign_opts *o, int *head, int *head16);
+ * steps k < *head48 in FP64, *head48 <= k < *head in 48-bit storage, *head <= k < *head16 in
+ * FP32, k >= *head16 in FP16 storage */
+int ovl_sign_plan(const ovl_sign_opts *o, int *head48, int *head, int *head16);
 
 /* ---- gauge field (GaugeLinks / ElectricFields objects, LATTICE/GaugeLinks.cpp:11-89,
  *      LATTICE/ElectricFields.cpp:9-88) ---- */
@@ -249,21 +253,22 @@ typedef struct {
     double hop_ms;            /* summed event time of the stencil kernels */
     long long hop_launches;   /* number of stencil kernel launches */
     double hop_bytes;         /* algorithmic bytes of those launches */
-    int fp64_steps, fp32_steps, fp16_steps;
+    int fp64_steps, fp32_steps, fp16_steps, fp48_steps;
     /* per stencil-kernel variant: [0] FP64 in/out, [1] FP32 in/out, [2] FP32 in / FP64 out,
-     * [3] FP16 storage in/out */
-    double var_ms[4];
-    long long var_launches[4];
-    double var_bytes[4];
+     * [3] FP16 storage in/out, [4] 48-bit storage in/out */
+    double var_ms[5];
+    long long var_launches[5];
+    double var_bytes[5];
 } ovl_timing;
 int ovl_get_timing(ovl_ctx *ctx, ovl_timing *t);
 int ovl_set_profiling(ovl_ctx *ctx, int on);
 
 /* ---- tuning (no counterpart in the reference) ----
- * times every occupancy variant of the stencil kernel on this context's lattice and modes, for
- * FP64 and FP32, keeps the fastest for later calls and writes a one-line report */
+ * times the traversal, store and occupancy variants of the stencil kernel on this context's
+ * lattice and modes for every storage precision, keeps the fastest for later calls and writes a
+ * one-line report */
 int ovl_autotune(ovl_ctx *ctx, char *report, int len);
-/* the stencil kernel's tuning per storage precision (0 = FP64, 1 = FP32, 2 = FP16 storage), as
+/* the stencil kernel's tuning per storage precision (0 = FP64, 1 = FP32, 2 = FP16, 3 = 48-bit), as
  * ovl_autotune leaves it: lane traversal (1 lexicographic, 2 XCD boxes, 3 XCD mode split; see
  * ovl_params.traversal), y slabs of the traversal, streaming output stores (0/1), occupancy
  * variant (minimum waves per SIMD, 1-4).  Setting one replays a saved tuning without the sweep. */

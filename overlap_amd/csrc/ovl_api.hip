@@ -23,6 +23,8 @@ using ovl::HopArgs;
 using ovl::P32;
 using ovl::P64;
 using ovl::P16;
+using ovl::P48;
+using ovl::NUM_PREC;
 using ovl::SysDesc;
 typedef std::complex<double> cplx;
 
@@ -31,6 +33,9 @@ struct DevBuf {
     size_t bytes = 0;
 };
 
+// stencil variants timed separately: [0] FP64, [1] FP32, [2] FP32 in / FP64 out, [3] FP16, [4] 48-bit
+constexpr int NVAR = 5;
+
 struct ovl_ctx {
     ovl_params prm{};
     Geom g{};
@@ -38,6 +43,7 @@ struct ovl_ctx {
     size_t field64_bytes = 0, field32_bytes = 0;  // full field (all blocks)
     size_t blk64_elems = 0, blk32_elems = 0;      // per mode block (double2 / float4 units)
     size_t blk16_elems = 0;                       // per mode block (16-byte FP16 units)
+    size_t blk48_elems = 0;                       // per mode block (16-byte units of the 48-bit layout)
     hipStream_t st = nullptr;
     std::string err;
 
@@ -57,18 +63,19 @@ struct ovl_ctx {
     void *w64 = nullptr, *a64 = nullptr, *b64 = nullptr;
     void *x32 = nullptr, *w32 = nullptr, *a32 = nullptr, *b32 = nullptr;
     void *x16 = nullptr, *w16 = nullptr, *a16 = nullptr, *b16 = nullptr;
+    void *x48 = nullptr, *w48 = nullptr, *a48 = nullptr, *b48 = nullptr;
 
     // reductions / small scratch
     double *partial = nullptr;
     size_t partial_len = 0;
     bool boxes_ok = false;     // the lattice splits into 8 equal boxes (TRAV_BOXES possible)
-    // per input precision (P64, P32, P16) of the stencil, set by ovl_autotune:
-    int *order[3] = {nullptr, nullptr, nullptr};  // device site order of the traversal (null: lexicographic)
-    bool order_set[3] = {false, false, false};
-    int trav[3] = {ovl::TRAV_LEX, ovl::TRAV_LEX, ovl::TRAV_LEX};  // lane traversal
-    int slabs[3] = {1, 1, 1};                   // y slabs (TRAV_MODES) or sub-slabs per box (TRAV_BOXES)
-    int nt_out[3] = {0, 0, 0};                  // streaming output stores
-    int wpe[3] = {1, 1, 1};                     // occupancy variant
+    // per input precision (P64, P32, P16, P48) of the stencil, set by ovl_autotune:
+    int *order[NUM_PREC] = {nullptr, nullptr, nullptr, nullptr};  // device site order (null: lexicographic)
+    bool order_set[NUM_PREC] = {false, false, false, false};
+    int trav[NUM_PREC] = {ovl::TRAV_LEX, ovl::TRAV_LEX, ovl::TRAV_LEX, ovl::TRAV_LEX};  // lane traversal
+    int slabs[NUM_PREC] = {1, 1, 1, 1};         // y slabs (TRAV_MODES) or sub-slabs per box (TRAV_BOXES)
+    int nt_out[NUM_PREC] = {0, 0, 0, 0};        // streaming output stores
+    int wpe[NUM_PREC] = {1, 1, 1, 1};           // occupancy variant
     double step_xmax = 0.0;  // bound of the mode norms during ovl_fermion_update (FP16 scales)
     double *dout = nullptr;  // 16 doubles
     double *gam = nullptr;   // current coupling matrices
@@ -90,11 +97,11 @@ struct ovl_ctx {
     std::vector<hipEvent_t> evpool;
     std::vector<int> evvar;  // stencil variant of each recorded event pair
     size_t evused = 0;
-    long long var_launches[4] = {0, 0, 0, 0};
-    double var_bytes[4] = {0, 0, 0, 0};
+    long long var_launches[NVAR] = {};
+    double var_bytes[NVAR] = {};
     long long hop_launches = 0;
     double hop_bytes = 0;
-    int fp64_steps = 0, fp32_steps = 0, fp16_steps = 0;
+    int fp64_steps = 0, fp32_steps = 0, fp16_steps = 0, fp48_steps = 0;
     ovl_timing last{};
     hipEvent_t t0 = nullptr, t1 = nullptr;
     bool capturing = false;
@@ -229,6 +236,7 @@ void *blk_ptr(void *base, int prec, size_t blk, const ovl_ctx *c)
     if (!base) return nullptr;
     if (prec == P64) return (char *)base + blk * c->blk64_elems * 16;
     if (prec == P16) return (char *)base + blk * c->blk16_elems * 16;
+    if (prec == P48) return (char *)base + blk * c->blk48_elems * 16;
     return (char *)base + blk * c->blk32_elems * 16;
 }
 
@@ -247,7 +255,7 @@ hipEvent_t next_event(ovl_ctx *c)
 double hop_alg_bytes(const ovl_ctx *c, const HopArgs &a, int pin, int pout)
 {
     const double sites_modes = (double)a.nblk * c->g.V * c->g.mb;
-    auto real_bytes = [](int p) { return p == P64 ? 8.0 : (p == P32 ? 4.0 : 2.0); };
+    auto real_bytes = [](int p) { return p == P64 ? 8.0 : (p == P32 ? 4.0 : (p == P48 ? 6.0 : 2.0)); };
     const double Sin = c->ncomp * 2.0 * real_bytes(pin);
     const double Sout = c->ncomp * 2.0 * real_bytes(pout);
     double b = sites_modes * (Sin + Sout);
@@ -255,14 +263,14 @@ double hop_alg_bytes(const ovl_ctx *c, const HopArgs &a, int pin, int pout)
     if (a.b1) b += sites_modes * Sout;
     if (a.b2) b += sites_modes * Sout;
     if (a.tgt) b += sites_modes * Sout;
-    b += 3.0 * c->link_sites * c->g.nc * c->g.nc * 2.0 * (pin == P64 ? 8 : 4);  // FP16 uses the FP32 links
+    // FP16 uses the FP32 links, the 48-bit layout the FP64 links
+    b += 3.0 * c->link_sites * c->g.nc * c->g.nc * 2.0 * ((pin == P64 || pin == P48) ? 8 : 4);
     return b;
 }
 
 int hop(ovl_ctx *c, const Geom &g, int pin, int pout, const HopArgs &a)
 {
-    // variants: [0] FP64, [1] FP32, [2] FP32 in / FP64 out, [3] FP16
-    const int var = (pin == P64) ? 0 : (pin == P16 ? 3 : (pout == P32 ? 1 : 2));
+    const int var = (pin == P64) ? 0 : (pin == P16 ? 3 : (pin == P48 ? 4 : (pout == P32 ? 1 : 2)));
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (c->profiling && !c->capturing) {
         e0 = next_event(c);
@@ -284,7 +292,7 @@ int hop(ovl_ctx *c, const Geom &g, int pin, int pout, const HopArgs &a)
 HopArgs hop_args(const ovl_ctx *c, int prec)
 {
     HopArgs a{};
-    a.links = prec == P64 ? (const void *)c->ul64 : (const void *)c->ul32;
+    a.links = (prec == P64 || prec == P48) ? (const void *)c->ul64 : (const void *)c->ul32;
     a.amass = amass(c);
     a.nblk = c->g.nblk;
     a.trav = c->trav[prec];
@@ -362,7 +370,7 @@ std::vector<int> slab_site_order(int n0, int n1, int n2, int t)
 // of the 4 MB L2 of an XCD
 int default_slabs(const ovl_ctx *c, int prec)
 {
-    const double site_bytes = (double)c->g.mb * c->ncomp * (prec == P64 ? 16 : (prec == P32 ? 8 : 4));
+    const double site_bytes = (double)c->g.mb * c->ncomp * (prec == P64 ? 16 : (prec == P32 ? 8 : (prec == P48 ? 12 : 4)));
     int t = 1;
     while (t < c->g.n1 && 3.0 * c->g.n0 * ((c->g.n1 + t - 1) / t) * site_bytes > 3.0 * (1 << 20)) ++t;
     return t;
@@ -398,8 +406,8 @@ void begin_timing(ovl_ctx *c)
     c->evused = 0;
     c->hop_launches = 0;
     c->hop_bytes = 0;
-    for (int v = 0; v < 4; ++v) { c->var_launches[v] = 0; c->var_bytes[v] = 0; }
-    c->fp64_steps = c->fp32_steps = c->fp16_steps = 0;
+    for (int v = 0; v < NVAR; ++v) { c->var_launches[v] = 0; c->var_bytes[v] = 0; }
+    c->fp64_steps = c->fp32_steps = c->fp16_steps = c->fp48_steps = 0;
     if (!c->capturing) hipEventRecord(c->t0, c->st);
 }
 
@@ -410,7 +418,7 @@ int end_timing(ovl_ctx *c)
     float ms = 0;
     hipEventElapsedTime(&ms, c->t0, c->t1);
     c->last.total_ms = ms;
-    double hop_ms = 0, var_ms[4] = {0, 0, 0, 0};
+    double hop_ms = 0, var_ms[NVAR] = {};
     for (size_t i = 0; i + 1 < c->evused; i += 2) {
         float t = 0;
         hipEventElapsedTime(&t, c->evpool[i], c->evpool[i + 1]);
@@ -418,7 +426,7 @@ int end_timing(ovl_ctx *c)
         var_ms[c->evvar[i / 2]] += t;
     }
     c->last.hop_ms = hop_ms;
-    for (int v = 0; v < 4; ++v) {
+    for (int v = 0; v < NVAR; ++v) {
         c->last.var_ms[v] = var_ms[v];
         c->last.var_launches[v] = c->var_launches[v];
         c->last.var_bytes[v] = c->var_bytes[v];
@@ -428,6 +436,7 @@ int end_timing(ovl_ctx *c)
     c->last.fp64_steps = c->fp64_steps;
     c->last.fp32_steps = c->fp32_steps;
     c->last.fp16_steps = c->fp16_steps;
+    c->last.fp48_steps = c->fp48_steps;
     return OVL_OK;
 }
 
@@ -492,17 +501,25 @@ int tier_start(const std::vector<double> &B, int n, double u, double tol, double
     return first;
 }
 
-int ensure_work(ovl_ctx *c, int blocks, bool need32, bool need16 = false)
+int ensure_work(ovl_ctx *c, int blocks, bool need32, bool need16 = false, bool need48 = false)
 {
     if (blocks > c->work_blocks) {
         dfree(c->w64); dfree(c->a64); dfree(c->b64);
         dfree(c->x32); dfree(c->w32); dfree(c->a32); dfree(c->b32);
         dfree(c->x16); dfree(c->w16); dfree(c->a16); dfree(c->b16);
+        dfree(c->x48); dfree(c->w48); dfree(c->a48); dfree(c->b48);
         c->work_blocks = blocks;
     }
     const size_t b64 = (size_t)c->work_blocks * c->blk64_elems * 16;
     const size_t b32 = (size_t)c->work_blocks * c->blk32_elems * 16;
     const size_t b16 = (size_t)c->work_blocks * c->blk16_elems * 16;
+    const size_t b48 = (size_t)c->work_blocks * c->blk48_elems * 16;
+    if (need48) {
+        RETCHK(dalloc(c, &c->x48, b48));
+        RETCHK(dalloc(c, &c->w48, b48));
+        RETCHK(dalloc(c, &c->a48, b48));
+        RETCHK(dalloc(c, &c->b48, b48));
+    }
     RETCHK(dalloc(c, &c->w64, b64));
     RETCHK(dalloc(c, &c->a64, b64));
     RETCHK(dalloc(c, &c->b64, b64));
@@ -523,35 +540,46 @@ int ensure_work(ovl_ctx *c, int blocks, bool need32, bool need16 = false)
 
 struct SignPlan {
     std::vector<double> coef;
-    int n = 0, head = 0, head16 = 0;  // k < head: FP64, head <= k < head16: FP32, k >= head16: FP16
+    // k < head48: FP64, head48 <= k < head: 48-bit, head <= k < head16: FP32, k >= head16: FP16
+    int n = 0, head48 = 0, head = 0, head16 = 0;
     double lam = 0, alpha = 0, beta = 0;
     // FP16 tier: b_k is stored as sk[k] * b_k, w = h b_{k+1} as sw[k] * w, x as sx * x (powers of two)
     std::vector<double> sk, sw;
     double sx = 1.0;
 };
 
-// the precision ladder: k < head FP64, head <= k < head16 FP32, k >= head16 FP16 storage
-void plan_tiers(const ovl_sign_opts *o, const std::vector<double> &coef, int *head, int *head16)
+// the precision ladder: k < head48 FP64, head48 <= k < head 48-bit storage, head <= k < head16
+// FP32, k >= head16 FP16 storage
+void plan_tiers(const ovl_sign_opts *o, const std::vector<double> &coef, int *head48, int *head, int *head16)
 {
     const int n = (int)coef.size() - 1;
-    const double u32 = 5.96e-8, u16 = 4.89e-4;
+    const double u32 = 5.96e-8, u16 = 4.89e-4, u48 = 7.28e-12;  // 2^-24 (+), 2^-11 (+), 2^-37
     const double tol = o->tol > 0 ? o->tol : 1e-10;
     *head16 = n + 1;
     if (o->precision == OVL_PREC_FP64) {
         *head = n + 1;
+        *head48 = n + 1;
     } else if (o->precision == OVL_PREC_FP32) {
         *head = 0;
+        *head48 = 0;
     } else {
         const std::vector<double> B = clenshaw_sup(coef, -1.0);
-        const bool fp16 = o->fp16_tail != 0;
-        // error budget: 3/4 to the FP32 tier, 1/4 to the FP16 tier
+        const bool fp16 = o->fp16_tail != 0, fp48 = o->fp48 != 0;
+        // error budget: 1/4 to each of the FP16 and 48-bit tiers that are on, the rest to FP32
+        const double s16 = fp16 ? 0.25 : 0.0, s48 = fp48 ? 0.25 : 0.0, s32 = 1.0 - s16 - s48;
         if (o->head >= 0) *head = std::min(o->head, n + 1);
-        else *head = tier_start(B, n, u32, fp16 ? 0.75 * tol : tol);
+        else *head = tier_start(B, n, u32, s32 * tol);
         if (fp16) {
             if (o->head16 > 0) *head16 = std::min(o->head16, n + 1);
-            else *head16 = tier_start(B, n, u16, 0.25 * tol);
+            else *head16 = tier_start(B, n, u16, s16 * tol);
             // at least one FP32 step between the FP16 tail and the FP64 head
-            *head16 = std::max(*head16, *head + 1);
+            *head16 = std::min(std::max(*head16, *head + 1), n + 1);
+        }
+        *head48 = *head;
+        if (fp48 && *head > 1) {
+            const int h = (o->head48 > 0) ? o->head48 : tier_start(B, n, u48, s48 * tol);
+            // the last step (k = 0) stays FP64: its output feeds the final FP64 stencil
+            *head48 = std::min(std::max(h, 1), *head);
         }
     }
 }
@@ -575,7 +603,7 @@ int make_plan(ovl_ctx *c, const ovl_sign_opts *o, SignPlan &p, double xmax = 0.0
     const double d2 = o->cut * o->cut;
     p.alpha = 2.0 / (p.lam * p.lam * (1.0 - d2));
     p.beta = -(1.0 + d2) / (1.0 - d2);
-    plan_tiers(o, p.coef, &p.head, &p.head16);
+    plan_tiers(o, p.coef, &p.head48, &p.head, &p.head16);
     if (p.head16 <= p.n) {
         if (!(xmax > 0.0) || !std::isfinite(xmax)) return fail(c, OVL_E_ARG, "FP16 tier needs a bound of the input norms");
         const std::vector<double> Ball = clenshaw_sup(p.coef, p.beta);
@@ -591,33 +619,41 @@ int make_plan(ovl_ctx *c, const ovl_sign_opts *o, SignPlan &p, double xmax = 0.0
 }
 
 // Clenshaw recurrence for one chunk of mode blocks:  p = P(T) x,  T = alpha h_W^2 + beta.
-// Steps k >= head16 store their vectors in FP16 (scaled), head <= k < head16 in FP32, k < head
-// in FP64.  Returns the buffer and precision of p.
+// Steps k >= head16 store their vectors in FP16 (scaled), head <= k < head16 in FP32,
+// head48 <= k < head in the 48-bit layout, k < head48 in FP64.  Returns the buffer and precision
+// of p.
 int clenshaw_chunk(ovl_ctx *c, const Geom &gc, const SignPlan &pl, const void *x64, void **pout, int *pprec)
 {
     const int n = pl.n;
     const bool any16 = pl.head16 <= n;
     const bool any32 = pl.head <= n;
-    RETCHK(ensure_work(c, gc.nblk, any32, any16));
+    const bool any48 = pl.head48 < pl.head && pl.head48 <= n;
+    RETCHK(ensure_work(c, gc.nblk, any32, any16, any48));
     if (any32) HIPCHK(c, ovl::launch_convert(gc, gc.nblk, x64, P64, c->x32, P32, c->st));
     if (any16) HIPCHK(c, ovl::launch_convert(gc, gc.nblk, x64, P64, c->x16, P16, c->st, pl.sx));
+    if (any48) HIPCHK(c, ovl::launch_convert(gc, gc.nblk, x64, P64, c->x48, P48, c->st));
     // buffer rotation: b1 = b_{k+1}, b2 = b_{k+2} (null = zero), stored times sc1 / sc2
     void *b1 = nullptr, *b2 = nullptr;
     double sc1 = 1.0, sc2 = 1.0;
     int prec_b = -1;  // precision of b1/b2
     for (int k = n; k >= 0; --k) {
-        const int pk = (k < pl.head) ? P64 : (k < pl.head16 ? P32 : P16);
+        const int pk = (k < pl.head48) ? P64 : (k < pl.head ? P48 : (k < pl.head16 ? P32 : P16));
         if (pk == P64) c->fp64_steps++;
+        else if (pk == P48) c->fp48_steps++;
         else if (pk == P32) c->fp32_steps++;
         else c->fp16_steps++;
-        void *xk = (pk == P64) ? (void *)x64 : (pk == P32 ? c->x32 : c->x16);
-        void *A = (pk == P64) ? c->a64 : (pk == P32 ? c->a32 : c->a16);
-        void *B = (pk == P64) ? c->b64 : (pk == P32 ? c->b32 : c->b16);
-        void *W = (pk == P64) ? c->w64 : (pk == P32 ? c->w32 : c->w16);
+        auto pick = [pk](void *p64, void *p48, void *p32, void *p16) {
+            return pk == P64 ? p64 : (pk == P48 ? p48 : (pk == P32 ? p32 : p16));
+        };
+        void *xk = pick((void *)x64, c->x48, c->x32, c->x16);
+        void *A = pick(c->a64, c->a48, c->a32, c->a16);
+        void *B = pick(c->b64, c->b48, c->b32, c->b16);
+        void *W = pick(c->w64, c->w48, c->w32, c->w16);
         const double sk = (pk == P16) ? pl.sk[k] : 1.0, sw = (pk == P16) ? pl.sw[k] : 1.0;
         const double sx = (pk == P16) ? pl.sx : 1.0;
         if (b1 && prec_b != pk) {
-            // FP16 -> FP32 or FP32 -> FP64 transition: promote b_{k+1}, b_{k+2} (and drop the scale)
+            // FP16 -> FP32 -> (48-bit ->) FP64 transitions: promote b_{k+1}, b_{k+2} (and drop the
+            // FP16 scale)
             HIPCHK(c, ovl::launch_convert(gc, gc.nblk, b1, prec_b, A, pk, c->st, 1.0 / sc1));
             if (b2) HIPCHK(c, ovl::launch_convert(gc, gc.nblk, b2, prec_b, B, pk, c->st, 1.0 / sc2));
             b1 = A;
@@ -626,8 +662,8 @@ int clenshaw_chunk(ovl_ctx *c, const Geom &gc, const SignPlan &pl, const void *x
             prec_b = pk;
         }
         if (!b1) {  // b_n = c_n x
-            if (pk == P16)
-                HIPCHK(c, ovl::launch_convert(gc, gc.nblk, x64, P64, A, P16, c->st, pl.coef[k] * sk));
+            if (pk == P16 || pk == P48)
+                HIPCHK(c, ovl::launch_convert(gc, gc.nblk, x64, P64, A, pk, c->st, pl.coef[k] * sk));
             else
                 HIPCHK(c, ovl::launch_lincomb(gc, gc.nblk, pk, A, xk, pl.coef[k], 0.0, nullptr, 0, 0, c->st));
             b1 = A;
@@ -997,14 +1033,16 @@ void ovl_sign_defaults(ovl_sign_opts *o)
     o->head = -1;
     o->fp16_tail = 1;
     o->head16 = 0;
+    o->fp48 = 1;
+    o->head48 = 0;
 }
 
-int ovl_sign_plan(const ovl_sign_opts *o, int *head, int *head16)
+int ovl_sign_plan(const ovl_sign_opts *o, int *head48, int *head, int *head16)
 {
-    if (!o || !head || !head16) return OVL_E_ARG;
+    if (!o || !head48 || !head || !head16) return OVL_E_ARG;
     if (o->method != OVL_SIGN_CHEBYSHEV || o->degree < 1 || o->degree > 100000 || !(o->cut > 0.0 && o->cut < 1.0))
         return OVL_E_ARG;
-    plan_tiers(o, cheb_coeffs(o->degree, o->cut), head, head16);
+    plan_tiers(o, cheb_coeffs(o->degree, o->cut), head48, head, head16);
     return OVL_OK;
 }
 
@@ -1027,6 +1065,7 @@ static int ctx_init(ovl_ctx *c)
     c->blk64_elems = (size_t)c->g.V * c->ncomp * mb;
     c->blk32_elems = (size_t)c->g.V * (c->ncomp / 2) * mb;
     c->blk16_elems = (size_t)c->g.V * (c->ncomp / 4) * mb;
+    c->blk48_elems = 3 * c->blk16_elems;
     c->field64_bytes = c->blk64_elems * 16 * c->g.nblk;
     c->field32_bytes = c->blk32_elems * 16 * c->g.nblk;
     c->sys_nb.assign(c->sys.size(), 0);
@@ -1060,7 +1099,7 @@ static int ctx_init(ovl_ctx *c)
     if (!c->ensemble) {
         // traversal 0 (automatic): mode split where the mode blocks divide over the XCDs, else
         // boxes; ovl_autotune may change it
-        for (int p = P64; p <= P16; ++p) {
+        for (int p = 0; p < NUM_PREC; ++p) {
             int t = ovl::TRAV_LEX;
             if (c->prm.traversal == 0) t = (c->g.nblk % ovl::NUM_XCD == 0) ? ovl::TRAV_MODES : ovl::TRAV_BOXES;
             if (c->prm.traversal == 2) t = ovl::TRAV_BOXES;
@@ -1213,7 +1252,7 @@ int ovl_destroy(ovl_ctx *c)
                   c->b64, c->x32, c->w32, c->a32, c->b32, c->partial, c->dout, c->gam, c->jv, c->ja, c->staging,
                   c->ms_ps, c->ms_r, c->ms_q, c->ms_w, c->ms_chi, c->ms_coef, c->ms_active, c->rz[0], c->rz[1],
                   c->rz[2], c->rz[3], c->rz[4], c->rz[5], c->d_sys, c->d_work, c->order[0], c->order[1],
-                  c->order[2], c->x16, c->w16, c->a16, c->b16};
+                  c->order[2], c->order[3], c->x16, c->w16, c->a16, c->b16, c->x48, c->w48, c->a48, c->b48};
     for (void *p : ps)
         if (p) hipFree(p);
     for (auto e : c->evpool) hipEventDestroy(e);
@@ -1869,9 +1908,10 @@ static int update_impl(ovl_ctx *c, const ovl_step_opts *o, int nsteps, bool obse
     }
     // capture one full step (both half steps and both gauge half steps) and replay it
     // every buffer the captured step touches is allocated before the capture
+    const bool mixed = o->kind == OVL_OVERLAP && o->sign.precision == OVL_PREC_MIXED;
     RETCHK(ensure_work(c, (o->sign.mode_chunk > 0) ? std::min(o->sign.mode_chunk, c->g.nblk) : c->g.nblk,
-                       o->kind == OVL_OVERLAP && o->sign.precision != OVL_PREC_FP64,
-                       o->kind == OVL_OVERLAP && o->sign.precision == OVL_PREC_MIXED && o->sign.fp16_tail));
+                       o->kind == OVL_OVERLAP && o->sign.precision != OVL_PREC_FP64, mixed && o->sign.fp16_tail,
+                       mixed && o->sign.fp48));
     hipGraph_t graph_h = nullptr;
     hipGraphExec_t exec = nullptr;
     HIPCHK(c, hipStreamSynchronize(c->st));
@@ -1884,10 +1924,10 @@ static int update_impl(ovl_ctx *c, const ovl_step_opts *o, int nsteps, bool obse
     }
     const long long l0 = c->hop_launches;
     const double b0 = c->hop_bytes;
-    long long vl0[4];
-    double vb0[4];
-    for (int v = 0; v < 4; ++v) { vl0[v] = c->var_launches[v]; vb0[v] = c->var_bytes[v]; }
-    const int s64 = c->fp64_steps, s32 = c->fp32_steps, s16 = c->fp16_steps;
+    long long vl0[NVAR];
+    double vb0[NVAR];
+    for (int v = 0; v < NVAR; ++v) { vl0[v] = c->var_launches[v]; vb0[v] = c->var_bytes[v]; }
+    const int s64 = c->fp64_steps, s32 = c->fp32_steps, s16 = c->fp16_steps, s48 = c->fp48_steps;
     int r = one_step(c, o);
     hipError_t ee = hipStreamEndCapture(c->st, &graph_h);
     c->capturing = false;
@@ -1899,6 +1939,7 @@ static int update_impl(ovl_ctx *c, const ovl_step_opts *o, int nsteps, bool obse
     const long long per_l = c->hop_launches - l0;
     const double per_b = c->hop_bytes - b0;
     const int p64 = c->fp64_steps - s64, p32 = c->fp32_steps - s32, p16 = c->fp16_steps - s16;
+    const int p48 = c->fp48_steps - s48;
     e = hipGraphInstantiate(&exec, graph_h, nullptr, nullptr, 0);
     if (e != hipSuccess) {
         hipGraphDestroy(graph_h);
@@ -1910,13 +1951,14 @@ static int update_impl(ovl_ctx *c, const ovl_step_opts *o, int nsteps, bool obse
     if (e != hipSuccess) return fail(c, OVL_E_HIP, std::string("hipGraphLaunch: ") + hipGetErrorString(e));
     c->hop_launches = l0 + per_l * nsteps;
     c->hop_bytes = b0 + per_b * nsteps;
-    for (int v = 0; v < 4; ++v) {
+    for (int v = 0; v < NVAR; ++v) {
         c->var_launches[v] = vl0[v] + (c->var_launches[v] - vl0[v]) * nsteps;
         c->var_bytes[v] = vb0[v] + (c->var_bytes[v] - vb0[v]) * nsteps;
     }
     c->fp64_steps = s64 + p64 * nsteps;
     c->fp32_steps = s32 + p32 * nsteps;
     c->fp16_steps = s16 + p16 * nsteps;
+    c->fp48_steps = s48 + p48 * nsteps;
     return end_timing(c);
 }
 
@@ -2437,7 +2479,7 @@ static int trav_public(int t) { return t == ovl::TRAV_BOXES ? 2 : (t == ovl::TRA
 
 int ovl_get_tuning(ovl_ctx *c, int precision, ovl_tuning *t)
 {
-    if (!c || !t || precision < P64 || precision > P16) return OVL_E_ARG;
+    if (!c || !t || precision < 0 || precision >= NUM_PREC) return OVL_E_ARG;
     t->traversal = trav_public(c->trav[precision]);
     t->slabs = c->slabs[precision];
     t->nt_store = c->nt_out[precision];
@@ -2447,7 +2489,7 @@ int ovl_get_tuning(ovl_ctx *c, int precision, ovl_tuning *t)
 
 int ovl_set_tuning(ovl_ctx *c, int precision, const ovl_tuning *t)
 {
-    if (!c || !t || precision < P64 || precision > P16) return OVL_E_ARG;
+    if (!c || !t || precision < 0 || precision >= NUM_PREC) return OVL_E_ARG;
     if (t->traversal < 1 || t->traversal > 3 || t->slabs < 1 || t->waves_per_eu < 1 || t->waves_per_eu > 4 ||
         (t->nt_store != 0 && t->nt_store != 1))
         return fail(c, OVL_E_ARG, "tuning out of range");
@@ -2462,12 +2504,15 @@ int ovl_autotune(ovl_ctx *c, char *report, int len)
 {
     // Times the Clenshaw step (pure stencil + epilogue stencil) of every lane traversal (when the
     // context was created with traversal 0; y-slab counts of the mode split around the L2 estimate)
-    // and every occupancy variant of the stencil kernel, for both precisions, on this context's
-    // lattice and modes, and keeps the fastest per precision.  Field contents are left unchanged
-    // except the work buffers.
+    // and every occupancy variant of the stencil kernel, for every storage precision, on this
+    // context's lattice and modes, and keeps the fastest per precision.  Field contents are left
+    // unchanged except the work buffers.
     if (!c) return OVL_E_ARG;
     RETCHK(ensure_stencil(c));
-    RETCHK(ensure_work(c, c->g.nblk, true, true));
+    RETCHK(ensure_work(c, c->g.nblk, true, true, true));
+    HIPCHK(c, ovl::launch_convert(c->g, c->g.nblk, c->field[OVL_PSI], P64, c->x48, P48, c->st));
+    HIPCHK(c, ovl::launch_convert(c->g, c->g.nblk, c->field[OVL_PSIMID], P64, c->a48, P48, c->st));
+    HIPCHK(c, ovl::launch_convert(c->g, c->g.nblk, c->field[OVL_PSI], P64, c->b48, P48, c->st));
     HIPCHK(c, ovl::launch_convert(c->g, c->g.nblk, c->field[OVL_PSI], P64, c->x32, P32, c->st));
     HIPCHK(c, ovl::launch_convert(c->g, c->g.nblk, c->field[OVL_PSI], P64, c->x16, P16, c->st, 64.0));
     HIPCHK(c, ovl::launch_convert(c->g, c->g.nblk, c->field[OVL_PSIMID], P64, c->a16, P16, c->st, 64.0));
@@ -2479,13 +2524,13 @@ int ovl_autotune(ovl_ctx *c, char *report, int len)
     std::string rep;
     const bool prof = c->profiling;
     c->profiling = false;
-    static const char *pname[3] = {"fp64", "fp32", "fp16"};
+    static const char *pname[NUM_PREC] = {"fp64", "fp32", "fp16", "fp48"};
     auto trav_name = [](const ovl_ctx *cc, int prec) {
         if (cc->trav[prec] == ovl::TRAV_MODES) return std::string("modes/") + std::to_string(cc->slabs[prec]);
         if (cc->trav[prec] == ovl::TRAV_BOXES) return std::string("boxes/") + std::to_string(cc->slabs[prec]);
         return std::string("lex");
     };
-    for (int prec = P64; prec <= P16; ++prec) {
+    for (int prec = 0; prec < NUM_PREC; ++prec) {
         // candidate traversals (trav, slabs)
         std::vector<std::pair<int, int>> cand;
         if (c->ensemble || c->prm.traversal != 0) {
@@ -2497,10 +2542,13 @@ int ovl_autotune(ovl_ctx *c, char *report, int len)
             for (int t : {1, 2, 3, 4, 6})
                 if (t <= c->g.n1) cand.push_back({ovl::TRAV_MODES, t});
         }
-        void *x = prec == P64 ? c->field[OVL_PSI] : (prec == P32 ? c->x32 : c->x16);
-        void *b1 = prec == P64 ? c->a64 : (prec == P32 ? c->a32 : c->a16);
-        void *b2 = prec == P64 ? c->b64 : (prec == P32 ? c->b32 : c->b16);
-        void *w = prec == P64 ? c->w64 : (prec == P32 ? c->w32 : c->w16);
+        auto pick = [prec](void *p64, void *p32, void *p16, void *p48) {
+            return prec == P64 ? p64 : (prec == P32 ? p32 : (prec == P16 ? p16 : p48));
+        };
+        void *x = pick(c->field[OVL_PSI], c->x32, c->x16, c->x48);
+        void *b1 = pick(c->a64, c->a32, c->a16, c->a48);
+        void *b2 = pick(c->b64, c->b32, c->b16, c->b48);
+        void *w = pick(c->w64, c->w32, c->w16, c->w48);
         double best = 1e30;
         int best_v = 1, best_nt = 0;
         std::pair<int, int> best_t = cand[0];
@@ -2553,7 +2601,7 @@ int ovl_autotune(ovl_ctx *c, char *report, int len)
     }
     c->profiling = prof;
     rep += "chosen";
-    for (int prec = P64; prec <= P16; ++prec) {
+    for (int prec = 0; prec < NUM_PREC; ++prec) {
         char buf[96];
         std::snprintf(buf, sizeof buf, "%s %s %s%s wpe=%d", prec == P64 ? "" : ",", pname[prec],
                       trav_name(c, prec).c_str(), c->nt_out[prec] ? " nt" : "", c->wpe[prec]);

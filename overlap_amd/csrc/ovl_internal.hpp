@@ -72,14 +72,18 @@ constexpr int NUM_XCD = 8;
 constexpr int STENCIL_TPB = 256;
 
 // field storage precisions; FP16 fields hold values scaled by a power of two chosen by the host
-// (the stencil computes in FP32 and the scale is folded into the epilogue coefficients)
-enum Prec { P64 = 0, P32 = 1, P16 = 2 };
+// (the stencil computes in FP32 and the scale is folded into the epilogue coefficients).  P48
+// stores a double without its low 16 mantissa bits (sign, exponent, 36 mantissa bits, rounded to
+// nearest: unit roundoff 2^-37) and computes in FP64.
+enum Prec { P64 = 0, P32 = 1, P16 = 2, P48 = 3 };
+constexpr int NUM_PREC = 4;
 
 // launchers (ovl_kernels.hip); all enqueue on `st` and return hipError_t
 // wpe: occupancy variant of the stencil kernel (1 = compiler default, 2/3/4 = at least that many
 // waves per SIMD)
 hipError_t launch_hop(const Geom &g, int pin, int pout, const HopArgs &a, int wpe, hipStream_t st);
-// dst = scale * src between precisions: P64->P32, P32->P64, P64->P16, P16->P32
+// dst = scale * src between precisions: P64->P32, P32->P64, P64->P16, P16->P32, P64->P48,
+// P32->P48, P48->P64
 hipError_t launch_convert(const Geom &g, int nblk, const void *src, int psrc, void *dst, int pdst, hipStream_t st,
                           double scale = 1.0);
 // out = ca * a + cb * b (same precision p), b may be null

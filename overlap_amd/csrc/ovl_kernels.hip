@@ -14,25 +14,102 @@ namespace {
 
 constexpr int TPB = STENCIL_TPB;  // threads per block for the streaming kernels
 
+// 48-bit storage of a double: its top 48 bits (sign, exponent, 36 mantissa bits) after rounding
+// the dropped 16 bits to nearest.  Only a tag for the templates: the bits live in 16-byte units.
+struct f48 { unsigned short h[3]; };
+static_assert(sizeof(f48) == 6, "f48 tag size selects the 48-bit layout");
+
 // storage type -> arithmetic type (C) and stencil link type (L).  FP16 storage computes in FP32
-// with the FP32 links.
+// with the FP32 links, 48-bit storage in FP64 with the FP64 links.
 template <typename T> struct Vec;
 template <> struct Vec<double> { using C = double; using L = double2; };
 template <> struct Vec<float> { using C = float; using L = float2; };
 template <> struct Vec<_Float16> { using C = float; using L = float2; };
+template <> struct Vec<f48> { using C = double; using L = double2; };
 
 // 16 bytes of FP16 storage: 4 complex numbers
 typedef _Float16 h8v __attribute__((ext_vector_type(8)));
+typedef unsigned u4v __attribute__((ext_vector_type(4)));
+
+// 48-bit units: 4 complex (v0..v7 = re0, im0, .., re3, im3) take three 16-byte units holding
+// six 64-bit words W0..W5 (W0, W1 | W2, W3 | W4, W5).  W_i (i < 6) is v_i's sign, exponent and top
+// 36 mantissa bits in bits 16..63 and one 16-bit piece of v6 (W0..W2) or v7 (W3..W5) in bits 0..15:
+// the piece of W_i, read as the low mantissa bits of v_i, is accounted for when v_i's top bits are
+// rounded, so W_i read as a double IS v_i to within 2^-37 (no decoding).  v6 and v7 are their top
+// 48 bits (rounded to nearest) reassembled from the three pieces.
+__device__ __forceinline__ unsigned long long r48(double v)
+{
+    // top 48 bits rounded to nearest (+2^15 then truncation, on the sign-magnitude pattern)
+    return ((unsigned long long)__double_as_longlong(v) + 0x8000ull) & ~0xffffull;
+}
+__device__ __forceinline__ unsigned long long w48(double v, unsigned long long piece)
+{
+    // top 48 bits t of v such that t | piece is the nearest such pattern to v's magnitude
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned long long sgn = b & 0x8000000000000000ull, mag = b ^ sgn;
+    const unsigned long long t = (mag < 0x10000ull) ? 0ull : ((mag - piece + 0x8000ull) & ~0xffffull);
+    return sgn | t | piece;
+}
+__device__ __forceinline__ double w_as_double(unsigned lo, unsigned hi)
+{
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+// v6 / v7 from the low 16 bits of the three words that carry its pieces
+__device__ __forceinline__ double pieces_as_double(unsigned w0, unsigned w1, unsigned w2)
+{
+    return w_as_double(w0 << 16, (w2 << 16) | (w1 & 0xffffu));
+}
+template <typename C>
+__device__ __forceinline__ void pack48(const C *re, const C *im, int u, u4v &A, u4v &B, u4v &M)
+{
+    double v[8];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        v[2 * j] = (double)re[4 * u + j];
+        v[2 * j + 1] = (double)im[4 * u + j];
+    }
+    const unsigned long long r6 = r48(v[6]), r7 = r48(v[7]);
+    unsigned long long w[6];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        w[i] = w48(v[i], (r6 >> (16 * (i + 1))) & 0xffffull);
+        w[i + 3] = w48(v[i + 3], (r7 >> (16 * (i + 1))) & 0xffffull);
+    }
+    A = {(unsigned)w[0], (unsigned)(w[0] >> 32), (unsigned)w[1], (unsigned)(w[1] >> 32)};
+    B = {(unsigned)w[2], (unsigned)(w[2] >> 32), (unsigned)w[3], (unsigned)(w[3] >> 32)};
+    M = {(unsigned)w[4], (unsigned)(w[4] >> 32), (unsigned)w[5], (unsigned)(w[5] >> 32)};
+}
+template <typename C>
+__device__ __forceinline__ void unpack48(const u4v &A, const u4v &B, const u4v &M, int u, C *re, C *im)
+{
+    re[4 * u + 0] = w_as_double(A.x, A.y);
+    im[4 * u + 0] = w_as_double(A.z, A.w);
+    re[4 * u + 1] = w_as_double(B.x, B.y);
+    im[4 * u + 1] = w_as_double(B.z, B.w);
+    re[4 * u + 2] = w_as_double(M.x, M.y);
+    im[4 * u + 2] = w_as_double(M.z, M.w);
+    re[4 * u + 3] = pieces_as_double(A.x, A.z, B.x);
+    im[4 * u + 3] = pieces_as_double(B.z, M.x, M.z);
+}
 
 // ---------------------------------------------------------------------------------------------
 // spinor load / store for one lane
 // ---------------------------------------------------------------------------------------------
 // FP16 layout: 16-byte units of 4 complex, (blk, site, comp/4, m) at ((blk*V + site)*NCOMP/4 + comp/4)*MB + m
+// 48-bit layout: three 16-byte units j = A, B, M per 4 complex at
+//   (((blk*V + site)*NCOMP/4 + comp/4)*3 + j)*MB + m
 template <int NC, int MB, typename T, typename C>
 __device__ __forceinline__ void load_sp(const void *__restrict__ base, int V, int blk, int s, int m, C *re, C *im)
 {
     constexpr int NCOMP = 4 * NC;
-    if constexpr (sizeof(T) == 2) {
+    if constexpr (sizeof(T) == 6) {
+        const u4v *p = reinterpret_cast<const u4v *>(base) + ((size_t)(blk * V + s) * (NCOMP / 4)) * 3 * MB + m;
+#pragma unroll
+        for (int u = 0; u < NCOMP / 4; ++u) {
+            const u4v A = p[(size_t)(3 * u) * MB], B = p[(size_t)(3 * u + 1) * MB], M = p[(size_t)(3 * u + 2) * MB];
+            unpack48(A, B, M, u, re, im);
+        }
+    } else if constexpr (sizeof(T) == 2) {
         const h8v *p = reinterpret_cast<const h8v *>(base) + ((size_t)(blk * V + s) * (NCOMP / 4)) * MB + m;
 #pragma unroll
         for (int u = 0; u < NCOMP / 4; ++u) {
@@ -73,7 +150,16 @@ template <int NC, int MB, typename T, typename C>
 __device__ __forceinline__ void load_sp_nt(const void *__restrict__ base, int V, int blk, int s, int m, C *re, C *im)
 {
     constexpr int NCOMP = 4 * NC;
-    if constexpr (sizeof(T) == 2) {
+    if constexpr (sizeof(T) == 6) {
+        const u4v *p = reinterpret_cast<const u4v *>(base) + ((size_t)(blk * V + s) * (NCOMP / 4)) * 3 * MB + m;
+#pragma unroll
+        for (int u = 0; u < NCOMP / 4; ++u) {
+            const u4v A = __builtin_nontemporal_load(p + (size_t)(3 * u) * MB);
+            const u4v B = __builtin_nontemporal_load(p + (size_t)(3 * u + 1) * MB);
+            const u4v M = __builtin_nontemporal_load(p + (size_t)(3 * u + 2) * MB);
+            unpack48(A, B, M, u, re, im);
+        }
+    } else if constexpr (sizeof(T) == 2) {
         const h8v *p = reinterpret_cast<const h8v *>(base) + ((size_t)(blk * V + s) * (NCOMP / 4)) * MB + m;
 #pragma unroll
         for (int u = 0; u < NCOMP / 4; ++u) {
@@ -123,7 +209,17 @@ __device__ __forceinline__ void store_sp(void *__restrict__ base, int V, int blk
                                          const C *im)
 {
     constexpr int NCOMP = 4 * NC;
-    if constexpr (sizeof(T) == 2) {
+    if constexpr (sizeof(T) == 6) {
+        u4v *p = reinterpret_cast<u4v *>(base) + ((size_t)(blk * V + s) * (NCOMP / 4)) * 3 * MB + m;
+#pragma unroll
+        for (int u = 0; u < NCOMP / 4; ++u) {
+            u4v A, B, M;
+            pack48(re, im, u, A, B, M);
+            p[(size_t)(3 * u) * MB] = A;
+            p[(size_t)(3 * u + 1) * MB] = B;
+            p[(size_t)(3 * u + 2) * MB] = M;
+        }
+    } else if constexpr (sizeof(T) == 2) {
         h8v *p = reinterpret_cast<h8v *>(base) + ((size_t)(blk * V + s) * (NCOMP / 4)) * MB + m;
 #pragma unroll
         for (int u = 0; u < NCOMP / 4; ++u) p[(size_t)u * MB] = pack_h8<NC, MB, T>(re, im, u);
@@ -145,7 +241,17 @@ __device__ __forceinline__ void store_sp_nt(void *__restrict__ base, int V, int 
                                             const C *im)
 {
     constexpr int NCOMP = 4 * NC;
-    if constexpr (sizeof(T) == 2) {
+    if constexpr (sizeof(T) == 6) {
+        u4v *p = reinterpret_cast<u4v *>(base) + ((size_t)(blk * V + s) * (NCOMP / 4)) * 3 * MB + m;
+#pragma unroll
+        for (int u = 0; u < NCOMP / 4; ++u) {
+            u4v A, B, M;
+            pack48(re, im, u, A, B, M);
+            __builtin_nontemporal_store(A, p + (size_t)(3 * u) * MB);
+            __builtin_nontemporal_store(B, p + (size_t)(3 * u + 1) * MB);
+            __builtin_nontemporal_store(M, p + (size_t)(3 * u + 2) * MB);
+        }
+    } else if constexpr (sizeof(T) == 2) {
         h8v *p = reinterpret_cast<h8v *>(base) + ((size_t)(blk * V + s) * (NCOMP / 4)) * MB + m;
 #pragma unroll
         for (int u = 0; u < NCOMP / 4; ++u) __builtin_nontemporal_store(pack_h8<NC, MB, T>(re, im, u), p + (size_t)u * MB);
@@ -211,6 +317,11 @@ __device__ __forceinline__ bool lane_coords(const HopArgs &a, int V, int &m, int
 //   backward (sg=-1): g = u - i sigma_k l,  v = U^+ g,    X_u -= v, X_l -= i sigma_k v
 // which is -(1 +- i gamma_k) U phi  in the Dirac basis of DiracAlgebra.cpp:5-9.
 // ---------------------------------------------------------------------------------------------
+// a * b + c rounded once: the complex multiply-accumulates of the colour mat-vec as two FMAs per
+// real part (the compiler does not contract a sum of two products into a running sum by itself)
+__device__ __forceinline__ double fmac(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float fmac(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
 template <int NC, int K, int SG, typename T, typename LT>
 __device__ __forceinline__ void hop_dir(const T *pr, const T *pi, const LT *__restrict__ U, T *xr, T *xi)
 {
@@ -244,13 +355,17 @@ __device__ __forceinline__ void hop_dir(const T *pr, const T *pi, const LT *__re
                 if constexpr (SG > 0) {
                     const LT u = U[i * NC + j];
                     const T ur = (T)u.x, ui = (T)u.y;
-                    ar += ur * hr[s][j] - ui * hi[s][j];
-                    ai += ur * hi[s][j] + ui * hr[s][j];
+                    ar = fmac(ur, hr[s][j], ar);
+                    ar = fmac(-ui, hi[s][j], ar);
+                    ai = fmac(ur, hi[s][j], ai);
+                    ai = fmac(ui, hr[s][j], ai);
                 } else {
                     const LT u = U[j * NC + i];  // (U^+)_{ij} = conj(U_{ji})
                     const T ur = (T)u.x, ui = (T)u.y;
-                    ar += ur * hr[s][j] + ui * hi[s][j];
-                    ai += ur * hi[s][j] - ui * hr[s][j];
+                    ar = fmac(ur, hr[s][j], ar);
+                    ar = fmac(ui, hi[s][j], ar);
+                    ai = fmac(ur, hi[s][j], ai);
+                    ai = fmac(-ui, hr[s][j], ai);
                 }
             }
             vr[s][i] = ar;
@@ -471,6 +586,90 @@ __global__ __launch_bounds__(TPB) void k_h2f(const h8v *__restrict__ src, float4
     dst[(2 * q) * mb + m] = make_float4(scale * (float)t[0], scale * (float)t[1], scale * (float)t[2], scale * (float)t[3]);
     dst[(2 * q + 1) * mb + m] =
         make_float4(scale * (float)t[4], scale * (float)t[5], scale * (float)t[6], scale * (float)t[7]);
+}
+
+// one group of 4 complex numbers (group q of the FP64 numbering: complex 4q..4q+3 of a mode
+// block's (site, comp) sequence, mode m) in any of the four layouts, as doubles
+template <typename T>
+__device__ __forceinline__ void load_group(const void *__restrict__ base, long q, long m, int mb, double *re, double *im)
+{
+    if constexpr (sizeof(T) == 6) {
+        const u4v *p = reinterpret_cast<const u4v *>(base);
+        unpack48(p[(3 * q) * mb + m], p[(3 * q + 1) * mb + m], p[(3 * q + 2) * mb + m], 0, re, im);
+    } else if constexpr (sizeof(T) == 8) {
+        const double2 *p = reinterpret_cast<const double2 *>(base);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const double2 t = p[(4 * q + j) * mb + m];
+            re[j] = t.x;
+            im[j] = t.y;
+        }
+    } else if constexpr (sizeof(T) == 4) {
+        const float4 *p = reinterpret_cast<const float4 *>(base);
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const float4 t = p[(2 * q + j) * mb + m];
+            re[2 * j] = t.x;
+            im[2 * j] = t.y;
+            re[2 * j + 1] = t.z;
+            im[2 * j + 1] = t.w;
+        }
+    } else {
+        const h8v t = reinterpret_cast<const h8v *>(base)[q * mb + m];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            re[j] = (double)t[2 * j];
+            im[j] = (double)t[2 * j + 1];
+        }
+    }
+}
+template <typename T>
+__device__ __forceinline__ void store_group(void *__restrict__ base, long q, long m, int mb, const double *re,
+                                            const double *im)
+{
+    if constexpr (sizeof(T) == 6) {
+        u4v A, B, M;
+        pack48(re, im, 0, A, B, M);
+        u4v *p = reinterpret_cast<u4v *>(base);
+        p[(3 * q) * mb + m] = A;
+        p[(3 * q + 1) * mb + m] = B;
+        p[(3 * q + 2) * mb + m] = M;
+    } else if constexpr (sizeof(T) == 8) {
+        double2 *p = reinterpret_cast<double2 *>(base);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) p[(4 * q + j) * mb + m] = make_double2(re[j], im[j]);
+    } else if constexpr (sizeof(T) == 4) {
+        float4 *p = reinterpret_cast<float4 *>(base);
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+            p[(2 * q + j) * mb + m] =
+                make_float4((float)re[2 * j], (float)im[2 * j], (float)re[2 * j + 1], (float)im[2 * j + 1]);
+    } else {
+        h8v v;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            v[2 * j] = (_Float16)re[j];
+            v[2 * j + 1] = (_Float16)im[j];
+        }
+        reinterpret_cast<h8v *>(base)[q * mb + m] = v;
+    }
+}
+// dst = scale * src, one (group, mode) per lane (the conversions to and from the 48-bit layout)
+template <typename TS, typename TD>
+__global__ __launch_bounds__(TPB) void k_convert(const void *__restrict__ src, void *__restrict__ dst, long nlanes,
+                                                 int mb, double scale)
+{
+    const long i = (long)blockIdx.x * TPB + threadIdx.x;
+    if (i >= nlanes) return;
+    const long m = i % mb, q = i / mb;
+    double re[4], im[4];
+    load_group<TS>(src, q, m, mb, re, im);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        re[j] *= scale;
+        im[j] *= scale;
+    }
+    store_group<TD>(dst, q, m, mb, re, im);
 }
 
 template <typename T>
@@ -1353,11 +1552,13 @@ hipError_t launch_hop(const Geom &g, int pin, int pout, const HopArgs &a, int wp
         if (pin == P32 && pout == P32) return hop_mb<2, float, float>(g, a, wpe, st);
         if (pin == P32 && pout == P64) return hop_mb<2, float, double>(g, a, wpe, st);
         if (pin == P16 && pout == P16) return hop_mb<2, _Float16, _Float16>(g, a, wpe, st);
+        if (pin == P48 && pout == P48) return hop_mb<2, f48, f48>(g, a, wpe, st);
     } else if (g.nc == 3) {
         if (pin == P64 && pout == P64) return hop_mb<3, double, double>(g, a, wpe, st);
         if (pin == P32 && pout == P32) return hop_mb<3, float, float>(g, a, wpe, st);
         if (pin == P32 && pout == P64) return hop_mb<3, float, double>(g, a, wpe, st);
         if (pin == P16 && pout == P16) return hop_mb<3, _Float16, _Float16>(g, a, wpe, st);
+        if (pin == P48 && pout == P48) return hop_mb<3, f48, f48>(g, a, wpe, st);
     }
     return hipErrorInvalidValue;
 }
@@ -1375,6 +1576,12 @@ hipError_t launch_convert(const Geom &g, int nblk, const void *src, int psrc, vo
         k_d2h<<<nblocks(nunits), TPB, 0, st>>>((const double2 *)src, (h8v *)dst, nunits, g.mb, scale);
     else if (psrc == P16 && pdst == P32)
         k_h2f<<<nblocks(nunits), TPB, 0, st>>>((const h8v *)src, (float4 *)dst, nunits, g.mb, (float)scale);
+    else if (psrc == P64 && pdst == P48)
+        k_convert<double, f48><<<nblocks(nunits), TPB, 0, st>>>(src, dst, nunits, g.mb, scale);
+    else if (psrc == P32 && pdst == P48)
+        k_convert<float, f48><<<nblocks(nunits), TPB, 0, st>>>(src, dst, nunits, g.mb, scale);
+    else if (psrc == P48 && pdst == P64)
+        k_convert<f48, double><<<nblocks(nunits), TPB, 0, st>>>(src, dst, nunits, g.mb, scale);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();

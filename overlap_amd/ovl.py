@@ -53,7 +53,8 @@ class EnsembleParams(C.Structure):
 class SignOpts(C.Structure):
     _fields_ = [("method", C.c_int), ("precision", C.c_int), ("degree", C.c_int), ("cut", C.c_double),
                 ("tol", C.c_double), ("head", C.c_int), ("lmin", C.c_double), ("lmax", C.c_double),
-                ("mode_chunk", C.c_int), ("fp16_tail", C.c_int), ("head16", C.c_int)]
+                ("mode_chunk", C.c_int), ("fp16_tail", C.c_int), ("head16", C.c_int),
+                ("fp48", C.c_int), ("head48", C.c_int)]
 
 
 class StepOpts(C.Structure):
@@ -74,8 +75,8 @@ class Observe(C.Structure):
 class Timing(C.Structure):
     _fields_ = [("total_ms", C.c_double), ("hop_ms", C.c_double), ("hop_launches", C.c_longlong),
                 ("hop_bytes", C.c_double), ("fp64_steps", C.c_int), ("fp32_steps", C.c_int),
-                ("fp16_steps", C.c_int), ("var_ms", C.c_double * 4), ("var_launches", C.c_longlong * 4),
-                ("var_bytes", C.c_double * 4)]
+                ("fp16_steps", C.c_int), ("fp48_steps", C.c_int), ("var_ms", C.c_double * 5),
+                ("var_launches", C.c_longlong * 5), ("var_bytes", C.c_double * 5)]
 
 
 _lib = None
@@ -118,12 +119,13 @@ def sign_opts(**kw) -> SignOpts:
 
 
 def sign_plan(opts: SignOpts):
-    """(head, head16): steps k < head run in FP64, head <= k < head16 in FP32, the rest in FP16."""
-    h, h16 = C.c_int(), C.c_int()
-    rc = lib().ovl_sign_plan(C.byref(opts), C.byref(h), C.byref(h16))
+    """(head48, head, head16): steps k < head48 run in FP64, head48 <= k < head in 48-bit storage,
+    head <= k < head16 in FP32, the rest in FP16 storage."""
+    h48, h, h16 = C.c_int(), C.c_int(), C.c_int()
+    rc = lib().ovl_sign_plan(C.byref(opts), C.byref(h48), C.byref(h), C.byref(h16))
     if rc != OK:
         raise OvlError(rc, "invalid sign options")
-    return h.value, h16.value
+    return h48.value, h.value, h16.value
 
 
 def zolotarev_coefficients(amin, amax, errtol=1e-12, no=25):

@@ -83,10 +83,11 @@ def test_create_reports_missing_device():
     assert e.value.code == ovl.E_HIP
 
 
-def _tier_starts_numpy(deg, cut, tol, fp16):
+def _tier_starts_numpy(deg, cut, tol, fp16, fp48=True):
     # the precision ladder restated in numpy: B_k = max_t |b_k(t)| of the scalar Clenshaw
     # recurrence of the Chebyshev coefficients (as orc.cheb_coeffs computes them) on t in [-1, 1];
-    # a tier with unit roundoff u starts at the smallest k whose whole tail has 2 u B_k <= its share
+    # a tier with unit roundoff u starts at the smallest k whose whole tail has 2 u B_k <= its
+    # share of tol: 1/4 each to the FP16 and 48-bit tiers when they are on, the rest to FP32
     import orc
     c = orc.cheb_coeffs(deg, cut)
     t = np.cos(np.pi * np.arange(2049) / 2048)
@@ -105,22 +106,34 @@ def _tier_starts_numpy(deg, cut, tol, fp16):
                 break
             first = k
         return first
-    head = start(5.96e-8, 0.75 * tol if fp16 else tol)
-    head16 = max(start(4.89e-4, 0.25 * tol), head + 1) if fp16 else deg + 1
-    return head, head16
+    s16, s48 = (0.25 if fp16 else 0.0), (0.25 if fp48 else 0.0)
+    head = start(2.0 ** -24, (1 - s16 - s48) * tol)
+    head16 = min(max(start(2.0 ** -11, s16 * tol), head + 1), deg + 1) if fp16 else deg + 1
+    head48 = min(max(start(2.0 ** -37, s48 * tol), 1), head) if fp48 else head
+    return head48, head, head16
 
 
 def test_sign_plan_host_only():
     o = ovl.sign_opts(degree=384, cut=0.05, precision=ovl.PREC_MIXED, tol=1e-10)
-    assert ovl.sign_plan(o) == (113, 211)  # the ladder of the 24^3 SU(3) bench runs
+    assert ovl.sign_plan(o) == (38, 117, 211)  # the ladder of the 24^3 SU(3) bench runs
     assert ovl.sign_plan(o) == _tier_starts_numpy(384, 0.05, 1e-10, True)
+    o.fp48 = 0
+    assert ovl.sign_plan(o) == (113, 113, 211) == _tier_starts_numpy(384, 0.05, 1e-10, True, False)
     o.fp16_tail = 0
-    assert ovl.sign_plan(o) == (110, 385) == _tier_starts_numpy(384, 0.05, 1e-10, False)
+    assert ovl.sign_plan(o) == (110, 110, 385) == _tier_starts_numpy(384, 0.05, 1e-10, False, False)
+    o.fp48 = 1
+    assert ovl.sign_plan(o) == _tier_starts_numpy(384, 0.05, 1e-10, False, True)
     o = ovl.sign_opts(degree=256, cut=0.05, precision=ovl.PREC_MIXED, tol=1e-11)
     assert ovl.sign_plan(o) == _tier_starts_numpy(256, 0.05, 1e-11, True)
-    assert ovl.sign_plan(ovl.sign_opts(degree=64, precision=ovl.PREC_FP64)) == (65, 65)
-    assert ovl.sign_plan(ovl.sign_opts(degree=64, precision=ovl.PREC_FP32)) == (0, 65)
+    assert ovl.sign_plan(ovl.sign_opts(degree=64, precision=ovl.PREC_FP64)) == (65, 65, 65)
+    assert ovl.sign_plan(ovl.sign_opts(degree=64, precision=ovl.PREC_FP32)) == (0, 0, 65)
     o = ovl.sign_opts(degree=64, precision=ovl.PREC_MIXED, head=4, head16=9)
-    assert ovl.sign_plan(o) == (4, 9)
+    assert ovl.sign_plan(o)[1:] == (4, 9)
+    o = ovl.sign_opts(degree=64, precision=ovl.PREC_MIXED, head=6, head16=9, head48=2)
+    assert ovl.sign_plan(o) == (2, 6, 9)
+    o.head48 = 20  # a 48-bit start above the FP32 start leaves no 48-bit steps
+    assert ovl.sign_plan(o) == (6, 6, 9)
+    o = ovl.sign_opts(degree=64, precision=ovl.PREC_MIXED, head=65, head48=1)  # all but k = 0 in 48 bits
+    assert ovl.sign_plan(o) == (1, 65, 65)
     with pytest.raises(ovl.OvlError):
         ovl.sign_plan(ovl.sign_opts(cut=1.5))

@@ -136,9 +136,11 @@ def test_mixed_precision_sign_reaches_1e10(gpu, nc, nb):
     t = ctx.timing()
     mix = ctx.get_modes(ovl.SCRATCH)
     err = max(np.linalg.norm(mix[k] - ref[k]) / np.linalg.norm(psi[k]) for k in range(64))
-    print(f"[mixed] nc={nc} steps fp64 {t.fp64_steps} fp32 {t.fp32_steps} fp16 {t.fp16_steps} err {err:.3e}",
-          flush=True)
-    assert t.fp32_steps > 0 and t.fp16_steps > 0 and t.fp64_steps < deg + 1
+    print(f"[mixed] nc={nc} steps fp64 {t.fp64_steps} 48-bit {t.fp48_steps} fp32 {t.fp32_steps} "
+          f"fp16 {t.fp16_steps} err {err:.3e}", flush=True)
+    assert t.fp32_steps > 0 and t.fp16_steps > 0 and t.fp48_steps > 0 and t.fp64_steps < deg + 1
+    assert (t.fp64_steps, t.fp48_steps, t.fp32_steps, t.fp16_steps) == tuple(
+        np.diff((0,) + ovl.sign_plan(om) + (deg + 1,)))
     assert err <= 1e-10
     # without the FP16 tail: FP64 head + FP32 rest
     om.fp16_tail = 0
@@ -159,10 +161,10 @@ def test_fp16_tier_vs_oracle(gpu, nc, nb):
     psi[3] *= 20.0  # modes of different norms share the scale of the largest
     ctx.set_modes(ovl.PSI, psi)
     deg, cut = 48, 0.1
-    o = ovl.sign_opts(degree=deg, cut=cut, precision=ovl.PREC_MIXED, head=4, head16=9)
+    o = ovl.sign_opts(degree=deg, cut=cut, precision=ovl.PREC_MIXED, head=4, head16=9, fp48=0)
     ctx.apply_sign(ovl.PSI, ovl.PSIMID, o)
     t = ctx.timing()
-    assert (t.fp64_steps, t.fp32_steps, t.fp16_steps) == (4, 5, deg + 1 - 9)
+    assert (t.fp64_steps, t.fp48_steps, t.fp32_steps, t.fp16_steps) == (4, 0, 5, deg + 1 - 9)
     out = ctx.get_modes(ovl.PSIMID)
     coef = orc.cheb_coeffs(deg, cut)
     lam = orc.norm_bound(L)
@@ -172,6 +174,36 @@ def test_fp16_tier_vs_oracle(gpu, nc, nb):
     assert max(errs) < 2e-2 and np.isfinite(out).all()
     # the FP16 variant is timed on its own
     assert t.var_launches[3] == 2 * (deg - 9) and t.var_launches[1] > 0  # b_n = c_n x needs no stencil
+
+
+@pytest.mark.parametrize("nc,nb,mb", [(2, 2, 16), (3, 5, 64)])
+def test_fp48_tier_vs_oracle(gpu, nc, nb, mb):
+    # every step but the last (k = 0) in 48-bit storage: the result carries the 48-bit rounding
+    # (u = 2^-37 times the size of the intermediates), the FP64 form does not; the two differ
+    # but stay far inside 1e-9, and the oracle agrees with the FP64 form to 1e-12
+    print(f"[case] 48-bit tier nc={nc} nb={nb} mb={mb}", flush=True)
+    n = (4, 4, 3)
+    ctx, L, ucc, *_, psi = make(nc, n, 16, 70 + nc, M=1.2, nb=nb, mb=mb)
+    psi[5] *= 1e3  # the 48-bit layout keeps the FP64 exponent range: no scales
+    ctx.set_modes(ovl.PSI, psi)
+    deg, cut = 64, 0.1
+    o = ovl.sign_opts(degree=deg, cut=cut, precision=ovl.PREC_MIXED, head=deg + 1, head48=1)
+    assert ovl.sign_plan(o) == (1, deg + 1, deg + 1)
+    ctx.apply_sign(ovl.PSI, ovl.PSIMID, o)
+    t = ctx.timing()
+    assert (t.fp64_steps, t.fp48_steps, t.fp32_steps, t.fp16_steps) == (1, deg, 0, 0)
+    assert t.var_launches[4] == 2 * (deg - 1) and t.var_launches[0] == 3  # b_n = c_n x needs no stencil
+    out = ctx.get_modes(ovl.PSIMID)
+    ctx.apply_sign(ovl.PSI, ovl.SCRATCH, ovl.sign_opts(degree=deg, cut=cut, precision=ovl.PREC_FP64))
+    ref = ctx.get_modes(ovl.SCRATCH)
+    coef = orc.cheb_coeffs(deg, cut)
+    lam = orc.norm_bound(L)
+    e48 = [np.linalg.norm(out[k] - ref[k]) / np.linalg.norm(psi[k]) for k in range(16)]
+    print(f"  48-bit tier vs FP64: max {max(e48):.2e} min {min(e48):.2e}", flush=True)
+    assert 0 < max(e48) < 1e-9
+    for k in (0, 5, 15):
+        assert rel(ref[k], orc.sign_cheb(L, ucc, lam, cut, deg, coef, psi[k])) < 1e-12
+        assert np.linalg.norm(out[k] - orc.sign_cheb(L, ucc, lam, cut, deg, coef, psi[k])) / np.linalg.norm(psi[k]) < 1e-9
 
 
 def test_sign_residual_and_mode_chunks(gpu):
