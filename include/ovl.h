@@ -271,7 +271,8 @@ int ovl_autotune(ovl_ctx *ctx, char *report, int len);
 /* the stencil kernel's tuning per storage precision (0 = FP64, 1 = FP32, 2 = FP16, 3 = 48-bit), as
  * ovl_autotune leaves it: lane traversal (1 lexicographic, 2 XCD boxes, 3 XCD mode split; see
  * ovl_params.traversal), y slabs of the traversal, streaming output stores (0/1), occupancy
- * variant (minimum waves per SIMD, 1-4).  Setting one replays a saved tuning without the sweep. */
+ * variant (minimum waves per SIMD, 1-4; 5 = one workgroup per CU through an LDS reservation, where
+ * the device allows 96 KiB per workgroup).  Setting one replays a saved tuning without the sweep. */
 typedef struct {
     int traversal;
     int slabs;

@@ -2490,7 +2490,7 @@ int ovl_get_tuning(ovl_ctx *c, int precision, ovl_tuning *t)
 int ovl_set_tuning(ovl_ctx *c, int precision, const ovl_tuning *t)
 {
     if (!c || !t || precision < 0 || precision >= NUM_PREC) return OVL_E_ARG;
-    if (t->traversal < 1 || t->traversal > 3 || t->slabs < 1 || t->waves_per_eu < 1 || t->waves_per_eu > 4 ||
+    if (t->traversal < 1 || t->traversal > 3 || t->slabs < 1 || t->waves_per_eu < 1 || t->waves_per_eu > 5 ||
         (t->nt_store != 0 && t->nt_store != 1))
         return fail(c, OVL_E_ARG, "tuning out of range");
     const int trav = t->traversal == 2 ? ovl::TRAV_BOXES : (t->traversal == 3 ? ovl::TRAV_MODES : ovl::TRAV_LEX);
@@ -2525,6 +2525,11 @@ int ovl_autotune(ovl_ctx *c, char *report, int len)
     const bool prof = c->profiling;
     c->profiling = false;
     static const char *pname[NUM_PREC] = {"fp64", "fp32", "fp16", "fp48"};
+    // occupancy variant 5 reserves 96 KiB of LDS per workgroup (one workgroup per CU)
+    int dev = 0, max_lds = 0;
+    HIPCHK(c, hipGetDevice(&dev));
+    HIPCHK(c, hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    const bool one_wg = max_lds >= 96 * 1024;
     auto trav_name = [](const ovl_ctx *cc, int prec) {
         if (cc->trav[prec] == ovl::TRAV_MODES) return std::string("modes/") + std::to_string(cc->slabs[prec]);
         if (cc->trav[prec] == ovl::TRAV_BOXES) return std::string("boxes/") + std::to_string(cc->slabs[prec]);
@@ -2555,7 +2560,7 @@ int ovl_autotune(ovl_ctx *c, char *report, int len)
         for (const auto &tr : cand) {
             RETCHK(set_traversal(c, prec, tr.first, tr.second));
             for (int nt = 0; nt <= 1; ++nt)
-                for (int v = 1; v <= 4; ++v) {
+                for (int v = 1; v <= (one_wg ? 5 : 4); ++v) {
                     c->wpe[prec] = v;
                     c->nt_out[prec] = nt;
                     float ms = 0;

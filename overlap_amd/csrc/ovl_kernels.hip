@@ -7,6 +7,8 @@
 #include "ovl_internal.hpp"
 
 #include <cmath>
+#include <mutex>
+#include <unordered_set>
 
 namespace ovl {
 
@@ -1488,6 +1490,19 @@ __global__ __launch_bounds__(TPB) void k_axpy_modes(double2 *__restrict__ y, con
 
 inline unsigned nblocks(long n) { return (unsigned)((n + TPB - 1) / TPB); }
 
+// occupancy variant 5: the wpe = 1 kernel with an LDS reservation it does not use, larger than
+// half of a CU's 160 KiB, so that one workgroup (one wave per SIMD) runs per CU: fewer sites in
+// flight per XCD, so the neighbour planes of the stencil input fit in L2 beside them
+constexpr unsigned LDS_ONE_WG = 96 * 1024;
+hipError_t reserve_lds(const void *kernel)
+{
+    static std::mutex mu;
+    static std::unordered_set<const void *> done;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!done.insert(kernel).second) return hipSuccess;
+    return hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, LDS_ONE_WG);
+}
+
 template <int NC, typename TI, typename TO, int MB>
 void hop_wpe(const HopArgs &b, unsigned nb, int wpe, const Geom &g, hipStream_t st)
 {
@@ -1495,6 +1510,10 @@ void hop_wpe(const HopArgs &b, unsigned nb, int wpe, const Geom &g, hipStream_t 
     case 2: k_hop<NC, TI, TO, MB, 2><<<nb, TPB, 0, st>>>(b, g.n0, g.n1, g.n2); break;
     case 3: k_hop<NC, TI, TO, MB, 3><<<nb, TPB, 0, st>>>(b, g.n0, g.n1, g.n2); break;
     case 4: k_hop<NC, TI, TO, MB, 4><<<nb, TPB, 0, st>>>(b, g.n0, g.n1, g.n2); break;
+    case 5:
+        reserve_lds(reinterpret_cast<const void *>(k_hop<NC, TI, TO, MB, 1>));
+        k_hop<NC, TI, TO, MB, 1><<<nb, TPB, LDS_ONE_WG, st>>>(b, g.n0, g.n1, g.n2);
+        break;
     default: k_hop<NC, TI, TO, MB, 1><<<nb, TPB, 0, st>>>(b, g.n0, g.n1, g.n2); break;
     }
 }
@@ -1507,6 +1526,10 @@ void hop_ens_wpe(const HopArgs &b, int wpe, hipStream_t st)
     case 2: k_hop_ens<NC, TI, TO, MB, 2><<<nb, TPB, 0, st>>>(b); break;
     case 3: k_hop_ens<NC, TI, TO, MB, 3><<<nb, TPB, 0, st>>>(b); break;
     case 4: k_hop_ens<NC, TI, TO, MB, 4><<<nb, TPB, 0, st>>>(b); break;
+    case 5:
+        reserve_lds(reinterpret_cast<const void *>(k_hop_ens<NC, TI, TO, MB, 1>));
+        k_hop_ens<NC, TI, TO, MB, 1><<<nb, TPB, LDS_ONE_WG, st>>>(b);
+        break;
     default: k_hop_ens<NC, TI, TO, MB, 1><<<nb, TPB, 0, st>>>(b); break;
     }
 }

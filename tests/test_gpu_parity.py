@@ -373,6 +373,29 @@ def test_traversals_are_permutations(gpu, nc, n, nm, mb):
     assert err < 1e-5
 
 
+@pytest.mark.parametrize("nc,mb", [(3, 64), (2, 16)])
+def test_occupancy_variants_are_bit_identical(gpu, nc, mb):
+    # occupancy variants 1-5 (5: one workgroup per CU through an LDS reservation) run the same
+    # per-lane code: every tier of the recurrence gives the same bits under each
+    print(f"[case] occupancy variants nc={nc} mb={mb}", flush=True)
+    n = (4, 4, 8)
+    ctx, L, ucc, *_, psi = make(nc, n, 64, 505 + nc, M=1.2, nb=2, mb=mb)
+    o = ovl.sign_opts(degree=32, cut=0.1, precision=ovl.PREC_MIXED, head48=2, head=8, head16=20)
+    assert ovl.sign_plan(o) == (2, 8, 20)
+    outs = []
+    for v in (1, 2, 3, 4, 5):
+        for prec in range(4):
+            t = ctx.get_tuning(prec)
+            ctx.set_tuning(prec, t.traversal, t.slabs, t.nt_store, v)
+            assert ctx.get_tuning(prec).waves_per_eu == v
+        ctx.apply_sign(ovl.PSI, ovl.PSIMID, o)
+        outs.append(ctx.get_modes(ovl.PSIMID))
+    for x in outs[1:]:
+        assert np.array_equal(x, outs[0])
+    with pytest.raises(ovl.OvlError):
+        ctx.set_tuning(0, 1, 1, 1, 6)
+
+
 @pytest.mark.parametrize("nc", [2, 3])
 def test_gauss_restore_vs_oracle(gpu, nc):
     print(f"[case] gauss restore nc={nc}", flush=True)
