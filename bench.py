@@ -409,7 +409,7 @@ def main():
 
     out = {"metric": METRIC, "value": value, "unit": "site*mode updates/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
-           "vs_baseline": None, "dtype": "fp64+fp32+fp16-storage (mixed)", "data": "synthetic",
+           "vs_baseline": None, "dtype": "fp64+48-bit+fp32+fp16-storage (mixed)", "data": "synthetic",
            "config": {"workload": c.name, "nc": c.nc, "lattice": list(c.n), "modes": c.nmodes,
                       "chebyshev_degree": c.cheb_n, "cut": c.cheb_cut, "rho": c.m_dw, "flux_quanta": c.nb,
                       "dt": c.dt, "tol": args.tol, "mode_block": args.mb, "mode_chunk": args.chunk,

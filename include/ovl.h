@@ -70,8 +70,9 @@ typedef struct {
     int precision;       /* OVL_PREC_FP64 / OVL_PREC_FP32 / OVL_PREC_MIXED (Chebyshev only) */
     int degree;          /* Chebyshev degree n of P(x) ~ x^-1/2 */
     double cut;          /* spectrum cut: approximation on |h_W| in [cut, 1] * lambda_max */
-    double tol;          /* MIXED: target relative error; selects the FP64 head length */
-    int head;            /* MIXED: FP64 head length (>= 0 overrides tol) */
+    double tol;          /* MIXED: target relative error; selects where each storage tier starts */
+    int head;            /* MIXED: first FP32 step, i.e. the length of the FP64 + 48-bit head
+                            (>= 0 overrides tol) */
     double lmin, lmax;   /* ZOLOTAREV: interval of h_W^2; lmin = lmax = 0 -> Ritz estimate */
     int mode_chunk;      /* mode blocks per recurrence pass (0 = all) */
     int fp16_tail;       /* MIXED: 1 = store the far tail of the recurrence in FP16 (scaled by powers

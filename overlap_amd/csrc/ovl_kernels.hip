@@ -1511,7 +1511,7 @@ void hop_wpe(const HopArgs &b, unsigned nb, int wpe, const Geom &g, hipStream_t 
     case 3: k_hop<NC, TI, TO, MB, 3><<<nb, TPB, 0, st>>>(b, g.n0, g.n1, g.n2); break;
     case 4: k_hop<NC, TI, TO, MB, 4><<<nb, TPB, 0, st>>>(b, g.n0, g.n1, g.n2); break;
     case 5:
-        reserve_lds(reinterpret_cast<const void *>(k_hop<NC, TI, TO, MB, 1>));
+        (void)reserve_lds(reinterpret_cast<const void *>(k_hop<NC, TI, TO, MB, 1>));
         k_hop<NC, TI, TO, MB, 1><<<nb, TPB, LDS_ONE_WG, st>>>(b, g.n0, g.n1, g.n2);
         break;
     default: k_hop<NC, TI, TO, MB, 1><<<nb, TPB, 0, st>>>(b, g.n0, g.n1, g.n2); break;
@@ -1527,7 +1527,7 @@ void hop_ens_wpe(const HopArgs &b, int wpe, hipStream_t st)
     case 3: k_hop_ens<NC, TI, TO, MB, 3><<<nb, TPB, 0, st>>>(b); break;
     case 4: k_hop_ens<NC, TI, TO, MB, 4><<<nb, TPB, 0, st>>>(b); break;
     case 5:
-        reserve_lds(reinterpret_cast<const void *>(k_hop_ens<NC, TI, TO, MB, 1>));
+        (void)reserve_lds(reinterpret_cast<const void *>(k_hop_ens<NC, TI, TO, MB, 1>));
         k_hop_ens<NC, TI, TO, MB, 1><<<nb, TPB, LDS_ONE_WG, st>>>(b);
         break;
     default: k_hop_ens<NC, TI, TO, MB, 1><<<nb, TPB, 0, st>>>(b); break;
