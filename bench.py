@@ -206,6 +206,54 @@ def bench_ensemble(ec: cf.EnsembleConfig, steps: int, warmup: int, tol: float):
     return res
 
 
+VARIANT_NAMES = ["k_hop FP64", "k_hop FP32", "k_hop FP32->FP64", "k_hop FP16", "k_hop 48-bit"]
+
+
+def bench_other_workloads(tol: float):
+    """The other three single-lattice workloads, timed the same way as the main one (inputs
+    resident, autotuned stencil, HIP events around every stencil launch): Wilson leapfrogs in
+    FP64, the SU(2) overlap in the mixed form.  Their cost does not depend on the mode values, so
+    only the first 64 modes are drawn and repeated to fill the mode count."""
+    out = {}
+    for name, steps in (("su2_wilson_8", 20), ("su2_overlap_16", 2), ("su3_wilson_32", 3)):
+        c = cf.CONFIGS[name]
+        nm = min(c.nmodes, 64)
+        links, e, psi = c.generate(nm)
+        ctx = ovl.Context(c.nc, c.n, c.nmodes, r_wilson=c.r_wilson, m_fermion=c.m_fermion, m_dw=c.m_dw)
+        ctx.set_links(links)
+        ctx.set_efield(e)
+        if c.nb:
+            ctx.set_u1_flux(c.nb, c.q)
+        for f in range(0, c.nmodes, nm):
+            k = min(nm, c.nmodes - f)
+            ctx.set_modes(ovl.PSI, psi[:k], first=f)
+            ctx.set_modes(ovl.PSIMID, psi[:k], first=f)
+        del psi
+        ctx.autotune()
+        prec = ovl.PREC_MIXED if c.kind == "overlap" else ovl.PREC_FP64
+        so = step_opts(c, prec, False, tol)
+        ctx.fermion_update(so, 1)
+        ctx.set_profiling(True)
+        t0 = time.perf_counter()
+        ctx.fermion_update(so, steps)
+        wall = time.perf_counter() - t0
+        ctx.set_profiling(False)
+        t = ctx.timing()
+        var = int(np.argmax([t.var_ms[i] for i in range(5)]))
+        gbps = t.var_bytes[var] / max(t.var_ms[var], 1e-9) / 1e6
+        out[name] = {"lattice": list(c.n), "nc": c.nc, "modes": c.nmodes, "kind": c.kind, "steps": steps,
+                     "precision": "mixed" if prec == ovl.PREC_MIXED else "fp64",
+                     "ms_per_step": round(1e3 * wall / steps, 3),
+                     "value": c.volume * c.nmodes * steps / wall, "unit": "site*mode updates/s",
+                     "kernel": VARIANT_NAMES[var], "kernel_GBps": round(gbps, 1),
+                     "kernel_frac_of_8TBps": round(gbps / HBM_PEAK_GBS, 4),
+                     "kernel_share_of_step": round(t.var_ms[var] / max(t.total_ms, 1e-9), 3)}
+        log(f"{name}: {out[name]['ms_per_step']:.2f} ms/step {out[name]['value']:.4e} site*mode/s, "
+            f"{VARIANT_NAMES[var]} {gbps:.0f} GB/s")
+        ctx.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -227,6 +275,7 @@ def main():
     ap.add_argument("--tuning", default="", help="fixed stencil tuning instead of the autotune sweep, e.g. "
                     "'64:2/1/1/2,32:2/1/1/1,16:2/1/0/1' = precision:traversal/slabs/nt_store/waves_per_eu")
     ap.add_argument("--ensemble-only", action="store_true")
+    ap.add_argument("--no-other", action="store_true", help="skip the other three single-lattice workloads")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -265,7 +314,7 @@ def main():
     som = step_opts(c, prec_mixed, False, args.tol, args.chunk).sign
     res = {}
     if args.quick:
-        args.no_fp64 = args.no_cpu = args.no_ensemble = args.no_traffic = True
+        args.no_fp64 = args.no_cpu = args.no_ensemble = args.no_traffic = args.no_other = True
     if c.kind == "overlap" and not args.quick:
         ctx.apply_overlap(ovl.PSI, ovl.SCRATCH, so64)
         ref = ctx.get_modes(ovl.SCRATCH, 0, min(16, c.nmodes))
@@ -311,7 +360,7 @@ def main():
 
     # dominant stencil variant of the timed region
     var = int(np.argmax([t.var_ms[i] for i in range(5)]))
-    names = ["k_hop FP64", "k_hop FP32", "k_hop FP32->FP64", "k_hop FP16", "k_hop 48-bit"]
+    names = VARIANT_NAMES
     avg_ms = t.var_ms[var] / max(1, t.var_launches[var])
     bytes_per_launch = t.var_bytes[var] / max(1, t.var_launches[var])
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
@@ -397,6 +446,10 @@ def main():
     if not args.no_ensemble:
         ens = bench_ensemble(cf.ENSEMBLES["su2_overlap_ensemble"], 2, 1, args.tol)
 
+    other = None
+    if not args.no_other:
+        other = bench_other_workloads(args.tol)
+
     cpu = None
     if not args.no_cpu:
         links, e, psi = c.generate(nmodes=args.cpu_modes)
@@ -416,7 +469,8 @@ def main():
                       "traversal": args.traversal, "parallelism": "single GPU"},
            "roofline": roofline, "cpu_baseline": cpu, "fp64_gpu": fp64,
            "speedup_vs_fp64": (value / fp64["value"]) if fp64 else None,
-           "graph_ms_per_step": graph_ms, "accuracy": res, "observables": observables, "ensemble": ens}
+           "graph_ms_per_step": graph_ms, "accuracy": res, "observables": observables, "ensemble": ens,
+           "other_workloads": other}
     print(json.dumps(out))
 
 
