@@ -254,6 +254,48 @@ def bench_other_workloads(tol: float):
     return out
 
 
+def bench_next_rows(c: cf.Config, links, e):
+    """Wall time of the calls either side of the leapfrog on the bench lattice: Gauss-law
+    restoration (100 sweeps, E relaxed on the fixed links), the configuration text files (save and
+    load of links and E), and the overlap eigenbasis of the U(1) background of the same lattice and
+    flux (FP64 Chebyshev sign of the workload's degree; 256 modes set)."""
+    out = {}
+    with ovl.Context(c.nc, c.n, 64) as ctx:
+        ctx.set_links(links)
+        ctx.set_efield(e)
+        ctx.gauss_restore(tol=0.0, max_iter=2)
+        t0 = time.perf_counter()
+        it, vmax = ctx.gauss_restore(tol=0.0, max_iter=100)
+        w = time.perf_counter() - t0
+        out["gauss_restore"] = {"lattice": list(c.n), "nc": c.nc, "sweeps": it, "ms_per_sweep": round(1e3 * w / it, 4),
+                                "max_violation_after": vmax}
+        d = tempfile.mkdtemp(prefix="ovl_io_", dir="/tmp")
+        try:
+            uf, ef = os.path.join(d, "u.txt"), os.path.join(d, "e.txt")
+            t0 = time.perf_counter()
+            ctx.save_configuration(uf, ef)
+            ws = time.perf_counter() - t0
+            nbytes = os.path.getsize(uf) + os.path.getsize(ef)
+            t0 = time.perf_counter()
+            ctx.load_configuration(uf, ef)
+            wl = time.perf_counter() - t0
+        finally:
+            shutil.rmtree(d, ignore_errors=True)
+        out["configuration_files"] = {"bytes": nbytes, "save_s": round(ws, 3), "load_s": round(wl, 3),
+                                      "save_MBps": round(nbytes / ws / 1e6, 1), "load_MBps": round(nbytes / wl / 1e6, 1)}
+    with ovl.Context(c.nc, c.n, 256, m_fermion=c.m_fermion, m_dw=c.m_dw) as ctx:
+        o = ovl.sign_opts(degree=c.cheb_n, cut=c.cheb_cut, precision=ovl.PREC_FP64)
+        t0 = time.perf_counter()
+        ev, herm = ctx.qed_eigenbasis(o, c.nb, c.q, c.dt)
+        w = time.perf_counter() - t0
+        out["qed_eigenbasis"] = {"lattice": list(c.n), "nc": c.nc, "flux_quanta": c.nb, "dimension": int(ev.size),
+                                 "blocks": f"{c.n[2]} x {4 * c.n[0] * c.n[1]} + 4", "modes_set": 256,
+                                 "chebyshev_degree": c.cheb_n, "seconds": round(w, 3), "hermiticity": herm,
+                                 "spectrum": [float(ev.min()), float(ev.max())]}
+    log("next rows", json.dumps(out))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -276,6 +318,7 @@ def main():
                     "'64:2/1/1/2,32:2/1/1/1,16:2/1/0/1' = precision:traversal/slabs/nt_store/waves_per_eu")
     ap.add_argument("--ensemble-only", action="store_true")
     ap.add_argument("--no-other", action="store_true", help="skip the other three single-lattice workloads")
+    ap.add_argument("--no-next", action="store_true", help="skip the Gauss restore / files / eigenbasis timings")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -314,7 +357,7 @@ def main():
     som = step_opts(c, prec_mixed, False, args.tol, args.chunk).sign
     res = {}
     if args.quick:
-        args.no_fp64 = args.no_cpu = args.no_ensemble = args.no_traffic = args.no_other = True
+        args.no_fp64 = args.no_cpu = args.no_ensemble = args.no_traffic = args.no_other = args.no_next = True
     if c.kind == "overlap" and not args.quick:
         ctx.apply_overlap(ovl.PSI, ovl.SCRATCH, so64)
         ref = ctx.get_modes(ovl.SCRATCH, 0, min(16, c.nmodes))
@@ -450,6 +493,11 @@ def main():
     if not args.no_other:
         other = bench_other_workloads(args.tol)
 
+    nxt = None
+    if not args.no_next:
+        links, e, _ = c.generate(nmodes=1)
+        nxt = bench_next_rows(c, links, e)
+
     cpu = None
     if not args.no_cpu:
         links, e, psi = c.generate(nmodes=args.cpu_modes)
@@ -470,7 +518,7 @@ def main():
            "roofline": roofline, "cpu_baseline": cpu, "fp64_gpu": fp64,
            "speedup_vs_fp64": (value / fp64["value"]) if fp64 else None,
            "graph_ms_per_step": graph_ms, "accuracy": res, "observables": observables, "ensemble": ens,
-           "other_workloads": other}
+           "other_workloads": other, "next_rows": nxt}
     print(json.dumps(out))
 
 

@@ -230,12 +230,18 @@ int ovl_gauss_restore(ovl_ctx *ctx, int sys, double tol, int max_iter, int *iter
  * (INITIALCONDITIONS/FERMIONS/BASIS/OverlapQEDEigenfunctions.cpp:141-591,
  * SetOverlapQEDEigenfunctions.cpp:5-56): the overlap Hamiltonian of the U(1) background alone (flux
  * nb, charge q; colour 0 carries the phase and the other colours have no hopping, as the reference's
- * ConvertU1GaugeLinks sets them) is assembled column by column from batched overlap applies and
- * diagonalised on the device (rocSOLVER zheevd).  Mode s of Psi and PsiMid (global index
- * g = first_mode + s: particle g % 2, level g / 2) becomes sqrt(V) times eigenvector D/2 - 1 - level
- * (particle) or D/2 + level (antiparticle) of the descending spectrum, PsiMid = Psi exp(-i omega dt/2).
- * D = 4 Nc V <= 16384.  Overwrites the scratch field; the SU(Nc) links are untouched.  eigenvalues
- * (D doubles, descending) and hermiticity (max |H - H^+| of the assembled matrix) may be NULL. */
+ * ConvertU1GaugeLinks sets them).  The reference diagonalises the dense D x D matrix (D = 4 Nc V);
+ * here its exact block structure is used: colours do not mix, colours >= 1 reduce to one 4x4 spin
+ * block, and colour 0 commutes with translations in z, so it splits into n2 z-momentum blocks of
+ * size 4 n0 n1, assembled from overlap applies to the 4 n0 n1 unit sources of the z = 0 plane and
+ * diagonalised together on the device (rocSOLVER zheevd, strided batched).  Mode s of Psi and PsiMid
+ * (global index g = first_mode + s: particle g % 2, level g / 2) becomes sqrt(V) times eigenvector
+ * D/2 - 1 - level (particle) or D/2 + level (antiparticle) of the merged descending spectrum,
+ * PsiMid = Psi exp(-i omega dt/2); within a degenerate level the basis is the block one (plane
+ * waves in z), any orthonormal basis of the level being an eigenbasis.  Limits: 4 n0 n1 <= 16384 and
+ * n2 (4 n0 n1)^2 complex <= 32 GB.  Overwrites the scratch field; the SU(Nc) links are untouched.
+ * eigenvalues (D doubles, descending) and hermiticity (max |B - B^+| over the assembled blocks) may
+ * be NULL. */
 int ovl_qed_eigenbasis(ovl_ctx *ctx, const ovl_sign_opts *o, int nb, double q, double dt, int first_mode,
                        double *eigenvalues, double *hermiticity);
 

@@ -2310,9 +2310,12 @@ int ovl_qed_eigenbasis(ovl_ctx *c, const ovl_sign_opts *o, int nb, double q, dou
     if (!c || !o || !std::isfinite(q) || !std::isfinite(dt) || first_mode < 0) return OVL_E_ARG;
     RETCHK(single_lattice(c, "ovl_qed_eigenbasis"));
     RETCHK(check_sign_opts(c, o));
-    const int nc = c->g.nc, V = c->g.V, n0 = c->g.n0, n1 = c->g.n1;
+    const int nc = c->g.nc, V = c->g.V, n0 = c->g.n0, n1 = c->g.n1, n2 = c->g.n2;
     const long D = 4L * nc * V;
-    if (D > 16384) return fail(c, OVL_E_ARG, "4 Nc V above 16384: the dense eigenproblem is not built for it");
+    // colour-0 block of one z momentum: (x, y, spin), Dz = 4 n0 n1; all n2 of them on the device at once
+    const long Dz = 4L * n0 * n1;
+    if (Dz > 16384) return fail(c, OVL_E_ARG, "4 n0 n1 above 16384: the xy blocks of the eigenproblem are too large");
+    if ((double)n2 * Dz * Dz * 16.0 > 32e9) return fail(c, OVL_E_ARG, "the n2 blocks of 4 n0 n1 squared exceed 32 GB");
     const int nm = c->prm.n_modes;
     // every requested mode must name an eigenvector
     for (int s = 0; s < nm; ++s) {
@@ -2344,115 +2347,198 @@ int ovl_qed_eigenbasis(ovl_ctx *c, const ovl_sign_opts *o, int nb, double q, dou
         ~Restore() { c->stencil_custom = false; }
     } restore{c};
 
-    // H = i (-i h_ov) column by column: unit vectors in batches of the context's modes
-    std::vector<cplx> H((size_t)D * D);
+    // Structure of H = i (-i h_ov) on this background (exact, not an approximation):
+    //  * colours do not mix (the link matrices are diag(phase, 0, ..)); colours >= 1 have no hopping,
+    //    so there H is one 4x4 spin matrix, the same at every site and colour;
+    //  * the phases do not depend on z, so H commutes with translations in z: on colour 0 it is block
+    //    diagonal in the z momentum k = 2 pi j / n2, with blocks
+    //      B_j[(x', y', s'), (x, y, s)] = sum_z' H[(x', y', z', s'), (x, y, 0, s)] exp(-i k z').
+    // So H is applied to the Dz unit sources of the z = 0 plane (colour 0) and to the 4 spin sources of
+    // site 0, colour 1, the n2 Hermitian blocks of size Dz are diagonalised in one batched rocSOLVER
+    // call, and the spectrum is merged: the same eigenvalues and eigenspaces as the dense matrix of
+    // the reference, at Nc n2 times fewer overlap applies and an (Nc n2)^3 smaller eigensolve.
     const size_t per_mode = (size_t)V * 4 * nc;  // complex entries of one mode
     std::vector<cplx> batch((size_t)nm * per_mode);
-    // flat index m = site + V (spin + 4 colour) <-> AoS (site Nc + colour) 4 + spin
-    auto aos_index = [&](long m) {
-        const long site = m % V, r = m / V, spin = r % 4, colour = r / 4;
-        return (size_t)((site * nc + colour) * 4 + spin);
-    };
-    for (long k0 = 0; k0 < D; k0 += nm) {
-        const int cnt = (int)std::min<long>(nm, D - k0);
+    // AoS index of (site, colour, spin) in a host mode
+    auto aos = [&](long site, int colour, int spin) { return (size_t)((site * nc + colour) * 4 + spin); };
+    std::vector<cplx> tw((size_t)n2 * n2);  // exp(-2 pi i j z / n2)
+    for (int j = 0; j < n2; ++j)
+        for (int z = 0; z < n2; ++z) tw[(size_t)j * n2 + z] = std::polar(1.0, -2.0 * M_PI * (double)((long)j * z % n2) / n2);
+    std::vector<cplx> blocks((size_t)n2 * Dz * Dz);  // column-major, block j at j Dz^2
+    cplx spin_block[16];                            // colour >= 1, column-major
+    double leak = 0;                                // largest entry outside the structure above
+    const long nsrc = Dz + (nc > 1 ? 4 : 0);
+    const size_t plane = (size_t)n0 * n1;
+    for (long k0 = 0; k0 < nsrc; k0 += nm) {
+        const int cnt = (int)std::min<long>(nm, nsrc - k0);
         std::fill(batch.begin(), batch.end(), cplx(0, 0));
-        for (int j = 0; j < cnt; ++j) batch[(size_t)j * per_mode + aos_index(k0 + j)] = 1.0;
+        for (int j = 0; j < cnt; ++j) {
+            const long src = k0 + j;
+            if (src < Dz) batch[(size_t)j * per_mode + aos(src / 4, 0, (int)(src % 4))] = 1.0;  // (x, y) at z = 0
+            else batch[(size_t)j * per_mode + aos(0, 1, (int)(src - Dz))] = 1.0;
+        }
         RETCHK(modes_io(c, 0, OVL_SCRATCH, reinterpret_cast<double *>(batch.data()), 0, nm, true));
         if (o->method == OVL_SIGN_ZOLOTAREV)
             RETCHK(zolo_overlap(c, c->g, c->field[OVL_SCRATCH], c->field[OVL_PSIMID], o->lmin, o->lmax, 0, 0.0));
         else
             RETCHK(sign_driver(c, o, c->field[OVL_SCRATCH], c->field[OVL_PSIMID], FIN_OVERLAP, 0.0, 1.0));
         RETCHK(modes_io(c, 0, OVL_PSIMID, reinterpret_cast<double *>(batch.data()), 0, nm, false));
-        for (int j = 0; j < cnt; ++j)
-            for (long m = 0; m < D; ++m) H[(size_t)(k0 + j) * D + m] = cplx(0, 1) * batch[(size_t)j * per_mode + aos_index(m)];
-    }
-    // Hermiticity (the reference prints entries with |H - H^+|^2 > 1e-12), then the Hermitian part
-    double asym = 0;
-    for (long i = 0; i < D; ++i)
-        for (long j = 0; j <= i; ++j) {
-            cplx &a = H[(size_t)j * D + i], &b = H[(size_t)i * D + j];
-            asym = std::max(asym, std::abs(a - std::conj(b)));
-            const cplx h = 0.5 * (a + std::conj(b));
-            a = h;
-            b = std::conj(h);
+        for (int j = 0; j < cnt; ++j) {
+            const long src = k0 + j;
+            const cplx *r = batch.data() + (size_t)j * per_mode;
+            if (src < Dz) {
+                for (size_t xy = 0; xy < plane; ++xy)
+                    for (int sp = 0; sp < 4; ++sp) {
+                        const size_t row = xy * 4 + sp;
+                        for (int jz = 0; jz < n2; ++jz) {
+                            cplx acc = 0;
+                            const cplx *w = tw.data() + (size_t)jz * n2;
+                            for (int z = 0; z < n2; ++z) acc += w[z] * r[aos((long)z * plane + xy, 0, sp)];
+                            blocks[(size_t)jz * Dz * Dz + (size_t)src * Dz + row] = cplx(0, 1) * acc;
+                        }
+                    }
+                for (long site = 0; site < V; ++site)
+                    for (int col = 1; col < nc; ++col)
+                        for (int sp = 0; sp < 4; ++sp) leak = std::max(leak, std::abs(r[aos(site, col, sp)]));
+            } else {
+                const int sc = (int)(src - Dz);
+                for (long site = 0; site < V; ++site)
+                    for (int col = 0; col < nc; ++col)
+                        for (int sp = 0; sp < 4; ++sp) {
+                            const cplx v = r[aos(site, col, sp)];
+                            if (site == 0 && col == 1) spin_block[sc * 4 + sp] = cplx(0, 1) * v;
+                            else leak = std::max(leak, std::abs(v));
+                        }
+            }
         }
+    }
+    // Hermiticity of every block (the reference prints entries with |H - H^+|^2 > 1e-12), then the
+    // Hermitian part
+    double asym = 0;
+    auto hermitize = [&](cplx *m, long n) {
+        for (long i = 0; i < n; ++i)
+            for (long j = 0; j <= i; ++j) {
+                cplx &a = m[(size_t)j * n + i], &b = m[(size_t)i * n + j];
+                asym = std::max(asym, std::abs(a - std::conj(b)));
+                const cplx h = 0.5 * (a + std::conj(b));
+                a = h;
+                b = std::conj(h);
+            }
+    };
+    for (int jz = 0; jz < n2; ++jz) hermitize(blocks.data() + (size_t)jz * Dz * Dz, Dz);
+    if (nc > 1) hermitize(spin_block, 4);
     if (hermiticity) *hermiticity = asym;
-    for (const cplx &v : H)
+    for (const cplx &v : blocks)
         if (!std::isfinite(v.real()) || !std::isfinite(v.imag()))
             return fail(c, OVL_E_SPECTRUM, "the assembled overlap operator has a non-finite entry");
+    if (!(leak <= 1e-12))
+        return fail(c, OVL_E_SPECTRUM, "the U(1) overlap operator mixes colours (entry " + std::to_string(leak) + ")");
 
-    // diagonalisation on the device (the reference: LAPACK zheev, MISC/LapackDiagonalization.cpp:68-120)
+    // diagonalisation on the device (the reference: LAPACK zheev of the dense matrix,
+    // MISC/LapackDiagonalization.cpp:68-120): all z-momentum blocks in one batched divide and
+    // conquer, the 4x4 spin block of the other colours as one more small problem
+    const int nprob = nc > 1 ? 2 : 1;
+    std::vector<double> Wz((size_t)n2 * Dz), W4(4, 0.0);
     void *dA = nullptr, *dW = nullptr, *dE = nullptr, *dInfo = nullptr;
     auto release = [&]() {
         hipStreamSynchronize(c->st);
         dfree(dA); dfree(dW); dfree(dE); dfree(dInfo);
     };
-    int r = dalloc(c, &dA, sizeof(cplx) * (size_t)D * D);
-    if (r == OVL_OK) r = dalloc(c, &dW, sizeof(double) * D);
-    if (r == OVL_OK) r = dalloc(c, &dE, sizeof(double) * D);
-    if (r == OVL_OK) r = dalloc(c, &dInfo, sizeof(int));
+    int r = dalloc(c, &dA, sizeof(cplx) * blocks.size());
+    if (r == OVL_OK) r = dalloc(c, &dW, sizeof(double) * Wz.size());
+    if (r == OVL_OK) r = dalloc(c, &dE, sizeof(double) * Wz.size());
+    if (r == OVL_OK) r = dalloc(c, &dInfo, sizeof(int) * n2);
     if (r != OVL_OK) { release(); return r; }
-    std::vector<double> W(D);
-    std::vector<cplx> Hin = H;  // kept for the QR fallback
     rocblas_handle handle = nullptr;
     if (rocblas_create_handle(&handle) != rocblas_status_success) { release(); return fail(c, OVL_E_HIP, "rocblas_create_handle failed"); }
     rocblas_set_stream(handle, c->st);
-    // divide and conquer first; the QR iteration if it leaves non-finite values
     std::string why;
-    bool ok = false;
-    for (int method = 0; method < 2 && !ok; ++method) {
-        int info = -1;
-        hipError_t he = hipMemcpyAsync(dA, Hin.data(), sizeof(cplx) * (size_t)D * D, hipMemcpyHostToDevice, c->st);
+    for (int prob = 0; prob < nprob && why.empty(); ++prob) {
+        const long n = prob == 0 ? Dz : 4;
+        const int count = prob == 0 ? n2 : 1;
+        const cplx *src = prob == 0 ? blocks.data() : spin_block;
+        double *w = prob == 0 ? Wz.data() : W4.data();
+        std::vector<int> info(count, -1);
+        hipError_t he = hipMemcpyAsync(dA, src, sizeof(cplx) * (size_t)count * n * n, hipMemcpyHostToDevice, c->st);
         rocblas_status rs = rocblas_status_success;
         if (he == hipSuccess)
-            rs = method == 0 ? rocsolver_zheevd(handle, rocblas_evect_original, rocblas_fill_lower, (rocblas_int)D,
-                                                (rocblas_double_complex *)dA, (rocblas_int)D, (double *)dW,
-                                                (double *)dE, (rocblas_int *)dInfo)
-                             : rocsolver_zheev(handle, rocblas_evect_original, rocblas_fill_lower, (rocblas_int)D,
-                                               (rocblas_double_complex *)dA, (rocblas_int)D, (double *)dW,
-                                               (double *)dE, (rocblas_int *)dInfo);
+            rs = rocsolver_zheevd_strided_batched(handle, rocblas_evect_original, rocblas_fill_lower, (rocblas_int)n,
+                                                  (rocblas_double_complex *)dA, (rocblas_int)n, (rocblas_stride)(n * n),
+                                                  (double *)dW, (rocblas_stride)n, (double *)dE, (rocblas_stride)n,
+                                                  (rocblas_int *)dInfo, count);
         if (he == hipSuccess && rs == rocblas_status_success)
-            he = hipMemcpyAsync(W.data(), dW, sizeof(double) * D, hipMemcpyDeviceToHost, c->st);
+            he = hipMemcpyAsync(w, dW, sizeof(double) * (size_t)count * n, hipMemcpyDeviceToHost, c->st);
         if (he == hipSuccess && rs == rocblas_status_success)
-            he = hipMemcpyAsync(H.data(), dA, sizeof(cplx) * (size_t)D * D, hipMemcpyDeviceToHost, c->st);
+            he = hipMemcpyAsync(prob == 0 ? (void *)blocks.data() : (void *)spin_block, dA,
+                                sizeof(cplx) * (size_t)count * n * n, hipMemcpyDeviceToHost, c->st);
         if (he == hipSuccess && rs == rocblas_status_success)
-            he = hipMemcpyAsync(&info, dInfo, sizeof(int), hipMemcpyDeviceToHost, c->st);
+            he = hipMemcpyAsync(info.data(), dInfo, sizeof(int) * count, hipMemcpyDeviceToHost, c->st);
         if (he == hipSuccess) he = hipStreamSynchronize(c->st);
-        const char *name = method == 0 ? "zheevd" : "zheev";
-        if (he != hipSuccess) { why += std::string(name) + ": " + hipGetErrorString(he) + "; "; continue; }
-        if (rs != rocblas_status_success) { why += std::string(name) + ": status " + std::to_string((int)rs) + "; "; continue; }
-        if (info != 0) { why += std::string(name) + ": info " + std::to_string(info) + "; "; continue; }
-        long bad_w = 0, bad_v = 0;
-        for (double w : W) bad_w += !std::isfinite(w);
-        for (const cplx &v : H) bad_v += !(std::isfinite(v.real()) && std::isfinite(v.imag()));
-        if (bad_w || bad_v) {
-            why += std::string(name) + ": " + std::to_string(bad_w) + " non-finite eigenvalues, " + std::to_string(bad_v) +
-                   " non-finite vector entries; ";
-            continue;
-        }
-        ok = true;
+        if (he != hipSuccess) { why = std::string("zheevd: ") + hipGetErrorString(he); break; }
+        if (rs != rocblas_status_success) { why = "zheevd: status " + std::to_string((int)rs); break; }
+        for (int b = 0; b < count; ++b)
+            if (info[b] != 0) why = "zheevd: info " + std::to_string(info[b]) + " in block " + std::to_string(b);
+        for (long i = 0; i < (long)count * n; ++i)
+            if (!std::isfinite(w[i])) why = "zheevd: non-finite eigenvalue";
     }
     rocblas_destroy_handle(handle);
     release();
-    if (!ok) return fail(c, OVL_E_CONVERGE, "eigensolver failed: " + why);
-    // the reference stores the spectrum in descending order (LapackDiagonalization.cpp:103-112)
+    if (!why.empty()) return fail(c, OVL_E_CONVERGE, "eigensolver failed: " + why);
+    for (const cplx &v : blocks)
+        if (!(std::isfinite(v.real()) && std::isfinite(v.imag())))
+            return fail(c, OVL_E_CONVERGE, "eigensolver failed: non-finite eigenvector entries");
+
+    // the merged spectrum in descending order, as the reference stores it (LapackDiagonalization.cpp:
+    // 103-112); ties (degenerate levels) are ordered by block and by index, the reference's order
+    // within a degenerate eigenspace is LAPACK's and any orthonormal basis of it is an eigenbasis.
+    // Entry: {value, block (0..n2-1: z momentum; -1: colours >= 1), index (block eigenvector or
+    // 4 * copy + spin eigenvector)}
+    struct Ent { double w; int blk; long idx; };
+    std::vector<Ent> ent;
+    ent.reserve((size_t)D);
+    for (int jz = 0; jz < n2; ++jz)
+        for (long l = 0; l < Dz; ++l) ent.push_back({Wz[(size_t)jz * Dz + l], jz, l});
+    for (long copy = 0; copy < (long)V * (nc - 1); ++copy)
+        for (int l = 0; l < 4; ++l) ent.push_back({W4[l], -1, 4 * copy + l});
+    std::stable_sort(ent.begin(), ent.end(), [](const Ent &a, const Ent &b) { return a.w > b.w; });
+    if ((long)ent.size() != D) return fail(c, OVL_E_STATE, "eigenbasis size mismatch");
     if (eigenvalues)
-        for (long b = 0; b < D; ++b) eigenvalues[b] = W[D - 1 - b];
+        for (long b = 0; b < D; ++b) eigenvalues[b] = ent[b].w;
 
     // Psi(s) = sqrt(V) v_b, PsiMid(s) = Psi(s) exp(-i omega dt / 2), b from (particle, level) of the
     // global mode index (FermionField.cpp:63-74, SetOverlapQEDEigenfunctions.cpp:14-44)
-    std::vector<cplx> psi((size_t)nm * per_mode), mid((size_t)nm * per_mode);
+    std::vector<cplx> psi((size_t)nm * per_mode, cplx(0, 0)), mid((size_t)nm * per_mode, cplx(0, 0));
     const double norm = std::sqrt((double)V);
-    for (int s = 0; s < nm; ++s) {
-        const long g = (long)first_mode + s, particle = g % 2, level = (g / 2) % (2L * V * nc);
+    for (int sm = 0; sm < nm; ++sm) {
+        const long g = (long)first_mode + sm, particle = g % 2, level = (g / 2) % (2L * V * nc);
         const long b = particle == 0 ? D / 2 - 1 - level : D / 2 + level;
-        const long a = D - 1 - b;  // ascending index
-        const double w = W[a];
-        const cplx rot = std::exp(cplx(0, -0.5 * w * dt));
-        for (long m = 0; m < D; ++m) {
-            const cplx v = norm * H[(size_t)a * D + m];
-            psi[(size_t)s * per_mode + aos_index(m)] = v;
-            mid[(size_t)s * per_mode + aos_index(m)] = v * rot;
+        const Ent &e = ent[b];
+        const cplx rot = std::exp(cplx(0, -0.5 * e.w * dt));
+        cplx *ps = psi.data() + (size_t)sm * per_mode, *pm = mid.data() + (size_t)sm * per_mode;
+        if (e.blk >= 0) {
+            // exp(i k z) chi(x, y, s) / sqrt(n2) on colour 0
+            const cplx *chi = blocks.data() + (size_t)e.blk * Dz * Dz + (size_t)e.idx * Dz;
+            const double f = norm / std::sqrt((double)n2);
+            for (int z = 0; z < n2; ++z) {
+                const cplx ph = f * std::conj(tw[(size_t)e.blk * n2 + z]);
+                for (size_t xy = 0; xy < plane; ++xy)
+                    for (int sp = 0; sp < 4; ++sp) {
+                        const cplx v = ph * chi[xy * 4 + sp];
+                        const size_t k = aos((long)z * plane + xy, 0, sp);
+                        ps[k] = v;
+                        pm[k] = v * rot;
+                    }
+            }
+        } else {
+            // spin eigenvector at one (site, colour >= 1)
+            const long copy = e.idx / 4, site = copy % V;
+            const int col = 1 + (int)(copy / V), l = (int)(e.idx % 4);
+            for (int sp = 0; sp < 4; ++sp) {
+                const cplx v = norm * spin_block[l * 4 + sp];
+                const size_t k = aos(site, col, sp);
+                ps[k] = v;
+                pm[k] = v * rot;
+            }
         }
     }
     RETCHK(modes_io(c, 0, OVL_PSI, reinterpret_cast<double *>(psi.data()), 0, nm, true));

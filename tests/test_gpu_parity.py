@@ -373,6 +373,36 @@ def test_traversals_are_permutations(gpu, nc, n, nm, mb):
     assert err < 1e-5
 
 
+def test_qed_eigenbasis_beyond_dense_size(gpu):
+    # SU(3) 12 x 10 x 12: D = 4 Nc V = 17280, more than a dense 16384 problem; the block solve gives
+    # the whole spectrum, the returned modes are orthogonal eigenvectors of the operator the oracle
+    # applies (U(1) phase on colour 0, no other hopping)
+    n, nc, nb, deg, cut, M, dt = (12, 10, 12), 3, 3, 48, 0.1, 1.2, 0.02
+    print(f"[case] qed eigenbasis n={n} nc={nc}", flush=True)
+    V = int(np.prod(n))
+    D = 4 * nc * V
+    nm = 8
+    ctx = ovl.Context(nc, n, nm, m_dw=M, mode_block=16)
+    ev, herm = ctx.qed_eigenbasis(ovl.sign_opts(degree=deg, cut=cut, precision=ovl.PREC_FP64), nb, 1.0, dt,
+                                  first_mode=5)
+    assert ev.shape == (D,) and np.all(np.diff(ev) <= 0) and herm < 1e-12 and np.isfinite(ev).all()
+    L = orc.lat(nc, n, 1.0, 0.0, M)
+    ph = orc.u1_phases(n, nb, 1.0)
+    ucc = np.zeros((3, V, nc, nc), complex)
+    ucc[:, :, 0, 0] = np.asarray(ph).reshape(3, V)
+    coef = orc.cheb_coeffs(deg, cut)
+    lam = orc.norm_bound(L)
+    psi = ctx.get_modes(ovl.PSI)
+    gram = psi.conj() @ psi.T
+    assert np.abs(gram - V * np.eye(nm)).max() < 1e-9 * V
+    for s in range(nm):
+        g = 5 + s
+        particle, level = g % 2, g // 2
+        b = D // 2 - 1 - level if particle == 0 else D // 2 + level
+        hv = 1j * orc.overlap_evo_cheb(L, ucc.reshape(-1), lam, cut, deg, coef, psi[s])
+        assert np.linalg.norm(hv - ev[b] * psi[s]) < 1e-9 * np.sqrt(V)
+
+
 @pytest.mark.parametrize("nc,mb", [(3, 64), (2, 16)])
 def test_occupancy_variants_are_bit_identical(gpu, nc, mb):
     # occupancy variants 1-5 (5: one workgroup per CU through an LDS reservation) run the same
