@@ -219,6 +219,7 @@ def bench_other_workloads(tol: float):
         c = cf.CONFIGS[name]
         nm = min(c.nmodes, 64)
         links, e, psi = c.generate(nm)
+        psi0 = psi[:4].copy()
         ctx = ovl.Context(c.nc, c.n, c.nmodes, r_wilson=c.r_wilson, m_fermion=c.m_fermion, m_dw=c.m_dw)
         ctx.set_links(links)
         ctx.set_efield(e)
@@ -251,7 +252,36 @@ def bench_other_workloads(tol: float):
         log(f"{name}: {out[name]['ms_per_step']:.2f} ms/step {out[name]['value']:.4e} site*mode/s, "
             f"{VARIANT_NAMES[var]} {gbps:.0f} GB/s")
         ctx.close()
+        if c.nc == 2:
+            out[name]["cpu_reference"] = reference_cpu(c, links, psi0)
     return out
+
+
+def reference_cpu(c: cf.Config, links, psi):
+    """The reference's own operator on the host (oracle/_ref, built from the reference's SU(2)
+    sources), one core: WilsonEvoOperator::Compute (WilsonOperator.cpp:30) or OverlapU with its
+    Zolotarev coefficients (OverlapOperator.cpp:25-64 without the unused random vector) on the
+    interval the Chebyshev sign uses, timed per mode.  A leapfrog step applies it twice per mode
+    (OverlapUpdate.cpp:9-58); the reference's four Ritz solves per step are not counted."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import orc
+    if not orc.ref_available():
+        return None
+    L = orc.lat(c.nc, c.n, c.r_wilson, c.m_fermion, c.m_dw)
+    ucc = orc.links_to_ucc(L, links, orc.u1_phases(c.n, c.nb, c.q) if c.nb else None)
+    k, t0 = 0, time.perf_counter()
+    while k < len(psi) and (k == 0 or time.perf_counter() - t0 < 5.0):
+        if c.kind == "overlap":
+            lam = orc.norm_bound(L)
+            orc.ref_overlap_evo(L, ucc, psi[k], (c.cheb_cut * lam) ** 2, lam ** 2)
+        else:
+            orc.ref_wilson_evo(L, ucc, psi[k])
+        k += 1
+    per_mode = (time.perf_counter() - t0) / k
+    fn = "OverlapU (Zolotarev, 25 poles, multi-shift CG)" if c.kind == "overlap" else "WilsonEvoOperator::Compute"
+    return {"kind": "reference", "function": fn, "cores": 1, "modes_timed": k,
+            "seconds_per_mode_apply": round(per_mode, 5), "unit": "site*mode updates/s",
+            "value": c.volume / (2 * per_mode)}
 
 
 def bench_next_rows(c: cf.Config, links, e):
