@@ -2369,6 +2369,9 @@ int ovl_qed_eigenbasis(ovl_ctx *c, const ovl_sign_opts *o, int nb, double q, dou
     double leak = 0;                                // largest entry outside the structure above
     const long nsrc = Dz + (nc > 1 ? 4 : 0);
     const size_t plane = (size_t)n0 * n1;
+    // Zolotarev: one spectral interval for every batch (the Ritz estimate of the U(1) operator once)
+    double zlmin = o->lmin, zlmax = o->lmax;
+    if (o->method == OVL_SIGN_ZOLOTAREV && zlmin == 0.0 && zlmax == 0.0) RETCHK(ritz_bounds(c, &zlmin, &zlmax));
     for (long k0 = 0; k0 < nsrc; k0 += nm) {
         const int cnt = (int)std::min<long>(nm, nsrc - k0);
         std::fill(batch.begin(), batch.end(), cplx(0, 0));
@@ -2379,7 +2382,7 @@ int ovl_qed_eigenbasis(ovl_ctx *c, const ovl_sign_opts *o, int nb, double q, dou
         }
         RETCHK(modes_io(c, 0, OVL_SCRATCH, reinterpret_cast<double *>(batch.data()), 0, nm, true));
         if (o->method == OVL_SIGN_ZOLOTAREV)
-            RETCHK(zolo_overlap(c, c->g, c->field[OVL_SCRATCH], c->field[OVL_PSIMID], o->lmin, o->lmax, 0, 0.0));
+            RETCHK(zolo_overlap(c, c->g, c->field[OVL_SCRATCH], c->field[OVL_PSIMID], zlmin, zlmax, 0, 0.0));
         else
             RETCHK(sign_driver(c, o, c->field[OVL_SCRATCH], c->field[OVL_PSIMID], FIN_OVERLAP, 0.0, 1.0));
         RETCHK(modes_io(c, 0, OVL_PSIMID, reinterpret_cast<double *>(batch.data()), 0, nm, false));

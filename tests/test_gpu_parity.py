@@ -373,6 +373,30 @@ def test_traversals_are_permutations(gpu, nc, n, nm, mb):
     assert err < 1e-5
 
 
+def test_qed_eigenbasis_zolotarev_vs_dense_oracle(gpu):
+    # the reference's own sign (25-pole Zolotarev, multi-shift CG) on a fixed interval of h_W^2:
+    # the block spectrum against numpy's eigh of the oracle's dense matrix of the same operator
+    n, nc, nb, M, dt, lmin, lmax = (3, 3, 4), 2, 1, 1.0, 0.02, 0.2, 30.0
+    print(f"[case] qed eigenbasis zolotarev n={n}", flush=True)
+    V = int(np.prod(n))
+    D = 4 * nc * V
+    ctx = ovl.Context(nc, n, 16, m_dw=M, mode_block=16)
+    ev, herm = ctx.qed_eigenbasis(ovl.sign_opts(method=ovl.SIGN_ZOLOTAREV, lmin=lmin, lmax=lmax), nb, 1.0, dt)
+    L = orc.lat(nc, n, 1.0, 0.0, M)
+    ph = orc.u1_phases(n, nb, 1.0)
+    ucc = np.zeros((3, V, nc, nc), complex)
+    ucc[:, :, 0, 0] = np.asarray(ph).reshape(3, V)
+    H = np.empty((D, D), complex)
+    for k in range(D):
+        e = np.zeros(D, complex)
+        e[k] = 1.0
+        H[:, k] = 1j * orc.overlap_evo_zolo(L, ucc.reshape(-1), e, lmin, lmax)[0]
+    Hh = 0.5 * (H + H.conj().T)
+    w = np.linalg.eigvalsh(Hh)
+    assert herm < 1e-8
+    assert np.abs(ev - w[::-1]).max() < 1e-8
+
+
 def test_qed_eigenbasis_beyond_dense_size(gpu):
     # SU(3) 12 x 10 x 12: D = 4 Nc V = 17280, more than a dense 16384 problem; the block solve gives
     # the whole spectrum, the returned modes are orthogonal eigenvectors of the operator the oracle
