@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 import orc
+import tiers
 from overlap_amd import configs as cf
 from overlap_amd import ovl
 
@@ -151,10 +152,19 @@ def test_mixed_precision_sign_reaches_1e10(gpu, nc, nb):
     assert t.fp16_steps == 0 and t.fp32_steps > 0 and err <= 1e-10
 
 
+def _agg(a, b):
+    return np.sqrt(sum(np.linalg.norm(x - y) ** 2 for x, y in zip(a, b)))
+
+
 @pytest.mark.parametrize("nc,nb", [(2, 1), (3, 3)])
 def test_fp16_tier_vs_oracle(gpu, nc, nb):
     # nearly the whole recurrence in FP16 storage (steps k >= 9 of 48): the result carries the FP16
-    # rounding (u16 = 4.9e-4 times the size of the intermediates), the FP64 form does not
+    # rounding (u16 = 4.9e-4 times the size of the intermediates), the FP64 form does not.  The
+    # host model of the same roundings (tiers.sign_tiers) sets the expected size of that error:
+    # FP16 rounding under the FP32 arithmetic of the stencil is chaotic (a 3e-7 perturbation of
+    # every h_W decorrelates the roundings), so the two agree in size, not pointwise; the size
+    # moves by under 8% between perturbed host runs, by 1.4-3.7x with truncation instead of
+    # rounding to nearest, by 1.3-2.4x with a scale 2^10 too small
     print(f"[case] fp16 tier nc={nc} nb={nb}", flush=True)
     n = (4, 3, 4)
     ctx, L, ucc, *_, psi = make(nc, n, 20, 60 + nc, M=1.2, nb=nb, mb=16)
@@ -172,6 +182,11 @@ def test_fp16_tier_vs_oracle(gpu, nc, nb):
             for k in range(20)]
     print(f"  fp16-tier error: max {max(errs):.2e} min {min(errs):.2e}", flush=True)
     assert max(errs) < 2e-2 and np.isfinite(out).all()
+    ref = np.array([orc.sign_cheb(L, ucc, lam, cut, deg, coef, x) for x in psi])
+    model = tiers.sign_tiers(L, ucc, lam, cut, coef, psi, ovl.sign_plan(o))
+    ratio = _agg(out, ref) / _agg(model, ref)
+    print(f"  fp16-tier error size GPU / host model: {ratio:.3f}", flush=True)
+    assert 0.8 < ratio < 1.25
     # the FP16 variant is timed on its own
     assert t.var_launches[3] == 2 * (deg - 9) and t.var_launches[1] > 0  # b_n = c_n x needs no stencil
 

@@ -449,3 +449,36 @@ def test_leapfrog_conserves_re_psi_psimid(kind):
     assert np.abs(inv - 1).max() < 1e-13
     assert np.abs(np.linalg.norm(p, axis=1) ** 2 - 1).max() > 1e-4  # the norm itself moves
     assert V * nc * 4 == p.shape[1]
+
+
+# ---------------------------------------------------------------- storage-rounding model
+def test_tier_model_all_fp64_is_the_oracle_sign():
+    # the host model of the tiered recurrence with every step in FP64 is the oracle's sign; with
+    # an FP16 tail its error stays inside 2 u16 B_head16 + 2 u32 B_head (u = unit roundoff,
+    # B_k = sup of the scalar Clenshaw intermediates on the approximation interval)
+    import tiers
+    n = (3, 3, 4)
+    g = cf.rng(5)
+    links = cf.su3_links(n, 0.4, g)
+    psi = cf.spinors(n, 3, 3, g)
+    L = orc.lat(3, n, 1.0, 0.0, 1.2)
+    ucc = orc.links_to_ucc(L, links, orc.u1_phases(n, 2, 1.0))
+    deg, cut = 32, 0.1
+    coef = orc.cheb_coeffs(deg, cut)
+    lam = orc.norm_bound(L)
+    got = tiers.sign_tiers(L, ucc, lam, cut, coef, psi, (deg + 1,) * 3)
+    for x, y in zip(got, psi):
+        assert np.abs(x - orc.sign_cheb(L, ucc, lam, cut, deg, coef, y)).max() < 1e-13 * np.abs(x).max()
+    B = tiers.clenshaw_sup(coef, -1.0)
+    f16 = tiers.sign_tiers(L, ucc, lam, cut, coef, psi, (4, 4, 12))
+    for x, y, z in zip(f16, got, psi):
+        e = np.linalg.norm(x - y) / np.linalg.norm(z)
+        assert 1e-3 * 2 * 2.0 ** -11 * B[12] < e < 2 * 2.0 ** -11 * B[12] + 2 * 2.0 ** -24 * B[4]
+
+
+def test_tier_model_fp16_rounding():
+    import tiers
+    v = np.array([1.0, 1.0 + 2.0 ** -11, 1.0 + 3 * 2.0 ** -11, 65504.0, 2.0 ** -24, 2.0 ** -26, 0.1])
+    want = np.array([1.0, 1.0, 1.0 + 2.0 ** -9, 65504.0, 2.0 ** -24, 0.0, 0.0999755859375])
+    assert np.array_equal(tiers.q16(v), want)  # ties to even, subnormal floor 2^-24
+    assert tiers.pow2_scale(3.0) == 64.0 and tiers.pow2_scale(256.0) == 1.0 and tiers.pow2_scale(0.0) == 1.0
