@@ -1083,7 +1083,7 @@ static int ctx_init(ovl_ctx *c)
     for (int f = 0; f < 3; ++f) HIPCHK(c, hipMemsetAsync(c->field[f], 0, c->field64_bytes, c->st));
     HIPCHK(c, hipMemsetAsync(c->efield, 0, esz, c->st));
     size_t plen = std::max(std::max(ovl::mode_dot_partial_len(c->g), ovl::reduce_partial_len(c->g)),
-                           ovl::currents_partial_len(c->g));
+                           std::max(ovl::currents_partial_len(c->g), ovl::eig_partial_len()));
     int maxblk = c->g.nblk;
     for (int s = 0; s < nsys(c); ++s) {
         const Geom gs = sys_geom(c, s);
@@ -2358,122 +2358,154 @@ int ovl_qed_eigenbasis(ovl_ctx *c, const ovl_sign_opts *o, int nb, double q, dou
     // call, and the spectrum is merged: the same eigenvalues and eigenspaces as the dense matrix of
     // the reference, at Nc n2 times fewer overlap applies and an (Nc n2)^3 smaller eigensolve.
     const size_t per_mode = (size_t)V * 4 * nc;  // complex entries of one mode
-    std::vector<cplx> batch((size_t)nm * per_mode);
     // AoS index of (site, colour, spin) in a host mode
     auto aos = [&](long site, int colour, int spin) { return (size_t)((site * nc + colour) * 4 + spin); };
-    std::vector<cplx> tw((size_t)n2 * n2);  // exp(-2 pi i j z / n2)
+    std::vector<cplx> tw((size_t)n2 * n2);  // exp(-2 pi i j z / n2), symmetric in (j, z)
     for (int j = 0; j < n2; ++j)
         for (int z = 0; z < n2; ++z) tw[(size_t)j * n2 + z] = std::polar(1.0, -2.0 * M_PI * (double)((long)j * z % n2) / n2);
-    std::vector<cplx> blocks((size_t)n2 * Dz * Dz);  // column-major, block j at j Dz^2
-    cplx spin_block[16];                            // colour >= 1, column-major
-    double leak = 0;                                // largest entry outside the structure above
+    cplx spin_block[16];  // colour >= 1, column-major
+    double leak = 0;      // largest entry outside the structure above
     const long nsrc = Dz + (nc > 1 ? 4 : 0);
     const size_t plane = (size_t)n0 * n1;
     // Zolotarev: one spectral interval for every batch (the Ritz estimate of the U(1) operator once)
     double zlmin = o->lmin, zlmax = o->lmax;
     if (o->method == OVL_SIGN_ZOLOTAREV && zlmin == 0.0 && zlmax == 0.0) RETCHK(ritz_bounds(c, &zlmin, &zlmax));
+
+    // device buffers: the n2 blocks (column-major, block j at j Dz^2), the z-momentum planes of one
+    // batch (the FP64 work field w64: per mode block n2 planes of plane * 4 Nc * MB complex, the size
+    // of one field), the DFT matrix, the eigensolver's outputs
+    const int nprob = nc > 1 ? 2 : 1;
+    std::vector<double> Wz((size_t)n2 * Dz), W4(4, 0.0);
+    void *dA = nullptr, *dTW = nullptr, *dW = nullptr, *dE = nullptr, *dInfo = nullptr, *dS = nullptr;
+    rocblas_handle handle = nullptr;
+    auto release = [&]() {
+        hipStreamSynchronize(c->st);
+        if (handle) rocblas_destroy_handle(handle);
+        handle = nullptr;
+        dfree(dA); dfree(dTW); dfree(dW); dfree(dE); dfree(dInfo); dfree(dS);
+    };
+    struct Release {
+        decltype(release) &f;
+        ~Release() { f(); }
+    } release_on_exit{release};
+    RETCHK(ensure_work(c, c->g.nblk, false));
+    RETCHK(dalloc(c, &dA, sizeof(cplx) * (size_t)n2 * Dz * Dz));
+    RETCHK(dalloc(c, &dTW, sizeof(cplx) * tw.size()));
+    RETCHK(dalloc(c, &dW, sizeof(double) * Wz.size()));
+    RETCHK(dalloc(c, &dE, sizeof(double) * Wz.size()));
+    RETCHK(dalloc(c, &dInfo, sizeof(int) * n2));
+    RETCHK(dalloc(c, &dS, sizeof(cplx) * 16 + sizeof(double) * 8));  // 4x4 spin problem: matrix, W, E
+    HIPCHK(c, hipMemcpyAsync(dTW, tw.data(), sizeof(cplx) * tw.size(), hipMemcpyHostToDevice, c->st));
+    if (rocblas_create_handle(&handle) != rocblas_status_success) {
+        handle = nullptr;
+        return fail(c, OVL_E_HIP, "rocblas_create_handle failed");
+    }
+    rocblas_set_stream(handle, c->st);
+    // host maxima of the per-workgroup partial results written to c->partial
+    auto partial_max = [&](int stride, int which, double *out) -> int {
+        const int nb = ovl::eig_partial_blocks();
+        std::vector<double> h((size_t)nb * stride);
+        HIPCHK(c, hipMemcpyAsync(h.data(), c->partial, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->st));
+        HIPCHK(c, hipStreamSynchronize(c->st));
+        double m = 0;
+        for (int b = 0; b < nb; ++b) m = std::max(m, h[(size_t)b * stride + which]);
+        *out = m;
+        return OVL_OK;
+    };
+    const Geom &g = c->g;
+    const long ncol = (long)plane * c->ncomp * g.mb;  // entries of one z plane of one mode block
+    std::vector<cplx> spin_resp;
     for (long k0 = 0; k0 < nsrc; k0 += nm) {
         const int cnt = (int)std::min<long>(nm, nsrc - k0);
-        std::fill(batch.begin(), batch.end(), cplx(0, 0));
-        for (int j = 0; j < cnt; ++j) {
-            const long src = k0 + j;
-            if (src < Dz) batch[(size_t)j * per_mode + aos(src / 4, 0, (int)(src % 4))] = 1.0;  // (x, y) at z = 0
-            else batch[(size_t)j * per_mode + aos(0, 1, (int)(src - Dz))] = 1.0;
-        }
-        RETCHK(modes_io(c, 0, OVL_SCRATCH, reinterpret_cast<double *>(batch.data()), 0, nm, true));
+        const int nsb = (int)std::max<long>(0, std::min<long>(cnt, Dz - k0));  // colour-0 sources of the batch
+        HIPCHK(c, hipMemsetAsync(c->field[OVL_SCRATCH], 0, c->field64_bytes, c->st));
+        HIPCHK(c, ovl::launch_eig_sources(g, c->field[OVL_SCRATCH], k0, cnt, Dz, c->st));
         if (o->method == OVL_SIGN_ZOLOTAREV)
-            RETCHK(zolo_overlap(c, c->g, c->field[OVL_SCRATCH], c->field[OVL_PSIMID], zlmin, zlmax, 0, 0.0));
+            RETCHK(zolo_overlap(c, g, c->field[OVL_SCRATCH], c->field[OVL_PSIMID], zlmin, zlmax, 0, 0.0));
         else
             RETCHK(sign_driver(c, o, c->field[OVL_SCRATCH], c->field[OVL_PSIMID], FIN_OVERLAP, 0.0, 1.0));
-        RETCHK(modes_io(c, 0, OVL_PSIMID, reinterpret_cast<double *>(batch.data()), 0, nm, false));
-        for (int j = 0; j < cnt; ++j) {
-            const long src = k0 + j;
-            const cplx *r = batch.data() + (size_t)j * per_mode;
-            if (src < Dz) {
-                for (size_t xy = 0; xy < plane; ++xy)
-                    for (int sp = 0; sp < 4; ++sp) {
-                        const size_t row = xy * 4 + sp;
-                        for (int jz = 0; jz < n2; ++jz) {
-                            cplx acc = 0;
-                            const cplx *w = tw.data() + (size_t)jz * n2;
-                            for (int z = 0; z < n2; ++z) acc += w[z] * r[aos((long)z * plane + xy, 0, sp)];
-                            blocks[(size_t)jz * Dz * Dz + (size_t)src * Dz + row] = cplx(0, 1) * acc;
-                        }
-                    }
-                for (long site = 0; site < V; ++site)
-                    for (int col = 1; col < nc; ++col)
-                        for (int sp = 0; sp < 4; ++sp) leak = std::max(leak, std::abs(r[aos(site, col, sp)]));
-            } else {
-                const int sc = (int)(src - Dz);
+        if (nsb > 0) {
+            // z momenta of every mode block: out[blk][j] = i sum_z r[blk][z] exp(-2 pi i j z / n2), one
+            // strided batched GEMM over the mode blocks (column-major: out (ncol x n2) = R (ncol x n2,
+            // leading dimension ncol: the z planes of a mode block follow each other) times TW)
+            const rocblas_double_complex one_i = {0.0, 1.0}, zero = {0.0, 0.0};
+            const rocblas_status rs = rocblas_zgemm_strided_batched(
+                handle, rocblas_operation_none, rocblas_operation_none, (rocblas_int)ncol, n2, n2, &one_i,
+                (const rocblas_double_complex *)c->field[OVL_PSIMID], (rocblas_int)ncol, (rocblas_stride)(ncol * n2),
+                (const rocblas_double_complex *)dTW, n2, 0, &zero, (rocblas_double_complex *)c->w64,
+                (rocblas_int)ncol, (rocblas_stride)(ncol * n2), g.nblk);
+            if (rs != rocblas_status_success) return fail(c, OVL_E_HIP, "zgemm: status " + std::to_string((int)rs));
+            HIPCHK(c, ovl::launch_eig_gather(g, c->w64, dA, k0, nsb, c->st));
+            // colours >= 1 of the colour-0 responses must vanish
+            HIPCHK(c, ovl::launch_eig_leak(g, c->field[OVL_PSIMID], nsb, c->partial, c->st));
+            double lk = 0;
+            RETCHK(partial_max(1, 0, &lk));
+            leak = std::max(leak, lk);
+        }
+        if (nsb < cnt) {
+            // the 4 spin sources of colour 1 at site 0: responses on the host
+            spin_resp.assign((size_t)(cnt - nsb) * per_mode, cplx(0, 0));
+            RETCHK(modes_io(c, 0, OVL_PSIMID, reinterpret_cast<double *>(spin_resp.data()), nsb, cnt - nsb, false));
+            for (int j = nsb; j < cnt; ++j) {
+                const int sc = (int)(k0 + j - Dz);
+                const cplx *r = spin_resp.data() + (size_t)(j - nsb) * per_mode;
                 for (long site = 0; site < V; ++site)
                     for (int col = 0; col < nc; ++col)
                         for (int sp = 0; sp < 4; ++sp) {
                             const cplx v = r[aos(site, col, sp)];
                             if (site == 0 && col == 1) spin_block[sc * 4 + sp] = cplx(0, 1) * v;
-                            else leak = std::max(leak, std::abs(v));
+                            else leak = std::max(leak, std::abs(v) <= 1.7976931348623157e308 ? std::abs(v) : INFINITY);
                         }
             }
         }
     }
     // Hermiticity of every block (the reference prints entries with |H - H^+|^2 > 1e-12), then the
     // Hermitian part
-    double asym = 0;
-    auto hermitize = [&](cplx *m, long n) {
-        for (long i = 0; i < n; ++i)
-            for (long j = 0; j <= i; ++j) {
-                cplx &a = m[(size_t)j * n + i], &b = m[(size_t)i * n + j];
+    double asym = 0, nonfinite = 0;
+    HIPCHK(c, ovl::launch_eig_hermitize(dA, Dz, n2, c->partial, c->st));
+    RETCHK(partial_max(2, 0, &asym));
+    RETCHK(partial_max(2, 1, &nonfinite));
+    if (nc > 1) {
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j <= i; ++j) {
+                cplx &a = spin_block[j * 4 + i], &b = spin_block[i * 4 + j];
+                if (!(std::isfinite(a.real()) && std::isfinite(a.imag()) && std::isfinite(b.real()) && std::isfinite(b.imag())))
+                    nonfinite = 1;
                 asym = std::max(asym, std::abs(a - std::conj(b)));
                 const cplx h = 0.5 * (a + std::conj(b));
                 a = h;
                 b = std::conj(h);
             }
-    };
-    for (int jz = 0; jz < n2; ++jz) hermitize(blocks.data() + (size_t)jz * Dz * Dz, Dz);
-    if (nc > 1) hermitize(spin_block, 4);
+    }
     if (hermiticity) *hermiticity = asym;
-    for (const cplx &v : blocks)
-        if (!std::isfinite(v.real()) || !std::isfinite(v.imag()))
-            return fail(c, OVL_E_SPECTRUM, "the assembled overlap operator has a non-finite entry");
+    if (nonfinite != 0) return fail(c, OVL_E_SPECTRUM, "the assembled overlap operator has a non-finite entry");
     if (!(leak <= 1e-12))
         return fail(c, OVL_E_SPECTRUM, "the U(1) overlap operator mixes colours (entry " + std::to_string(leak) + ")");
 
     // diagonalisation on the device (the reference: LAPACK zheev of the dense matrix,
     // MISC/LapackDiagonalization.cpp:68-120): all z-momentum blocks in one batched divide and
-    // conquer, the 4x4 spin block of the other colours as one more small problem
-    const int nprob = nc > 1 ? 2 : 1;
-    std::vector<double> Wz((size_t)n2 * Dz), W4(4, 0.0);
-    void *dA = nullptr, *dW = nullptr, *dE = nullptr, *dInfo = nullptr;
-    auto release = [&]() {
-        hipStreamSynchronize(c->st);
-        dfree(dA); dfree(dW); dfree(dE); dfree(dInfo);
-    };
-    int r = dalloc(c, &dA, sizeof(cplx) * blocks.size());
-    if (r == OVL_OK) r = dalloc(c, &dW, sizeof(double) * Wz.size());
-    if (r == OVL_OK) r = dalloc(c, &dE, sizeof(double) * Wz.size());
-    if (r == OVL_OK) r = dalloc(c, &dInfo, sizeof(int) * n2);
-    if (r != OVL_OK) { release(); return r; }
-    rocblas_handle handle = nullptr;
-    if (rocblas_create_handle(&handle) != rocblas_status_success) { release(); return fail(c, OVL_E_HIP, "rocblas_create_handle failed"); }
-    rocblas_set_stream(handle, c->st);
+    // conquer, in place; the 4x4 spin block of the other colours as one more small problem
     std::string why;
     for (int prob = 0; prob < nprob && why.empty(); ++prob) {
         const long n = prob == 0 ? Dz : 4;
         const int count = prob == 0 ? n2 : 1;
-        const cplx *src = prob == 0 ? blocks.data() : spin_block;
         double *w = prob == 0 ? Wz.data() : W4.data();
+        void *mat = prob == 0 ? dA : dS;
+        double *wdev = prob == 0 ? (double *)dW : (double *)((cplx *)dS + 16);
+        double *edev = prob == 0 ? (double *)dE : wdev + 4;
         std::vector<int> info(count, -1);
-        hipError_t he = hipMemcpyAsync(dA, src, sizeof(cplx) * (size_t)count * n * n, hipMemcpyHostToDevice, c->st);
+        hipError_t he = hipSuccess;
+        if (prob == 1) he = hipMemcpyAsync(mat, spin_block, sizeof(spin_block), hipMemcpyHostToDevice, c->st);
         rocblas_status rs = rocblas_status_success;
         if (he == hipSuccess)
             rs = rocsolver_zheevd_strided_batched(handle, rocblas_evect_original, rocblas_fill_lower, (rocblas_int)n,
-                                                  (rocblas_double_complex *)dA, (rocblas_int)n, (rocblas_stride)(n * n),
-                                                  (double *)dW, (rocblas_stride)n, (double *)dE, (rocblas_stride)n,
+                                                  (rocblas_double_complex *)mat, (rocblas_int)n, (rocblas_stride)(n * n),
+                                                  wdev, (rocblas_stride)n, edev, (rocblas_stride)n,
                                                   (rocblas_int *)dInfo, count);
         if (he == hipSuccess && rs == rocblas_status_success)
-            he = hipMemcpyAsync(w, dW, sizeof(double) * (size_t)count * n, hipMemcpyDeviceToHost, c->st);
-        if (he == hipSuccess && rs == rocblas_status_success)
-            he = hipMemcpyAsync(prob == 0 ? (void *)blocks.data() : (void *)spin_block, dA,
-                                sizeof(cplx) * (size_t)count * n * n, hipMemcpyDeviceToHost, c->st);
+            he = hipMemcpyAsync(w, wdev, sizeof(double) * (size_t)count * n, hipMemcpyDeviceToHost, c->st);
+        if (he == hipSuccess && rs == rocblas_status_success && prob == 1)
+            he = hipMemcpyAsync(spin_block, mat, sizeof(spin_block), hipMemcpyDeviceToHost, c->st);
         if (he == hipSuccess && rs == rocblas_status_success)
             he = hipMemcpyAsync(info.data(), dInfo, sizeof(int) * count, hipMemcpyDeviceToHost, c->st);
         if (he == hipSuccess) he = hipStreamSynchronize(c->st);
@@ -2484,10 +2516,12 @@ int ovl_qed_eigenbasis(ovl_ctx *c, const ovl_sign_opts *o, int nb, double q, dou
         for (long i = 0; i < (long)count * n; ++i)
             if (!std::isfinite(w[i])) why = "zheevd: non-finite eigenvalue";
     }
-    rocblas_destroy_handle(handle);
-    release();
     if (!why.empty()) return fail(c, OVL_E_CONVERGE, "eigensolver failed: " + why);
-    for (const cplx &v : blocks)
+    HIPCHK(c, ovl::launch_nonfinite(dA, (long)n2 * Dz * Dz * 2, c->partial, c->st));
+    double bad = 0;
+    RETCHK(partial_max(1, 0, &bad));
+    if (bad != 0) return fail(c, OVL_E_CONVERGE, "eigensolver failed: non-finite eigenvector entries");
+    for (const cplx &v : spin_block)
         if (!(std::isfinite(v.real()) && std::isfinite(v.imag())))
             return fail(c, OVL_E_CONVERGE, "eigensolver failed: non-finite eigenvector entries");
 
@@ -2512,6 +2546,7 @@ int ovl_qed_eigenbasis(ovl_ctx *c, const ovl_sign_opts *o, int nb, double q, dou
     // global mode index (FermionField.cpp:63-74, SetOverlapQEDEigenfunctions.cpp:14-44)
     std::vector<cplx> psi((size_t)nm * per_mode, cplx(0, 0)), mid((size_t)nm * per_mode, cplx(0, 0));
     const double norm = std::sqrt((double)V);
+    std::vector<cplx> chi((size_t)Dz);
     for (int sm = 0; sm < nm; ++sm) {
         const long g = (long)first_mode + sm, particle = g % 2, level = (g / 2) % (2L * V * nc);
         const long b = particle == 0 ? D / 2 - 1 - level : D / 2 + level;
@@ -2519,8 +2554,10 @@ int ovl_qed_eigenbasis(ovl_ctx *c, const ovl_sign_opts *o, int nb, double q, dou
         const cplx rot = std::exp(cplx(0, -0.5 * e.w * dt));
         cplx *ps = psi.data() + (size_t)sm * per_mode, *pm = mid.data() + (size_t)sm * per_mode;
         if (e.blk >= 0) {
-            // exp(i k z) chi(x, y, s) / sqrt(n2) on colour 0
-            const cplx *chi = blocks.data() + (size_t)e.blk * Dz * Dz + (size_t)e.idx * Dz;
+            // exp(i k z) chi(x, y, s) / sqrt(n2) on colour 0; chi is column idx of block blk on the device
+            HIPCHK(c, hipMemcpyAsync(chi.data(), (const cplx *)dA + (size_t)e.blk * Dz * Dz + (size_t)e.idx * Dz,
+                                     sizeof(cplx) * Dz, hipMemcpyDeviceToHost, c->st));
+            HIPCHK(c, hipStreamSynchronize(c->st));
             const double f = norm / std::sqrt((double)n2);
             for (int z = 0; z < n2; ++z) {
                 const cplx ph = f * std::conj(tw[(size_t)e.blk * n2 + z]);

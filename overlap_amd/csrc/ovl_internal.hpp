@@ -125,4 +125,19 @@ hipError_t launch_mscg_update(const Geom &g, int nshift, double *ps, double *chi
 hipError_t launch_axpy_modes(const Geom &g, void *y, const void *x, const double *coef, int coef_stride,
                              double sign, const int *active, hipStream_t st);
 
+// QED eigenbasis (FP64 fields): unit sources of one batch into a zeroed field (mode mm < cnt:
+// source k0 + mm, colour 0 of the z = 0 plane below dz = 4 n0 n1, colour 1 of site 0 above);
+// gather of the nsb colour-0 sources of a batch from the z-momentum planes `out` (per mode block,
+// n2 planes of n0 n1 4 Nc MB complex) into the column-major dz x dz blocks A; per-workgroup
+// partial results (eig_partial_blocks() of them, two per workgroup for the hermitize) of the
+// largest colour >= 1 entry of the first nvalid modes, of the Hermitian defect (and a non-finite
+// flag) while replacing each block by its Hermitian part, and of a non-finite flag over n doubles
+hipError_t launch_eig_sources(const Geom &g, void *field, long k0, int cnt, long dz, hipStream_t st);
+hipError_t launch_eig_gather(const Geom &g, const void *out, void *A, long k0, int nsb, hipStream_t st);
+hipError_t launch_eig_leak(const Geom &g, const void *field, int nvalid, double *partial, hipStream_t st);
+hipError_t launch_eig_hermitize(void *A, long dz, int n2, double *partial, hipStream_t st);
+hipError_t launch_nonfinite(const void *x, long n, double *partial, hipStream_t st);
+size_t eig_partial_len();
+int eig_partial_blocks();
+
 }  // namespace ovl

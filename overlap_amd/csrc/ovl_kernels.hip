@@ -6,6 +6,7 @@
 // read once per wavefront (MB = 64) or once per MB-lane group.
 #include "ovl_internal.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <mutex>
 #include <unordered_set>
@@ -1563,6 +1564,116 @@ hipError_t hop_mb(const Geom &g, const HopArgs &a, int wpe, hipStream_t st)
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------------------------
+// QED eigenbasis: unit sources, z-momentum blocks and their checks
+// ---------------------------------------------------------------------------------------------
+// One batch of unit sources in a zeroed FP64 field: mode mm < cnt gets a 1 at source src = k0 + mm,
+// i.e. colour 0, spin src % 4 at site src / 4 of the z = 0 plane when src < dz, else colour 1,
+// spin src - dz at site 0.
+template <int NC, int MB>
+__global__ __launch_bounds__(TPB) void k_eig_sources(double2 *field, int V, long k0, int cnt, long dz)
+{
+    constexpr int NCOMP = 4 * NC;
+    const int mm = blockIdx.x * TPB + threadIdx.x;
+    if (mm >= cnt) return;
+    const long src = k0 + mm;
+    const int blk = mm / MB, m = mm - blk * MB;
+    const long site = src < dz ? src / 4 : 0;
+    const int comp = src < dz ? (int)(src % 4) * NC : (int)(src - dz) * NC + 1;
+    field[(((size_t)blk * V + site) * NCOMP + comp) * MB + m] = make_double2(1.0, 0.0);
+}
+
+// A[j dz^2 + (k0 + mm) dz + row] = out[blk][j ncol + (xy NCOMP + sp NC) MB + m] for row = 4 xy + sp
+// and the first nsb modes mm (blk = mm / MB, m = mm % MB) of a batch; `out` holds per mode block
+// the n2 z-momentum planes of ncol = plane NCOMP MB entries each
+template <int NC, int MB>
+__global__ __launch_bounds__(TPB) void k_eig_gather(const double2 *__restrict__ out, double2 *__restrict__ A,
+                                                    long plane, int n2, long k0, int nsb)
+{
+    constexpr int NCOMP = 4 * NC;
+    const long dz = 4 * plane, ncol = plane * NCOMP * MB;
+    const long total = (long)n2 * nsb * dz;
+    for (long i = (long)blockIdx.x * TPB + threadIdx.x; i < total; i += (long)gridDim.x * TPB) {
+        const long row = i % dz, r2 = i / dz;
+        const int mm = (int)(r2 % nsb), j = (int)(r2 / nsb);
+        const int blk = mm / MB, m = mm - blk * MB;
+        const long xy = row / 4;
+        const int sp = (int)(row % 4);
+        A[(size_t)j * dz * dz + (size_t)(k0 + mm) * dz + row] =
+            out[((size_t)blk * n2 + j) * ncol + (size_t)(xy * NCOMP + sp * NC) * MB + m];
+    }
+}
+
+__device__ __forceinline__ double abs_or_inf(double2 x)
+{
+    const double a = sqrt(x.x * x.x + x.y * x.y);
+    return (a <= 1.7976931348623157e308) ? a : __builtin_inf();  // NaN and inf count as inf
+}
+
+// largest |entry| of colours >= 1 among the first nvalid modes of an FP64 field (non-finite entries
+// count as inf); one partial maximum per workgroup
+template <int NC, int MB>
+__global__ __launch_bounds__(TPB) void k_eig_leak(const double2 *__restrict__ f, int V, int nblk, int nvalid,
+                                                  double *partial)
+{
+    constexpr int NCOMP = 4 * NC;
+    double v[1] = {0.0};
+    const long total = (long)nblk * V * NCOMP * MB;
+    for (long i = (long)blockIdx.x * TPB + threadIdx.x; i < total; i += (long)gridDim.x * TPB) {
+        const int m = (int)(i % MB);
+        const long r = i / MB;
+        const int comp = (int)(r % NCOMP), blk = (int)(r / NCOMP / V);
+        if (comp % NC == 0 || blk * MB + m >= nvalid) continue;
+        v[0] = fmax(v[0], abs_or_inf(f[i]));
+    }
+    const int ism[1] = {1};
+    block_reduce<1>(v, ism);
+    if (threadIdx.x == 0) partial[blockIdx.x] = v[0];
+}
+
+// Hermitian part of each of the n2 column-major dz x dz blocks, in place: the pair (row, col),
+// (col, row) with row >= col is handled by one thread.  partial[2 b] = largest |A_rc - conj(A_cr)|,
+// partial[2 b + 1] = 1 when an entry is not finite.
+__global__ __launch_bounds__(TPB) void k_eig_hermitize(double2 *A, long dz, int n2, double *partial)
+{
+    double v[2] = {0.0, 0.0};
+    const long total = (long)n2 * dz * dz;
+    for (long i = (long)blockIdx.x * TPB + threadIdx.x; i < total; i += (long)gridDim.x * TPB) {
+        const long row = i % dz, r2 = i / dz;
+        const long col = r2 % dz, j = r2 / dz;
+        if (row < col) continue;
+        double2 *b = A + (size_t)j * dz * dz;
+        const double2 x = b[(size_t)col * dz + row], y = b[(size_t)row * dz + col];
+        const double2 d = make_double2(x.x - y.x, x.y + y.y);  // x - conj(y)
+        const double a = abs_or_inf(d);
+        if (a > 1.7976931348623157e308 || abs_or_inf(x) > 1.7976931348623157e308 ||
+            abs_or_inf(y) > 1.7976931348623157e308)
+            v[1] = 1.0;
+        else
+            v[0] = fmax(v[0], a);
+        const double2 h = make_double2(0.5 * (x.x + y.x), 0.5 * (x.y - y.y));
+        b[(size_t)col * dz + row] = h;
+        b[(size_t)row * dz + col] = make_double2(h.x, -h.y);
+    }
+    const int ism[2] = {1, 1};
+    block_reduce<2>(v, ism);
+    if (threadIdx.x == 0) {
+        partial[2 * blockIdx.x] = v[0];
+        partial[2 * blockIdx.x + 1] = v[1];
+    }
+}
+
+// 1 in partial[blockIdx.x] when one of the n doubles is not finite
+__global__ __launch_bounds__(TPB) void k_nonfinite(const double *__restrict__ x, long n, double *partial)
+{
+    double v[1] = {0.0};
+    for (long i = (long)blockIdx.x * TPB + threadIdx.x; i < n; i += (long)gridDim.x * TPB)
+        if (!(fabs(x[i]) <= 1.7976931348623157e308)) v[0] = 1.0;
+    const int ism[1] = {1};
+    block_reduce<1>(v, ism);
+    if (threadIdx.x == 0) partial[blockIdx.x] = v[0];
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------------------------
@@ -1776,5 +1887,59 @@ hipError_t launch_axpy_modes(const Geom &g, void *y, const void *x, const double
                                                g.nblk, g.mb);
     return hipGetLastError();
 }
+
+// ---------------------------------------------------------------------------------------------
+// QED eigenbasis launchers
+// ---------------------------------------------------------------------------------------------
+#define OVL_EIG_DISPATCH(KERNEL, GRID, ...)                                       \
+    do {                                                                          \
+        if (g.nc == 2 && g.mb == 16) KERNEL<2, 16><<<GRID, TPB, 0, st>>>(__VA_ARGS__); \
+        else if (g.nc == 2 && g.mb == 32) KERNEL<2, 32><<<GRID, TPB, 0, st>>>(__VA_ARGS__); \
+        else if (g.nc == 2 && g.mb == 64) KERNEL<2, 64><<<GRID, TPB, 0, st>>>(__VA_ARGS__); \
+        else if (g.nc == 3 && g.mb == 16) KERNEL<3, 16><<<GRID, TPB, 0, st>>>(__VA_ARGS__); \
+        else if (g.nc == 3 && g.mb == 32) KERNEL<3, 32><<<GRID, TPB, 0, st>>>(__VA_ARGS__); \
+        else if (g.nc == 3 && g.mb == 64) KERNEL<3, 64><<<GRID, TPB, 0, st>>>(__VA_ARGS__); \
+        else return hipErrorInvalidValue;                                         \
+    } while (0)
+
+hipError_t launch_eig_sources(const Geom &g, void *field, long k0, int cnt, long dz, hipStream_t st)
+{
+    if (cnt < 1 || cnt > g.nblk * g.mb) return hipErrorInvalidValue;
+    const unsigned grid = (unsigned)((cnt + TPB - 1) / TPB);
+    OVL_EIG_DISPATCH(k_eig_sources, grid, (double2 *)field, g.V, k0, cnt, dz);
+    return hipGetLastError();
+}
+
+hipError_t launch_eig_gather(const Geom &g, const void *out, void *A, long k0, int nsb, hipStream_t st)
+{
+    const long plane = (long)g.n0 * g.n1;
+    if (nsb < 1 || nsb > g.nblk * g.mb) return hipErrorInvalidValue;
+    const long total = (long)g.n2 * nsb * 4 * plane;
+    const unsigned grid = (unsigned)std::min<long>((total + TPB - 1) / TPB, 65536);
+    OVL_EIG_DISPATCH(k_eig_gather, grid, (const double2 *)out, (double2 *)A, plane, g.n2, k0, nsb);
+    return hipGetLastError();
+}
+
+size_t eig_partial_len() { return (size_t)2 * RED_BLOCKS; }
+int eig_partial_blocks() { return RED_BLOCKS; }
+
+hipError_t launch_eig_leak(const Geom &g, const void *field, int nvalid, double *partial, hipStream_t st)
+{
+    OVL_EIG_DISPATCH(k_eig_leak, RED_BLOCKS, (const double2 *)field, g.V, g.nblk, nvalid, partial);
+    return hipGetLastError();
+}
+
+hipError_t launch_eig_hermitize(void *A, long dz, int n2, double *partial, hipStream_t st)
+{
+    k_eig_hermitize<<<RED_BLOCKS, TPB, 0, st>>>((double2 *)A, dz, n2, partial);
+    return hipGetLastError();
+}
+
+hipError_t launch_nonfinite(const void *x, long n, double *partial, hipStream_t st)
+{
+    k_nonfinite<<<RED_BLOCKS, TPB, 0, st>>>((const double *)x, n, partial);
+    return hipGetLastError();
+}
+#undef OVL_EIG_DISPATCH
 
 }  // namespace ovl
