@@ -485,12 +485,14 @@ std::vector<double> clenshaw_sup(const std::vector<double> &c, double t0)
 }
 
 // Precision ladder of the mixed recurrence: the storage of b_k (k >= first) in a format with unit
-// roundoff u perturbs the result by about kappa * u * B_first.  kappa = 2 is 10-20x above the
-// errors measured with FP32 storage on random SU(3) fields (0.1-0.2 u32 B; 0.7 u32 B on a 6^3
-// lattice with 8 flux quanta) and with FP16 storage (0.06 u16 B on the same lattice); the coherent
-// worst case (errors aligned with the extreme eigenvector and summed linearly over steps) would
-// ask for 1e4x more and is not what rounding produces.  Returns the smallest `first` >= 1 such
-// that every step k >= first satisfies the bound (n + 1: none).
+// roundoff u perturbs the result by about kappa * u * B_first.  kappa is set per format from the
+// largest error measured for that format alone (storage roundings of the whole tail from `first`
+// on, host model and GPU, random SU(2)/SU(3) fields and a 6^3 SU(3) lattice with 8 flux quanta, the
+// worst case): FP32 0.1-0.7 u32 B and FP16 up to 0.58 u16 B -> kappa 2; 48-bit storage 0.013-0.052
+// u48 B (its tier sits at low k) -> kappa 0.25.  The coherent worst case (errors aligned with the
+// extreme eigenvector and summed linearly over steps) would ask for 1e4x more and is not what
+// rounding produces.  Returns the smallest `first` >= 1 such that every step k >= first satisfies
+// the bound (n + 1: none).
 int tier_start(const std::vector<double> &B, int n, double u, double tol, double kappa = 2.0)
 {
     int first = n + 1;
@@ -577,7 +579,7 @@ void plan_tiers(const ovl_sign_opts *o, const std::vector<double> &coef, int *he
         }
         *head48 = *head;
         if (fp48 && *head > 1) {
-            const int h = (o->head48 > 0) ? o->head48 : tier_start(B, n, u48, s48 * tol);
+            const int h = (o->head48 > 0) ? o->head48 : tier_start(B, n, u48, s48 * tol, 0.25);
             // the last step (k = 0) stays FP64: its output feeds the final FP64 stencil
             *head48 = std::min(std::max(h, 1), *head);
         }
