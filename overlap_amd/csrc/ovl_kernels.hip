@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cmath>
 #include <mutex>
+#include <type_traits>
 #include <unordered_set>
 
 namespace ovl {
@@ -325,26 +326,47 @@ __device__ __forceinline__ bool lane_coords(const HopArgs &a, int V, int &m, int
 __device__ __forceinline__ double fmac(double a, double b, double c) { return __builtin_fma(a, b, c); }
 __device__ __forceinline__ float fmac(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 
-template <int NC, int K, int SG, typename T, typename LT>
-__device__ __forceinline__ void hop_dir(const T *pr, const T *pi, const LT *__restrict__ U, T *xr, T *xi)
+// u + S l (S = +-1) rounded once in T.  Operands read from FP16 storage are widened inside one
+// mixed-precision FMA, u * one + S l with one = 1 held in a register the compiler cannot see
+// through (a literal 1 turns the FMA back into an add of two separately widened values): the same
+// value, without the two conversion instructions.
+template <int S, typename T, typename TP>
+__device__ __forceinline__ T padd(TP u, TP l, float one)
+{
+    if constexpr (std::is_same<TP, _Float16>::value)
+        return S > 0 ? __builtin_fmaf((float)u, one, (float)l) : __builtin_fmaf((float)u, one, -(float)l);
+    else
+        return u + S * l;
+}
+__device__ __forceinline__ float opaque_one()
+{
+    float one;
+    asm("s_mov_b32 %0, 1.0" : "=s"(one));
+    return one;
+}
+
+// TP: element type of the input spinor as loaded (the storage type for FP16, else T)
+template <int NC, int K, int SG, typename TP, typename T, typename LT>
+__device__ __forceinline__ void hop_dir(const TP *pr, const TP *pi, const LT *__restrict__ U, T *xr, T *xi)
 {
     // spinor component index: spin*NC + colour; u = spins 0,1, l = spins 2,3
     T hr[2][NC], hi[2][NC];
+    const float one = std::is_same<TP, _Float16>::value ? opaque_one() : 1.0f;
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
-        const T u0r = pr[0 * NC + c], u0i = pi[0 * NC + c];
-        const T u1r = pr[1 * NC + c], u1i = pi[1 * NC + c];
-        const T l0r = pr[2 * NC + c], l0i = pi[2 * NC + c];
-        const T l1r = pr[3 * NC + c], l1i = pi[3 * NC + c];
+        const TP u0r = pr[0 * NC + c], u0i = pi[0 * NC + c];
+        const TP u1r = pr[1 * NC + c], u1i = pi[1 * NC + c];
+        const TP l0r = pr[2 * NC + c], l0i = pi[2 * NC + c];
+        const TP l1r = pr[3 * NC + c], l1i = pi[3 * NC + c];
         if constexpr (K == 0) {  // i sigma_x l = (i l1, i l0)
-            hr[0][c] = u0r - SG * l1i; hi[0][c] = u0i + SG * l1r;
-            hr[1][c] = u1r - SG * l0i; hi[1][c] = u1i + SG * l0r;
+            hr[0][c] = padd<-SG, T>(u0r, l1i, one); hi[0][c] = padd<SG, T>(u0i, l1r, one);
+            hr[1][c] = padd<-SG, T>(u1r, l0i, one); hi[1][c] = padd<SG, T>(u1i, l0r, one);
         } else if constexpr (K == 1) {  // i sigma_y l = (l1, -l0)
-            hr[0][c] = u0r + SG * l1r; hi[0][c] = u0i + SG * l1i;
-            hr[1][c] = u1r - SG * l0r; hi[1][c] = u1i - SG * l0i;
+            hr[0][c] = padd<SG, T>(u0r, l1r, one); hi[0][c] = padd<SG, T>(u0i, l1i, one);
+            hr[1][c] = padd<-SG, T>(u1r, l0r, one); hi[1][c] = padd<-SG, T>(u1i, l0i, one);
         } else {  // i sigma_z l = (i l0, -i l1)
-            hr[0][c] = u0r - SG * l0i; hi[0][c] = u0i + SG * l0r;
-            hr[1][c] = u1r + SG * l1i; hi[1][c] = u1i - SG * l1r;
+            hr[0][c] = padd<-SG, T>(u0r, l0i, one); hi[0][c] = padd<SG, T>(u0i, l0r, one);
+            hr[1][c] = padd<SG, T>(u1r, l1i, one); hi[1][c] = padd<-SG, T>(u1i, l1r, one);
         }
     }
     T vr[2][NC], vi[2][NC];
@@ -413,12 +435,14 @@ __device__ __forceinline__ void hop_site(const HopArgs &a, int n0, int n1, int n
     const LT *__restrict__ L = reinterpret_cast<const LT *>(a.links) + (size_t)l0 * NC * NC;
     constexpr int NN = NC * NC;
 
+    // FP16 input stays in FP16 registers until the arithmetic widens it (hop_dir's projection)
+    using PT = typename std::conditional<std::is_same<TI, _Float16>::value, _Float16, CI>::type;
     CI xr[NCOMP], xi[NCOMP];
-    CI pr[NCOMP], pi[NCOMP];
+    PT pr[NCOMP], pi[NCOMP];
     load_sp<NC, MB, TI>(a.in, 0, 0, q0 + s, m, pr, pi);
     const CI am = (CI)a.amass;
 #pragma unroll
-    for (int c = 0; c < NCOMP; ++c) { xr[c] = am * pr[c]; xi[c] = am * pi[c]; }
+    for (int c = 0; c < NCOMP; ++c) { xr[c] = am * (CI)pr[c]; xi[c] = am * (CI)pi[c]; }
 
     {  // x direction
         const int sp = s + ((x + 1 == n0) ? 1 - n0 : 1);
