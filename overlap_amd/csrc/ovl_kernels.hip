@@ -419,7 +419,9 @@ __device__ __forceinline__ void hop_dir(const TP *pr, const TP *pi, const LT *__
 // (WilsonOperator.cpp:30-114 computes -i h_W of one mode; here every mode of every block).
 // The site s lives on an n0 x n1 x n2 lattice; its spinors are site-block q0 + s of the fields
 // (q0 = first site-block of this lane's mode block), its links start at site index l0.
-template <int NC, typename TI, typename TO, int MB>
+// EPI = false: a pure launch (out = sh h_W(in)); the pointwise operands are then known to be absent
+// at compile time, which frees the registers their arguments would hold
+template <int NC, typename TI, typename TO, int MB, bool EPI = true>
 __device__ __forceinline__ void hop_site(const HopArgs &a, int n0, int n1, int n2, int s, int q0, int l0, int m)
 {
     constexpr int NCOMP = 4 * NC;
@@ -479,8 +481,9 @@ __device__ __forceinline__ void hop_site(const HopArgs &a, int n0, int n1, int n
         orr[c] = shr * hr - shi * hi;
         oi[c] = shr * hi + shi * hr;
     }
-    CO tr[NCOMP], ti[NCOMP];
     const int q = q0 + s;
+    if constexpr (EPI) {
+    CO tr[NCOMP], ti[NCOMP];
     if (a.x) {
         load_sp_nt<NC, MB, TO>(a.x, 0, 0, q, m, tr, ti);
         const CO cr = (CO)a.sx_r, ci = (CO)a.sx_i;
@@ -517,6 +520,7 @@ __device__ __forceinline__ void hop_site(const HopArgs &a, int n0, int n1, int n
 #pragma unroll
         for (int c = 0; c < NCOMP; ++c) { orr[c] += cr * tr[c] - ci * ti[c]; oi[c] += cr * ti[c] + ci * tr[c]; }
     }
+    }
     if (a.nt_out) store_sp_nt<NC, MB, TO>(a.out, 0, 0, q, m, orr, oi);
     else store_sp<NC, MB, TO>(a.out, 0, 0, q, m, orr, oi);
 }
@@ -524,7 +528,7 @@ __device__ __forceinline__ void hop_site(const HopArgs &a, int n0, int n1, int n
 // WPE: minimum waves per SIMD the register allocation must allow (1 = compiler's choice); the
 // stencil hoists its neighbour loads, so this trades memory-level parallelism per wave against
 // waves in flight.  The host picks the variant per problem shape (autotune).
-template <int NC, typename TI, typename TO, int MB, int WPE>
+template <int NC, typename TI, typename TO, int MB, int WPE, bool EPI>
 __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(WPE))) void k_hop(HopArgs a, int n0, int n1,
                                                                                        int n2)
 {
@@ -537,7 +541,7 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(WPE))) void
         s = __builtin_amdgcn_readfirstlane(s);
         blk = __builtin_amdgcn_readfirstlane(blk);
     }
-    hop_site<NC, TI, TO, MB>(a, n0, n1, n2, s, blk * V, 0, m);
+    hop_site<NC, TI, TO, MB, EPI>(a, n0, n1, n2, s, blk * V, 0, m);
 }
 
 // Ensemble launch: many independent lattices of different sizes in one grid.  Workgroup b works on
@@ -1528,19 +1532,27 @@ hipError_t reserve_lds(const void *kernel)
     return hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, LDS_ONE_WG);
 }
 
+template <int NC, typename TI, typename TO, int MB, bool EPI>
+void hop_wpe_epi(const HopArgs &b, unsigned nb, int wpe, const Geom &g, hipStream_t st)
+{
+    switch (wpe) {
+    case 2: k_hop<NC, TI, TO, MB, 2, EPI><<<nb, TPB, 0, st>>>(b, g.n0, g.n1, g.n2); break;
+    case 3: k_hop<NC, TI, TO, MB, 3, EPI><<<nb, TPB, 0, st>>>(b, g.n0, g.n1, g.n2); break;
+    case 4: k_hop<NC, TI, TO, MB, 4, EPI><<<nb, TPB, 0, st>>>(b, g.n0, g.n1, g.n2); break;
+    case 5:
+        (void)reserve_lds(reinterpret_cast<const void *>(k_hop<NC, TI, TO, MB, 1, EPI>));
+        k_hop<NC, TI, TO, MB, 1, EPI><<<nb, TPB, LDS_ONE_WG, st>>>(b, g.n0, g.n1, g.n2);
+        break;
+    default: k_hop<NC, TI, TO, MB, 1, EPI><<<nb, TPB, 0, st>>>(b, g.n0, g.n1, g.n2); break;
+    }
+}
+
 template <int NC, typename TI, typename TO, int MB>
 void hop_wpe(const HopArgs &b, unsigned nb, int wpe, const Geom &g, hipStream_t st)
 {
-    switch (wpe) {
-    case 2: k_hop<NC, TI, TO, MB, 2><<<nb, TPB, 0, st>>>(b, g.n0, g.n1, g.n2); break;
-    case 3: k_hop<NC, TI, TO, MB, 3><<<nb, TPB, 0, st>>>(b, g.n0, g.n1, g.n2); break;
-    case 4: k_hop<NC, TI, TO, MB, 4><<<nb, TPB, 0, st>>>(b, g.n0, g.n1, g.n2); break;
-    case 5:
-        (void)reserve_lds(reinterpret_cast<const void *>(k_hop<NC, TI, TO, MB, 1>));
-        k_hop<NC, TI, TO, MB, 1><<<nb, TPB, LDS_ONE_WG, st>>>(b, g.n0, g.n1, g.n2);
-        break;
-    default: k_hop<NC, TI, TO, MB, 1><<<nb, TPB, 0, st>>>(b, g.n0, g.n1, g.n2); break;
-    }
+    // the pure launch of the Clenshaw step (no pointwise operand) has a variant of its own
+    if (b.x || b.b1 || b.b2 || b.tgt || b.out2) hop_wpe_epi<NC, TI, TO, MB, true>(b, nb, wpe, g, st);
+    else hop_wpe_epi<NC, TI, TO, MB, false>(b, nb, wpe, g, st);
 }
 
 template <int NC, typename TI, typename TO, int MB>
