@@ -87,9 +87,11 @@ def _pmc_pass(counter: str, child_args: list, outdir: str, timeout: float):
 
 def _kernel_name(n: str) -> str:
     """Kernel name as in the source (rocprofv3 leaves the _Float16 instantiations mangled)."""
-    m = re.match(r"_ZN3ovl12_GLOBAL__N_1\d+(k_hop(?:_ens)?)ILi(\d)EDF16_DF16_Li(\d+)ELi(\d)E", n)
+    m = re.match(r"_ZN3ovl12_GLOBAL__N_1\d+(k_hop(?:_ens)?)ILi(\d)EDF16_DF16_Li(\d+)ELi(\d)E(?:Lb(\d)E)?", n)
     if m:
-        return "%s<%s, _Float16, _Float16, %s, %s>" % m.groups()
+        g = m.groups()
+        epi = "" if g[4] is None else (", true" if g[4] == "1" else ", false")
+        return "%s<%s, _Float16, _Float16, %s, %s%s>" % (g[0], g[1], g[2], g[3], epi)
     return n.replace("void ", "").replace("ovl::(anonymous namespace)::", "")
 
 
