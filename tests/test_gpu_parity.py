@@ -519,16 +519,19 @@ def test_observed_update_currents(gpu):
     assert np.abs(jv - jv0).max() < 1e-13 and np.abs(ja - ja0).max() < 1e-13
 
 
-@pytest.mark.parametrize("nc,n,nb", [(2, (4, 4, 3), 2), (3, (3, 3, 4), 1)])
-def test_qed_eigenbasis_vs_dense_oracle(gpu, nc, n, nb):
+@pytest.mark.parametrize("nc,n,nb,nm,mb", [(2, (4, 4, 3), 2, 16, 16), (3, (3, 3, 4), 1, 16, 16),
+                                           (2, (4, 4, 3), 2, 48, 16), (3, (3, 3, 4), 1, 64, 32)])
+def test_qed_eigenbasis_vs_dense_oracle(gpu, nc, n, nb, nm, mb):
     # the overlap Hamiltonian of the U(1) background alone (colour 0 carries the phase), assembled
-    # on the CPU from the oracle's overlap apply and diagonalised by numpy, against the GPU's
-    print(f"[case] qed eigenbasis nc={nc} n={n} nb={nb}", flush=True)
+    # on the CPU from the oracle's overlap apply and diagonalised by numpy, against the GPU's.  The
+    # cases cover one and several mode blocks per batch of unit sources (the z-momentum transform
+    # is one batched product over the blocks), several batches, and the colour-1 spin sources in
+    # a batch of their own, after the last colour-0 sources in the same batch, and in a later block
+    print(f"[case] qed eigenbasis nc={nc} n={n} nb={nb} modes={nm} mb={mb}", flush=True)
     V = int(np.prod(n))
     D = 4 * nc * V
     deg, cut, M, dt = 64, 0.1, 1.2, 0.02
-    nm = 16
-    ctx = ovl.Context(nc, n, nm, m_dw=M, mode_block=16)
+    ctx = ovl.Context(nc, n, nm, m_dw=M, mode_block=mb)
     ev, herm = ctx.qed_eigenbasis(ovl.sign_opts(degree=deg, cut=cut, precision=ovl.PREC_FP64), nb, 1.0, dt,
                                   first_mode=2)
     L = orc.lat(nc, n, 1.0, 0.0, M)
