@@ -61,9 +61,13 @@ __device__ __forceinline__ double w_as_double(unsigned lo, unsigned hi)
 // v6 / v7 from the low 16 bits of the three words that carry its pieces
 __device__ __forceinline__ double pieces_as_double(unsigned w0, unsigned w1, unsigned w2)
 {
-    return w_as_double(w0 << 16, (w2 << 16) | (w1 & 0xffffu));
+    // high word: low half of w1 below low half of w2, one byte permute
+    return w_as_double(w0 << 16, __builtin_amdgcn_perm(w2, w1, 0x05040100u));
 }
-template <typename C>
+// TRUNC: the six words keep their top 48 bits as they are (v_i's low 16 bits are replaced by the
+// piece: within 2^-36 instead of 2^-37) and v6 / v7 are truncated to 48 bits; two instructions per
+// word instead of about twelve.  Used for the intermediate w = h b of a Clenshaw step.
+template <bool TRUNC = false, typename C>
 __device__ __forceinline__ void pack48(const C *re, const C *im, int u, u4v &A, u4v &B, u4v &M)
 {
     double v[8];
@@ -72,12 +76,23 @@ __device__ __forceinline__ void pack48(const C *re, const C *im, int u, u4v &A, 
         v[2 * j] = (double)re[4 * u + j];
         v[2 * j + 1] = (double)im[4 * u + j];
     }
-    const unsigned long long r6 = r48(v[6]), r7 = r48(v[7]);
     unsigned long long w[6];
+    if constexpr (TRUNC) {
+        const unsigned long long r6 = (unsigned long long)__double_as_longlong(v[6]);
+        const unsigned long long r7 = (unsigned long long)__double_as_longlong(v[7]);
 #pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        w[i] = w48(v[i], (r6 >> (16 * (i + 1))) & 0xffffull);
-        w[i + 3] = w48(v[i + 3], (r7 >> (16 * (i + 1))) & 0xffffull);
+        for (int i = 0; i < 3; ++i) {
+            w[i] = ((unsigned long long)__double_as_longlong(v[i]) & ~0xffffull) | ((r6 >> (16 * (i + 1))) & 0xffffull);
+            w[i + 3] =
+                ((unsigned long long)__double_as_longlong(v[i + 3]) & ~0xffffull) | ((r7 >> (16 * (i + 1))) & 0xffffull);
+        }
+    } else {
+        const unsigned long long r6 = r48(v[6]), r7 = r48(v[7]);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            w[i] = w48(v[i], (r6 >> (16 * (i + 1))) & 0xffffull);
+            w[i + 3] = w48(v[i + 3], (r7 >> (16 * (i + 1))) & 0xffffull);
+        }
     }
     A = {(unsigned)w[0], (unsigned)(w[0] >> 32), (unsigned)w[1], (unsigned)(w[1] >> 32)};
     B = {(unsigned)w[2], (unsigned)(w[2] >> 32), (unsigned)w[3], (unsigned)(w[3] >> 32)};
@@ -208,7 +223,7 @@ __device__ __forceinline__ h8v pack_h8(const C *re, const C *im, int u)
     return v;
 }
 
-template <int NC, int MB, typename T, typename C>
+template <int NC, int MB, typename T, bool TRUNC = false, typename C>
 __device__ __forceinline__ void store_sp(void *__restrict__ base, int V, int blk, int s, int m, const C *re,
                                          const C *im)
 {
@@ -218,7 +233,7 @@ __device__ __forceinline__ void store_sp(void *__restrict__ base, int V, int blk
 #pragma unroll
         for (int u = 0; u < NCOMP / 4; ++u) {
             u4v A, B, M;
-            pack48(re, im, u, A, B, M);
+            pack48<TRUNC>(re, im, u, A, B, M);
             p[(size_t)(3 * u) * MB] = A;
             p[(size_t)(3 * u + 1) * MB] = B;
             p[(size_t)(3 * u + 2) * MB] = M;
@@ -240,7 +255,7 @@ __device__ __forceinline__ void store_sp(void *__restrict__ base, int V, int blk
 }
 
 // streaming store of an output that the launch never reads back (keeps L2 for the stencil input)
-template <int NC, int MB, typename T, typename C>
+template <int NC, int MB, typename T, bool TRUNC = false, typename C>
 __device__ __forceinline__ void store_sp_nt(void *__restrict__ base, int V, int blk, int s, int m, const C *re,
                                             const C *im)
 {
@@ -250,7 +265,7 @@ __device__ __forceinline__ void store_sp_nt(void *__restrict__ base, int V, int 
 #pragma unroll
         for (int u = 0; u < NCOMP / 4; ++u) {
             u4v A, B, M;
-            pack48(re, im, u, A, B, M);
+            pack48<TRUNC>(re, im, u, A, B, M);
             __builtin_nontemporal_store(A, p + (size_t)(3 * u) * MB);
             __builtin_nontemporal_store(B, p + (size_t)(3 * u + 1) * MB);
             __builtin_nontemporal_store(M, p + (size_t)(3 * u + 2) * MB);
@@ -521,8 +536,9 @@ __device__ __forceinline__ void hop_site(const HopArgs &a, int n0, int n1, int n
         for (int c = 0; c < NCOMP; ++c) { orr[c] += cr * tr[c] - ci * ti[c]; oi[c] += cr * ti[c] + ci * tr[c]; }
     }
     }
-    if (a.nt_out) store_sp_nt<NC, MB, TO>(a.out, 0, 0, q, m, orr, oi);
-    else store_sp<NC, MB, TO>(a.out, 0, 0, q, m, orr, oi);
+    // the pure launch's output w is an intermediate of the step: 48-bit storage truncates it
+    if (a.nt_out) store_sp_nt<NC, MB, TO, !EPI>(a.out, 0, 0, q, m, orr, oi);
+    else store_sp<NC, MB, TO, !EPI>(a.out, 0, 0, q, m, orr, oi);
 }
 
 // WPE: minimum waves per SIMD the register allocation must allow (1 = compiler's choice); the
