@@ -34,6 +34,27 @@ struct Geom {
     int nblk;    // mode blocks held per field
 };
 
+// Division by a launch constant d without a division sequence: for 0 <= n < 2^31,
+// n / d = umulhi(n, mul) >> shift with l = ceil(log2 d), mul = ceil(2^(31+l) / d) < 2^32, shift = l - 1
+// (Granlund and Montgomery, "Division by invariant integers using multiplication", 1994: m d -
+// 2^(31+l) < d <= 2^l); d = 1 is the identity.  On a wave-uniform n the product is one scalar
+// instruction.
+struct FastDiv {
+    unsigned mul;
+    int shift;
+    unsigned d;
+};
+inline FastDiv make_fastdiv(unsigned d)
+{
+    FastDiv f{0u, 0, d};
+    if (d <= 1) return f;
+    int l = 0;
+    while ((1ull << l) < d) ++l;
+    f.mul = (unsigned)(((1ull << (31 + l)) + d - 1) / d);
+    f.shift = l - 1;
+    return f;
+}
+
 // pointwise epilogue of the stencil kernel:
 //   out = sh * h_W(in) + sx * G(x) + s1 * b1 + s2 * b2 + st * tgt      (G = 1 or gamma0)
 // the five operands are block-relative pointers (already offset to the first block launched);
@@ -61,6 +82,9 @@ struct HopArgs {
     const SysDesc *ens_sys;
     const int2 *ens_work;
     int ens_grid;
+    // lane coordinates of a single-lattice launch (set by the launcher): V, nblk V / 8, V / 8,
+    // n0 n1, n0; lanes are counted in 32 bits (nblk V MB < 2^31)
+    FastDiv dv_V, dv_per, dv_v8, dv_n01, dv_n0;
 };
 
 // lane traversals of the stencil launch (see lane_coords)

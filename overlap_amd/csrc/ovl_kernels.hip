@@ -302,30 +302,33 @@ __device__ __forceinline__ void store_sp_nt(void *__restrict__ base, int V, int 
 //     block's site order (a z range, or a range of y slabs) -- with sites in the order of `order`
 //     (y slabs, z-major) or lexicographic, so few sites are fetched by two XCDs and the
 //     z-neighbour planes of a slab stay in L2.
+__device__ __forceinline__ unsigned fdiv(unsigned n, const FastDiv &f)
+{
+    return f.d <= 1 ? n : (__umulhi(n, f.mul) >> f.shift);
+}
+
 template <int MB>
 __device__ __forceinline__ bool lane_coords(const HopArgs &a, int V, int &m, int &s, int &blk)
 {
-    const long total = (long)a.nblk * V * MB;
-    long lb = blockIdx.x;
-    if (a.trav != TRAV_LEX) lb = (long)(blockIdx.x % NUM_XCD) * (gridDim.x / NUM_XCD) + blockIdx.x / NUM_XCD;
-    const long gl = lb * TPB + threadIdx.x;
+    const unsigned total = (unsigned)a.nblk * (unsigned)V * MB;  // < 2^31, checked by the launcher
+    unsigned lb = blockIdx.x;
+    if (a.trav != TRAV_LEX) lb = (blockIdx.x % NUM_XCD) * (gridDim.x / NUM_XCD) + blockIdx.x / NUM_XCD;
+    const unsigned gl = lb * TPB + threadIdx.x;
     if (gl >= total) return false;
     m = (int)(gl % MB);
-    const long r = gl / MB;
+    unsigned r = gl / MB;
+    if constexpr (MB == 64) r = __builtin_amdgcn_readfirstlane(r);  // one site per wavefront
     if (a.trav == TRAV_BOXES) {
-        const long per = (long)a.nblk * V / NUM_XCD;  // traversal slots per XCD
-        const int xc = (int)(r / per);
-        const long rr = r - xc * per;
-        const int v8 = V / NUM_XCD;
-        blk = (int)(rr / v8);
-        s = a.order[xc * v8 + (int)(rr - (long)blk * v8)];
-    } else if (a.trav == TRAV_MODES) {
-        blk = (int)(r / V);
-        const int i = (int)(r - (long)blk * V);
-        s = a.order ? a.order[i] : i;
+        const unsigned xc = fdiv(r, a.dv_per);  // XCD of the traversal slot (nblk V / 8 slots each)
+        const unsigned rr = r - xc * a.dv_per.d;
+        const unsigned b = fdiv(rr, a.dv_v8);
+        blk = (int)b;
+        s = a.order[xc * a.dv_v8.d + (rr - b * a.dv_v8.d)];
     } else {
-        s = (int)(r % V);
-        blk = (int)(r / V);
+        const unsigned b = fdiv(r, a.dv_V);
+        const unsigned i = r - b * (unsigned)V;
+        blk = (int)b;
+        s = (a.trav == TRAV_MODES && a.order) ? a.order[i] : (int)i;
     }
     return true;
 }
@@ -437,17 +440,14 @@ __device__ __forceinline__ void hop_dir(const TP *pr, const TP *pi, const LT *__
 // EPI = false: a pure launch (out = sh h_W(in)); the pointwise operands are then known to be absent
 // at compile time, which frees the registers their arguments would hold
 template <int NC, typename TI, typename TO, int MB, bool EPI = true>
-__device__ __forceinline__ void hop_site(const HopArgs &a, int n0, int n1, int n2, int s, int q0, int l0, int m)
+__device__ __forceinline__ void hop_site(const HopArgs &a, int n0, int n1, int n2, int s, int x, int y, int z, int q0,
+                                         int l0, int m)
 {
     constexpr int NCOMP = 4 * NC;
     using LT = typename Vec<TI>::L;
     using CI = typename Vec<TI>::C;  // arithmetic type of the stencil
     using CO = typename Vec<TO>::C;  // arithmetic type of the epilogue
     const int V = n0 * n1 * n2;
-    const int z = s / (n0 * n1);
-    const int rem = s - z * n0 * n1;
-    const int y = rem / n0;
-    const int x = rem - y * n0;
 
     const LT *__restrict__ L = reinterpret_cast<const LT *>(a.links) + (size_t)l0 * NC * NC;
     constexpr int NN = NC * NC;
@@ -557,7 +557,12 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(WPE))) void
         s = __builtin_amdgcn_readfirstlane(s);
         blk = __builtin_amdgcn_readfirstlane(blk);
     }
-    hop_site<NC, TI, TO, MB, EPI>(a, n0, n1, n2, s, blk * V, 0, m);
+    // site coordinates (s = x + n0 (y + n1 z))
+    const int z = (int)fdiv((unsigned)s, a.dv_n01);
+    const int rem = s - z * (int)a.dv_n01.d;
+    const int y = (int)fdiv((unsigned)rem, a.dv_n0);
+    const int x = rem - y * n0;
+    hop_site<NC, TI, TO, MB, EPI>(a, n0, n1, n2, s, x, y, z, blk * V, 0, m);
 }
 
 // Ensemble launch: many independent lattices of different sizes in one grid.  Workgroup b works on
@@ -579,7 +584,10 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(WPE))) void
         s = __builtin_amdgcn_readfirstlane(s);
         blk = __builtin_amdgcn_readfirstlane(blk);
     }
-    hop_site<NC, TI, TO, MB>(a, d.n0, d.n1, d.n2, s, d.q0 + blk * d.V, d.l0, m);
+    const int z = s / (d.n0 * d.n1);
+    const int rem = s - z * d.n0 * d.n1;
+    const int y = rem / d.n0;
+    hop_site<NC, TI, TO, MB>(a, d.n0, d.n1, d.n2, s, rem - y * d.n0, y, z, d.q0 + blk * d.V, d.l0, m);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1601,8 +1609,14 @@ hipError_t hop_mb(const Geom &g, const HopArgs &a, int wpe, hipStream_t st)
         return hipGetLastError();
     }
     const long total = (long)a.nblk * g.V * g.mb;
+    if (total >= (1l << 31)) return hipErrorInvalidValue;  // lanes are counted in 32 bits
     const unsigned nb = nblocks(total);
     HopArgs b = a;
+    b.dv_V = make_fastdiv((unsigned)g.V);
+    b.dv_per = make_fastdiv((unsigned)((long)a.nblk * g.V / NUM_XCD));
+    b.dv_v8 = make_fastdiv((unsigned)(g.V / NUM_XCD));
+    b.dv_n01 = make_fastdiv((unsigned)(g.n0 * g.n1));
+    b.dv_n0 = make_fastdiv((unsigned)g.n0);
     // the XCD traversals need whole workgroups spread evenly over the XCDs
     const bool even = total % ((long)TPB * NUM_XCD) == 0;
     if (b.trav == TRAV_BOXES && (!even || g.V % NUM_XCD != 0 || !b.order)) b.trav = TRAV_LEX;
