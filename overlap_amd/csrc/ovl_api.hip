@@ -1141,6 +1141,11 @@ static int ctx_init(ovl_ctx *c)
         c->ens_imbalance = (double)lmax * ovl::NUM_XCD / (double)ltot;
         RETCHK(dalloc(c, (void **)&c->d_work, sizeof(int2) * tab.size()));
         RETCHK(dalloc(c, (void **)&c->d_sys, sizeof(SysDesc) * c->sys.size()));
+        for (SysDesc &d : c->sys) {
+            d.dv_V = ovl::make_fastdiv((unsigned)d.V);
+            d.dv_n01 = ovl::make_fastdiv((unsigned)(d.n0 * d.n1));
+            d.dv_n0 = ovl::make_fastdiv((unsigned)d.n0);
+        }
         HIPCHK(c, hipMemcpy(c->d_work, tab.data(), sizeof(int2) * tab.size(), hipMemcpyHostToDevice));
         HIPCHK(c, hipMemcpy(c->d_sys, c->sys.data(), sizeof(SysDesc) * c->sys.size(), hipMemcpyHostToDevice));
     }

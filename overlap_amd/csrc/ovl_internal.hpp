@@ -17,23 +17,6 @@
 
 namespace ovl {
 
-// One lattice of an ensemble context: its fields are site-blocks [q0, q0 + nblk*V) of the
-// concatenated fields (block-major inside the system), its links start at link index l0 (= 3 x
-// the number of sites of the systems before it).
-struct SysDesc {
-    int n0, n1, n2, V;
-    int nblk;
-    int q0;
-    int l0;
-    int pad;
-};
-
-struct Geom {
-    int nc, n0, n1, n2, V;
-    int mb;      // modes per block
-    int nblk;    // mode blocks held per field
-};
-
 // Division by a launch constant d without a division sequence: for 0 <= n < 2^31,
 // n / d = umulhi(n, mul) >> shift with l = ceil(log2 d), mul = ceil(2^(31+l) / d) < 2^32, shift = l - 1
 // (Granlund and Montgomery, "Division by invariant integers using multiplication", 1994: m d -
@@ -54,6 +37,24 @@ inline FastDiv make_fastdiv(unsigned d)
     f.shift = l - 1;
     return f;
 }
+
+// One lattice of an ensemble context: its fields are site-blocks [q0, q0 + nblk*V) of the
+// concatenated fields (block-major inside the system), its links start at link index l0 (= 3 x
+// the number of sites of the systems before it).
+struct SysDesc {
+    int n0, n1, n2, V;
+    int nblk;
+    int q0;
+    int l0;
+    int pad;
+    FastDiv dv_V, dv_n01, dv_n0;  // V, n0 n1, n0 (filled before the table goes to the device)
+};
+
+struct Geom {
+    int nc, n0, n1, n2, V;
+    int mb;      // modes per block
+    int nblk;    // mode blocks held per field
+};
 
 // pointwise epilogue of the stencil kernel:
 //   out = sh * h_W(in) + sx * G(x) + s1 * b1 + s2 * b2 + st * tgt      (G = 1 or gamma0)

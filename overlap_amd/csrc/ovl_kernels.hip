@@ -575,18 +575,16 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(WPE))) void
     const int2 w = a.ens_work[blockIdx.x];
     if (w.x < 0) return;
     const SysDesc d = a.ens_sys[w.x];
-    const int gl = w.y * TPB + threadIdx.x;  // lane within the system
-    if (gl >= d.V * d.nblk * MB) return;
-    const int m = gl % MB;
-    int r = gl / MB;
-    int s = r % d.V, blk = r / d.V;
-    if constexpr (MB == 64) {
-        s = __builtin_amdgcn_readfirstlane(s);
-        blk = __builtin_amdgcn_readfirstlane(blk);
-    }
-    const int z = s / (d.n0 * d.n1);
+    const unsigned gl = (unsigned)w.y * TPB + threadIdx.x;  // lane within the system
+    if (gl >= (unsigned)(d.V * d.nblk * MB)) return;
+    const int m = (int)(gl % MB);
+    unsigned r = gl / MB;
+    if constexpr (MB == 64) r = __builtin_amdgcn_readfirstlane(r);  // one site per wavefront
+    const int blk = (int)fdiv(r, d.dv_V);
+    const int s = (int)r - blk * d.V;
+    const int z = (int)fdiv((unsigned)s, d.dv_n01);
     const int rem = s - z * d.n0 * d.n1;
-    const int y = rem / d.n0;
+    const int y = (int)fdiv((unsigned)rem, d.dv_n0);
     hop_site<NC, TI, TO, MB>(a, d.n0, d.n1, d.n2, s, rem - y * d.n0, y, z, d.q0 + blk * d.V, d.l0, m);
 }
 
