@@ -131,8 +131,9 @@ def measure_traffic(args, c: cf.Config, prefix: str, n_last: int, tuning: str):
                 return None
             factor = field_bytes / (float(calib[0]["Counter_Value"]) * 1024.0)
             per = [float(r["Counter_Value"]) * 1024.0 * factor for r in hops[-n_last:]]
-            res[counter] = {"factor": factor, "bytes_per_launch": sum(per) / len(per),
-                            "kernel": name(hops[-1]).split("(")[0]}
+            # the variant's launches of the step: pure h_W and epilogue instantiations, averaged
+            kinds = sorted({name(r).split("(")[0] for r in hops[-n_last:]})
+            res[counter] = {"factor": factor, "bytes_per_launch": sum(per) / len(per), "kernel": " + ".join(kinds)}
     finally:
         shutil.rmtree(outdir, ignore_errors=True)
     return res
