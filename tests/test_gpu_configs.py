@@ -169,3 +169,63 @@ def test_config4_ensemble_full_size(gpu):
     coef = orc.cheb_coeffs(ec.cheb_n, ec.cheb_cut)
     want = orc.overlap_evo_cheb(L, ucc, orc.norm_bound(L), ec.cheb_cut, ec.cheb_n, coef, psi[0])
     assert rel(ref[k][0], want) < 1e-12
+
+
+def test_config1_su2_overlap_16_all_50_steps(gpu):
+    # the workload's full run: 50 leapfrog steps of 64 modes, mixed against FP64 for every mode
+    # (the links move with E fixed and do not depend on the modes, so the two runs see the same
+    # operators and differ only by the mixed form's rounding, accumulated over 100 overlap applies)
+    c = cf.CONFIGS["su2_overlap_16"]
+    print(f"[case] {c.name} {c.steps} steps", flush=True)
+    runs = []
+    for prec in (ovl.PREC_FP64, ovl.PREC_MIXED):
+        ctx, links, e, psi = setup(c)
+        ctx.fermion_update(step_opts(c, prec), c.steps)
+        runs.append((ctx, ctx.get_modes(ovl.PSI), ctx.get_modes(ovl.PSIMID)))
+    (c64, p64, m64), (cmx, pmx, mmx) = runs
+    err = max(max(np.linalg.norm(a - b) / np.linalg.norm(b) for a, b in zip(x, y)) for x, y in ((pmx, p64), (mmx, m64)))
+    print(f"  mixed vs fp64 after {c.steps} steps: {err:.2e}", flush=True)
+    assert err < 1e-10
+    assert np.array_equal(cmx.get_links(), c64.get_links())
+    assert np.abs(stagger_invariant(cmx) - 1).max() < 1e-10
+    assert np.abs(stagger_invariant(c64) - 1).max() < 1e-12
+
+
+def test_config2_su3_wilson_32_all_200_steps(gpu):
+    # the workload's full run: 200 steps of 512 modes on 32^3 with 4 flux quanta: the leapfrog
+    # invariant of every mode, the links' unitarity, and the first mode against a 3-step-shorter
+    # run continued by 3 oracle steps (the oracle continues the GPU's state exactly)
+    c = cf.CONFIGS["su3_wilson_32"]
+    print(f"[case] {c.name} {c.steps} steps", flush=True)
+    ctx, links, e, psi = setup(c)
+    ctx.fermion_update(step_opts(c, ovl.PREC_FP64), c.steps - 3)
+    lk, p, pm = ctx.get_links(), ctx.get_modes(ovl.PSI, 0, 1), ctx.get_modes(ovl.PSIMID, 0, 1)
+    ctx.fermion_update(step_opts(c, ovl.PREC_FP64), 3)
+    L = orc.lat(c.nc, c.n, c.r_wilson, c.m_fermion, c.m_dw)
+    o = orc.StepOpts(orc.WILSON, c.dt, 1, c.nb, c.q, 0, 0.0, 0.0, 0.0, 8)
+    lk3, p3, pm3, _ = orc.leapfrog(L, lk, e, p, pm, o, 3)
+    assert rel(ctx.get_links(), lk3) < 1e-12
+    assert rel(ctx.get_modes(ovl.PSI, 0, 1), p3) < 1e-12 and rel(ctx.get_modes(ovl.PSIMID, 0, 1), pm3) < 1e-12
+    inv = stagger_invariant(ctx)
+    print(f"  leapfrog invariant deviation after {c.steps} steps: {np.abs(inv - 1).max():.2e}", flush=True)
+    assert np.abs(inv - 1).max() < 1e-11
+    assert ctx.unitarity_violation()[0] < 1e-12
+
+
+def test_config3_su3_overlap_24_ten_steps(gpu):
+    # ten leapfrog steps of the bench workload (256 modes, degree 384, 8 flux quanta): mixed
+    # against FP64 for every mode, and the leapfrog invariant
+    c = cf.CONFIGS["su3_overlap_24"]
+    steps = 10
+    print(f"[case] {c.name} {steps} steps", flush=True)
+    runs = []
+    for prec in (ovl.PREC_FP64, ovl.PREC_MIXED):
+        ctx, links, e, psi = setup(c)
+        ctx.fermion_update(step_opts(c, prec), steps)
+        runs.append((ctx.get_modes(ovl.PSI), ctx.get_modes(ovl.PSIMID), stagger_invariant(ctx)))
+        ctx.close()
+    (p64, m64, i64), (pmx, mmx, imx) = runs
+    err = max(max(np.linalg.norm(a - b) / np.linalg.norm(b) for a, b in zip(x, y)) for x, y in ((pmx, p64), (mmx, m64)))
+    print(f"  mixed vs fp64 after {steps} steps over 256 modes: {err:.2e}", flush=True)
+    assert err < 1e-10
+    assert np.abs(i64 - 1).max() < 1e-12 and np.abs(imx - 1).max() < 1e-10
