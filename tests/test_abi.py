@@ -138,3 +138,15 @@ def test_sign_plan_host_only():
     assert ovl.sign_plan(o) == (1, 65, 65)
     with pytest.raises(ovl.OvlError):
         ovl.sign_plan(ovl.sign_opts(cut=1.5))
+
+
+def test_bench_refuses_more_than_one_gpu():
+    # the bench measures one GPU; asked for more it exits with a message before touching a device
+    import json
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode != 0
+    assert "one GPU" in json.loads(r.stdout.strip().splitlines()[-1])["error"]
