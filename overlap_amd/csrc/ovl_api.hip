@@ -2383,13 +2383,13 @@ int ovl_qed_eigenbasis(ovl_ctx *c, const ovl_sign_opts *o, int nb, double q, dou
     // of one field), the DFT matrix, the eigensolver's outputs
     const int nprob = nc > 1 ? 2 : 1;
     std::vector<double> Wz((size_t)n2 * Dz), W4(4, 0.0);
-    void *dA = nullptr, *dTW = nullptr, *dW = nullptr, *dE = nullptr, *dInfo = nullptr, *dS = nullptr;
+    void *dA = nullptr, *dTW = nullptr, *dW = nullptr, *dE = nullptr, *dInfo = nullptr, *dS = nullptr, *dM = nullptr;
     rocblas_handle handle = nullptr;
     auto release = [&]() {
         hipStreamSynchronize(c->st);
         if (handle) rocblas_destroy_handle(handle);
         handle = nullptr;
-        dfree(dA); dfree(dTW); dfree(dW); dfree(dE); dfree(dInfo); dfree(dS);
+        dfree(dA); dfree(dTW); dfree(dW); dfree(dE); dfree(dInfo); dfree(dS); dfree(dM);
     };
     struct Release {
         decltype(release) &f;
@@ -2550,46 +2550,42 @@ int ovl_qed_eigenbasis(ovl_ctx *c, const ovl_sign_opts *o, int nb, double q, dou
         for (long b = 0; b < D; ++b) eigenvalues[b] = ent[b].w;
 
     // Psi(s) = sqrt(V) v_b, PsiMid(s) = Psi(s) exp(-i omega dt / 2), b from (particle, level) of the
-    // global mode index (FermionField.cpp:63-74, SetOverlapQEDEigenfunctions.cpp:14-44)
-    std::vector<cplx> psi((size_t)nm * per_mode, cplx(0, 0)), mid((size_t)nm * per_mode, cplx(0, 0));
+    // global mode index (FermionField.cpp:63-74, SetOverlapQEDEigenfunctions.cpp:14-44); the fields
+    // are assembled on the device from the eigenvector blocks, one descriptor per mode
+    std::vector<ovl::EigMode> em((size_t)nm);
     const double norm = std::sqrt((double)V);
-    std::vector<cplx> chi((size_t)Dz);
     for (int sm = 0; sm < nm; ++sm) {
         const long g = (long)first_mode + sm, particle = g % 2, level = (g / 2) % (2L * V * nc);
         const long b = particle == 0 ? D / 2 - 1 - level : D / 2 + level;
         const Ent &e = ent[b];
         const cplx rot = std::exp(cplx(0, -0.5 * e.w * dt));
-        cplx *ps = psi.data() + (size_t)sm * per_mode, *pm = mid.data() + (size_t)sm * per_mode;
+        ovl::EigMode &d = em[(size_t)sm];
+        d = ovl::EigMode{};
+        d.rot_r = rot.real();
+        d.rot_i = rot.imag();
         if (e.blk >= 0) {
-            // exp(i k z) chi(x, y, s) / sqrt(n2) on colour 0; chi is column idx of block blk on the device
-            HIPCHK(c, hipMemcpyAsync(chi.data(), (const cplx *)dA + (size_t)e.blk * Dz * Dz + (size_t)e.idx * Dz,
-                                     sizeof(cplx) * Dz, hipMemcpyDeviceToHost, c->st));
-            HIPCHK(c, hipStreamSynchronize(c->st));
-            const double f = norm / std::sqrt((double)n2);
-            for (int z = 0; z < n2; ++z) {
-                const cplx ph = f * std::conj(tw[(size_t)e.blk * n2 + z]);
-                for (size_t xy = 0; xy < plane; ++xy)
-                    for (int sp = 0; sp < 4; ++sp) {
-                        const cplx v = ph * chi[xy * 4 + sp];
-                        const size_t k = aos((long)z * plane + xy, 0, sp);
-                        ps[k] = v;
-                        pm[k] = v * rot;
-                    }
-            }
+            // exp(i k z) chi(x, y, s) sqrt(V / n2) on colour 0, chi column idx of block blk
+            d.blk = e.blk;
+            d.idx = (int)e.idx;
         } else {
             // spin eigenvector at one (site, colour >= 1)
-            const long copy = e.idx / 4, site = copy % V;
-            const int col = 1 + (int)(copy / V), l = (int)(e.idx % 4);
+            const long copy = e.idx / 4;
+            const int l = (int)(e.idx % 4);
+            d.blk = -1;
+            d.idx = (int)(copy % V);
+            d.col = 1 + (int)(copy / V);
             for (int sp = 0; sp < 4; ++sp) {
                 const cplx v = norm * spin_block[l * 4 + sp];
-                const size_t k = aos(site, col, sp);
-                ps[k] = v;
-                pm[k] = v * rot;
+                d.v[2 * sp] = v.real();
+                d.v[2 * sp + 1] = v.imag();
             }
         }
     }
-    RETCHK(modes_io(c, 0, OVL_PSI, reinterpret_cast<double *>(psi.data()), 0, nm, true));
-    RETCHK(modes_io(c, 0, OVL_PSIMID, reinterpret_cast<double *>(mid.data()), 0, nm, true));
+    RETCHK(dalloc(c, &dM, sizeof(ovl::EigMode) * em.size()));
+    HIPCHK(c, hipMemcpyAsync(dM, em.data(), sizeof(ovl::EigMode) * em.size(), hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, ovl::launch_eig_modes(g, (const ovl::EigMode *)dM, nm, dA, dTW, norm / std::sqrt((double)n2),
+                                    c->field[OVL_PSI], c->field[OVL_PSIMID], c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
     return OVL_OK;
 }
 

@@ -162,6 +162,18 @@ hipError_t launch_eig_gather(const Geom &g, const void *out, void *A, long k0, i
 hipError_t launch_eig_leak(const Geom &g, const void *field, int nvalid, double *partial, hipStream_t st);
 hipError_t launch_eig_hermitize(void *A, long dz, int n2, double *partial, hipStream_t st);
 hipError_t launch_nonfinite(const void *x, long n, double *partial, hipStream_t st);
+// one requested mode of the eigenbasis: blk >= 0: f exp(i k_blk z) chi(x, y, spin) on colour 0, chi
+// the eigenvector in column idx of z-momentum block blk; blk = -1: the four spin components v at
+// site idx, colour col >= 1.  The second field holds the mode times rot.
+struct EigMode {
+    int blk, idx, col, pad;
+    double rot_r, rot_i;
+    double v[8];
+};
+// Psi and PsiMid of nm modes from the eigenvector blocks A (n2 column-major 4 n0 n1 squared blocks)
+// and tw[j n2 + z] = exp(-2 pi i j z / n2); writes every entry of both FP64 fields (modes >= nm: 0)
+hipError_t launch_eig_modes(const Geom &g, const EigMode *modes, int nm, const void *A, const void *tw, double f,
+                            void *psi, void *mid, hipStream_t st);
 size_t eig_partial_len();
 int eig_partial_blocks();
 

@@ -1727,6 +1727,49 @@ __global__ __launch_bounds__(TPB) void k_eig_hermitize(double2 *A, long dz, int 
     }
 }
 
+// Psi and PsiMid of the requested modes, every entry of both fields (modes >= nm are zero): mode
+// e.blk >= 0 is f conj(tw[e.blk n2 + z]) A[e.blk dz^2 + e.idx dz + 4 xy + spin] on colour 0 (site =
+// z plane + xy), mode e.blk = -1 is e.v at site e.idx, colour e.col; PsiMid = Psi rot
+template <int NC, int MB>
+__global__ __launch_bounds__(TPB) void k_eig_modes(const EigMode *__restrict__ modes, int nm,
+                                                   const double2 *__restrict__ A, const double2 *__restrict__ tw,
+                                                   double f, int V, long plane, int n2, int nblk,
+                                                   double2 *__restrict__ psi, double2 *__restrict__ mid)
+{
+    constexpr int NCOMP = 4 * NC;
+    const long dz = 4 * plane;
+    const long total = (long)nblk * V * NCOMP * MB;
+    for (long i = (long)blockIdx.x * TPB + threadIdx.x; i < total; i += (long)gridDim.x * TPB) {
+        const int m = (int)(i % MB);
+        const long r = i / MB;
+        const int comp = (int)(r % NCOMP);
+        const long r2 = r / NCOMP;
+        const int site = (int)(r2 % V), blk = (int)(r2 / V);
+        const int mode = blk * MB + m, sp = comp / NC, colour = comp - sp * NC;
+        double vr = 0.0, vi = 0.0, rr = 0.0, ri = 0.0;
+        if (mode < nm) {
+            const EigMode &e = modes[mode];
+            rr = e.rot_r;
+            ri = e.rot_i;
+            if (e.blk >= 0) {
+                if (colour == 0) {
+                    const long z = site / plane, xy = site - z * plane;
+                    const double2 t = tw[(size_t)e.blk * n2 + z];
+                    const double ph_r = f * t.x, ph_i = -(f * t.y);
+                    const double2 a = A[(size_t)e.blk * dz * dz + (size_t)e.idx * dz + xy * 4 + sp];
+                    vr = ph_r * a.x - ph_i * a.y;
+                    vi = ph_r * a.y + ph_i * a.x;
+                }
+            } else if (site == e.idx && colour == e.col) {
+                vr = e.v[2 * sp];
+                vi = e.v[2 * sp + 1];
+            }
+        }
+        psi[i] = make_double2(vr, vi);
+        mid[i] = make_double2(vr * rr - vi * ri, vr * ri + vi * rr);
+    }
+}
+
 // 1 in partial[blockIdx.x] when one of the n doubles is not finite
 __global__ __launch_bounds__(TPB) void k_nonfinite(const double *__restrict__ x, long n, double *partial)
 {
@@ -2002,6 +2045,17 @@ hipError_t launch_eig_hermitize(void *A, long dz, int n2, double *partial, hipSt
 hipError_t launch_nonfinite(const void *x, long n, double *partial, hipStream_t st)
 {
     k_nonfinite<<<RED_BLOCKS, TPB, 0, st>>>((const double *)x, n, partial);
+    return hipGetLastError();
+}
+
+hipError_t launch_eig_modes(const Geom &g, const EigMode *modes, int nm, const void *A, const void *tw, double f,
+                            void *psi, void *mid, hipStream_t st)
+{
+    if (nm < 1 || nm > g.nblk * g.mb) return hipErrorInvalidValue;
+    const long total = (long)g.nblk * g.V * 4 * g.nc * g.mb;
+    const unsigned grid = (unsigned)std::min<long>((total + TPB - 1) / TPB, 65536);
+    OVL_EIG_DISPATCH(k_eig_modes, grid, modes, nm, (const double2 *)A, (const double2 *)tw, f, g.V,
+                     (long)g.n0 * g.n1, g.n2, g.nblk, (double2 *)psi, (double2 *)mid);
     return hipGetLastError();
 }
 #undef OVL_EIG_DISPATCH

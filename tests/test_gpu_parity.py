@@ -519,21 +519,24 @@ def test_observed_update_currents(gpu):
     assert np.abs(jv - jv0).max() < 1e-13 and np.abs(ja - ja0).max() < 1e-13
 
 
-@pytest.mark.parametrize("nc,n,nb,nm,mb", [(2, (4, 4, 3), 2, 16, 16), (3, (3, 3, 4), 1, 16, 16),
-                                           (2, (4, 4, 3), 2, 48, 16), (3, (3, 3, 4), 1, 64, 32)])
-def test_qed_eigenbasis_vs_dense_oracle(gpu, nc, n, nb, nm, mb):
+@pytest.mark.parametrize("nc,n,nb,nm,mb,first", [(2, (4, 4, 3), 2, 16, 16, 2), (3, (3, 3, 4), 1, 16, 16, 2),
+                                                 (2, (4, 4, 3), 2, 48, 16, 2), (3, (3, 3, 4), 1, 64, 32, 2),
+                                                 (3, (3, 3, 4), 1, 16, 16, 352)])
+def test_qed_eigenbasis_vs_dense_oracle(gpu, nc, n, nb, nm, mb, first):
     # the overlap Hamiltonian of the U(1) background alone (colour 0 carries the phase), assembled
     # on the CPU from the oracle's overlap apply and diagonalised by numpy, against the GPU's.  The
     # cases cover one and several mode blocks per batch of unit sources (the z-momentum transform
     # is one batched product over the blocks), several batches, and the colour-1 spin sources in
-    # a batch of their own, after the last colour-0 sources in the same batch, and in a later block
-    print(f"[case] qed eigenbasis nc={nc} n={n} nb={nb} modes={nm} mb={mb}", flush=True)
+    # a batch of their own, after the last colour-0 sources in the same batch, and in a later block.
+    # Modes from 352 on (levels 176-183 of the 432 levels) lie in the +-2.4 eigenspaces, where the
+    # colours without hopping have their eigenvectors: the single-site spin modes.
+    print(f"[case] qed eigenbasis nc={nc} n={n} nb={nb} modes={nm} mb={mb} first={first}", flush=True)
     V = int(np.prod(n))
     D = 4 * nc * V
     deg, cut, M, dt = 64, 0.1, 1.2, 0.02
     ctx = ovl.Context(nc, n, nm, m_dw=M, mode_block=mb)
     ev, herm = ctx.qed_eigenbasis(ovl.sign_opts(degree=deg, cut=cut, precision=ovl.PREC_FP64), nb, 1.0, dt,
-                                  first_mode=2)
+                                  first_mode=first)
     L = orc.lat(nc, n, 1.0, 0.0, M)
     ph = orc.u1_phases(n, nb, 1.0)  # complex [mu][site]
     ucc = np.zeros((3, V, nc, nc), complex)
@@ -550,8 +553,10 @@ def test_qed_eigenbasis_vs_dense_oracle(gpu, nc, n, nb, nm, mb):
     assert np.abs(ev - w[::-1]).max() < 1e-11
     # modes: eigenvectors of H with norm^2 V; PsiMid = Psi exp(-i omega dt / 2)
     psi, mid = ctx.get_modes(ovl.PSI), ctx.get_modes(ovl.PSIMID)
+    on_colour0 = (np.arange(D) // 4) % nc == 0  # AoS entry (site, colour, spin)
+    single_site = 0
     for s in range(nm):
-        g = 2 + s
+        g = first + s
         particle, level = g % 2, g // 2
         b = D // 2 - 1 - level if particle == 0 else D // 2 + level
         omega = ev[b]
@@ -559,6 +564,10 @@ def test_qed_eigenbasis_vs_dense_oracle(gpu, nc, n, nb, nm, mb):
         assert abs(np.vdot(v, v).real - V) < 1e-9 * V
         assert np.linalg.norm(H @ v - omega * v) < 1e-9 * np.sqrt(V)
         assert np.allclose(mid[s], v * np.exp(-0.5j * omega * dt), rtol=0, atol=1e-14)
+        if not np.any(v[on_colour0]):
+            assert np.count_nonzero(v) <= 4  # one site, one colour >= 1
+            single_site += 1
+    assert single_site == (nm if first == 352 else 0)
     # the SU(Nc) stencil is back afterwards
     ctx2, L2, ucc2, links, e2, psi2 = make(nc, n, nm, 5, M=M, mb=16)
     ctx2.qed_eigenbasis(ovl.sign_opts(degree=16, cut=0.2, precision=ovl.PREC_FP64), nb, 1.0, dt)
