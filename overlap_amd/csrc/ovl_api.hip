@@ -77,6 +77,12 @@ struct ovl_ctx {
     int nt_out[NUM_PREC] = {0, 0, 0, 0};        // streaming output stores
     int wpe[NUM_PREC] = {1, 1, 1, 1};           // occupancy variant
     double step_xmax = 0.0;  // bound of the mode norms during ovl_fermion_update (FP16 scales)
+    // the last sign polynomial (a function of degree and cut only): its Chebyshev coefficients and
+    // the Clenshaw bounds of the ladder (t0 = -1) and of the FP16 scales (t0 = beta), empty until
+    // first needed; every sign apply plans, and the coefficients alone cost milliseconds of host time
+    int memo_n = -1;
+    double memo_cut = 0.0;
+    std::vector<double> memo_coef, memo_B1, memo_Bb;
     double *dout = nullptr;  // 16 doubles
     double *gam = nullptr;   // current coupling matrices
     double *jv = nullptr, *ja = nullptr;
@@ -552,7 +558,9 @@ struct SignPlan {
 
 // the precision ladder: k < head48 FP64, head48 <= k < head 48-bit storage, head <= k < head16
 // FP32, k >= head16 FP16 storage
-void plan_tiers(const ovl_sign_opts *o, const std::vector<double> &coef, int *head48, int *head, int *head16)
+// B1: clenshaw_sup(coef, -1) when the caller holds it (null: computed here for the mixed ladder)
+void plan_tiers(const ovl_sign_opts *o, const std::vector<double> &coef, int *head48, int *head, int *head16,
+                const std::vector<double> *B1 = nullptr)
 {
     const int n = (int)coef.size() - 1;
     const double u32 = 5.96e-8, u16 = 4.89e-4, u48 = 7.28e-12;  // 2^-24 (+), 2^-11 (+), 2^-37
@@ -565,7 +573,7 @@ void plan_tiers(const ovl_sign_opts *o, const std::vector<double> &coef, int *he
         *head = 0;
         *head48 = 0;
     } else {
-        const std::vector<double> B = clenshaw_sup(coef, -1.0);
+        const std::vector<double> B = B1 ? *B1 : clenshaw_sup(coef, -1.0);
         const bool fp16 = o->fp16_tail != 0, fp48 = o->fp48 != 0;
         // error budget: 1/4 to each of the FP16 and 48-bit tiers that are on, the rest to FP32
         const double s16 = fp16 ? 0.25 : 0.0, s48 = fp48 ? 0.25 : 0.0, s32 = 1.0 - s16 - s48;
@@ -600,15 +608,24 @@ int make_plan(ovl_ctx *c, const ovl_sign_opts *o, SignPlan &p, double xmax = 0.0
     if (o->degree < 1 || o->degree > 100000) return fail(c, OVL_E_ARG, "Chebyshev degree must be in [1, 100000]");
     if (!(o->cut > 0.0 && o->cut < 1.0)) return fail(c, OVL_E_SPECTRUM, "cut must be in (0, 1)");
     p.n = o->degree;
-    p.coef = cheb_coeffs(p.n, o->cut);
+    if (c->memo_n != o->degree || c->memo_cut != o->cut) {
+        c->memo_coef = cheb_coeffs(o->degree, o->cut);
+        c->memo_B1.clear();
+        c->memo_Bb.clear();
+        c->memo_n = o->degree;
+        c->memo_cut = o->cut;
+    }
+    p.coef = c->memo_coef;
     p.lam = lambda_bound(c);
     const double d2 = o->cut * o->cut;
     p.alpha = 2.0 / (p.lam * p.lam * (1.0 - d2));
     p.beta = -(1.0 + d2) / (1.0 - d2);
-    plan_tiers(o, p.coef, &p.head48, &p.head, &p.head16);
+    if (o->precision == OVL_PREC_MIXED && c->memo_B1.empty()) c->memo_B1 = clenshaw_sup(p.coef, -1.0);
+    plan_tiers(o, p.coef, &p.head48, &p.head, &p.head16, o->precision == OVL_PREC_MIXED ? &c->memo_B1 : nullptr);
     if (p.head16 <= p.n) {
         if (!(xmax > 0.0) || !std::isfinite(xmax)) return fail(c, OVL_E_ARG, "FP16 tier needs a bound of the input norms");
-        const std::vector<double> Ball = clenshaw_sup(p.coef, p.beta);
+        if (c->memo_Bb.empty()) c->memo_Bb = clenshaw_sup(p.coef, p.beta);  // beta depends on cut only
+        const std::vector<double> &Ball = c->memo_Bb;
         p.sk.assign(p.n + 2, 1.0);
         p.sw.assign(p.n + 2, 1.0);
         for (int k = 0; k <= p.n; ++k) {

@@ -442,6 +442,28 @@ def test_qed_eigenbasis_beyond_dense_size(gpu):
         assert np.linalg.norm(hv - ev[b] * psi[s]) < 1e-9 * np.sqrt(V)
 
 
+def test_sign_polynomial_follows_degree_and_cut(gpu):
+    # the context keeps the last sign polynomial (coefficients and Clenshaw bounds): a change of
+    # degree or cut between applies on one context must give the bits of a fresh context
+    print("[case] sign polynomial per degree and cut", flush=True)
+    n = (4, 4, 6)
+    opts = [ovl.sign_opts(degree=32, cut=0.1, precision=ovl.PREC_MIXED, tol=1e-10),
+            ovl.sign_opts(degree=48, cut=0.1, precision=ovl.PREC_MIXED, tol=1e-10),
+            ovl.sign_opts(degree=48, cut=0.05, precision=ovl.PREC_MIXED, tol=1e-10),
+            ovl.sign_opts(degree=48, cut=0.05, precision=ovl.PREC_FP64)]
+    ctx, *_ = make(3, n, 16, 611, M=1.2, nb=2, mb=16)
+    seq = []
+    for o in opts + opts[:1]:
+        ctx.apply_sign(ovl.PSI, ovl.PSIMID, o)
+        seq.append(ctx.get_modes(ovl.PSIMID))
+    for o, got in zip(opts, seq):
+        fresh, *_ = make(3, n, 16, 611, M=1.2, nb=2, mb=16)
+        fresh.apply_sign(ovl.PSI, ovl.PSIMID, o)
+        assert np.array_equal(fresh.get_modes(ovl.PSIMID), got)
+    assert np.array_equal(seq[-1], seq[0])
+    assert rel(seq[1], seq[2]) > 1e-6  # the cut changes the polynomial
+
+
 @pytest.mark.parametrize("nc,mb", [(3, 64), (2, 16)])
 def test_occupancy_variants_are_bit_identical(gpu, nc, mb):
     # occupancy variants 1-5 (5: one workgroup per CU through an LDS reservation) run the same
