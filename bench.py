@@ -469,16 +469,19 @@ def main():
     timing = "eager stream, HIP events around every stencil launch"
     if not args.quick:
         sog = step_opts(c, prec_mixed, True, args.tol, args.chunk)
+        ctx.fermion_update(sog, 1)  # untimed warmup step: captures the step graph, which the context keeps
         t0 = time.perf_counter()
         ctx.fermion_update(sog, args.steps)
         wg = time.perf_counter() - t0
         graph_ms = 1e3 * wg / args.steps
-        log(f"mixed (hipGraph): {graph_ms:.1f} ms/step")
+        captures = int(ctx.timing().graph_captures)
+        log(f"mixed (hipGraph): {graph_ms:.1f} ms/step, {captures} captures in the timed call")
         if wg < wall:
             ms_per_step = graph_ms
             value = c.volume * c.nmodes * args.steps / wg
-            timing = ("hipGraph replay of one captured step (capture and instantiation timed); the roofline "
-                      "is from the eager run with HIP events around every stencil launch")
+            timing = ("hipGraph replay of the step graph captured in an untimed warmup step (the timed call "
+                      f"bounds the mode norms, checks the step's key and replays; captures in it: {captures}); "
+                      "the roofline is from the eager run with HIP events around every stencil launch")
             if alg_step_bytes:
                 roofline["step_alg_frac"] = round(alg_step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
 

@@ -88,7 +88,10 @@ typedef struct {
     double dt;           /* full leapfrog step */
     int gauge;           /* OVL_GAUGE_FROZEN / _LINKS (U <- exp(-i dt/2 E) U, E fixed) / _FULL */
     ovl_sign_opts sign;
-    int use_graph;       /* capture each half step into a hipGraph and replay it */
+    int use_graph;       /* capture one full step into a hipGraph and replay it; the context keeps
+                            the graph, and a later call replays it without capturing while these
+                            options, the FP16 scales of the sign, the context's buffers and the
+                            stencil tuning are unchanged */
 } ovl_step_opts;
 
 /* ---- context ---- */
@@ -266,6 +269,8 @@ typedef struct {
     double var_ms[5];
     long long var_launches[5];
     double var_bytes[5];
+    int graph_captures;       /* hipGraph captures the call made (a graph-mode ovl_fermion_update
+                                 whose step is unchanged since the previous one replays that graph: 0) */
 } ovl_timing;
 int ovl_get_timing(ovl_ctx *ctx, ovl_timing *t);
 int ovl_set_profiling(ovl_ctx *ctx, int on);

@@ -36,6 +36,19 @@ struct DevBuf {
 // stencil variants timed separately: [0] FP64, [1] FP32, [2] FP32 in / FP64 out, [3] FP16, [4] 48-bit
 constexpr int NVAR = 5;
 
+// The host-side inputs of a captured leapfrog step: the arguments of its kernels are built from
+// the options, the sign plan (ladder and FP16 scales), the context's device pointers and the
+// stencil tuning.  A kept graph is replayed only while all of them are unchanged; the contents of
+// device memory (links, fields, phases, site orders) are read by every replay afresh.
+struct StepKey {
+    ovl_step_opts o{};
+    int head48 = 0, head = 0, head16 = 0;
+    double sx = 0.0;
+    std::vector<double> sk, sw;
+    std::vector<const void *> ptrs;
+    std::vector<int> ints;
+};
+
 struct ovl_ctx {
     ovl_params prm{};
     Geom g{};
@@ -111,6 +124,15 @@ struct ovl_ctx {
     ovl_timing last{};
     hipEvent_t t0 = nullptr, t1 = nullptr;
     bool capturing = false;
+    int captures = 0;  // hipGraph captures made by the current call
+
+    // the last captured leapfrog step, kept for later calls with the same key, and its per-step
+    // counter increments (stencil launches and bytes, per variant, tier steps)
+    hipGraphExec_t step_exec = nullptr;
+    StepKey step_key;
+    long long step_dl = 0, step_dvl[NVAR] = {};
+    double step_db = 0, step_dvb[NVAR] = {};
+    int step_dp[4] = {};
 
     // lattices held: one for an ordinary context, or the systems of an ensemble, stored one after
     // another in every field (site-blocks q0..), link and E-field array (link index l0..)
@@ -414,6 +436,7 @@ void begin_timing(ovl_ctx *c)
     c->hop_bytes = 0;
     for (int v = 0; v < NVAR; ++v) { c->var_launches[v] = 0; c->var_bytes[v] = 0; }
     c->fp64_steps = c->fp32_steps = c->fp16_steps = c->fp48_steps = 0;
+    c->captures = 0;
     if (!c->capturing) hipEventRecord(c->t0, c->st);
 }
 
@@ -443,6 +466,7 @@ int end_timing(ovl_ctx *c)
     c->last.fp32_steps = c->fp32_steps;
     c->last.fp16_steps = c->fp16_steps;
     c->last.fp48_steps = c->fp48_steps;
+    c->last.graph_captures = c->captures;
     return OVL_OK;
 }
 
@@ -1277,6 +1301,7 @@ int ovl_destroy(ovl_ctx *c)
                   c->ms_ps, c->ms_r, c->ms_q, c->ms_w, c->ms_chi, c->ms_coef, c->ms_active, c->rz[0], c->rz[1],
                   c->rz[2], c->rz[3], c->rz[4], c->rz[5], c->d_sys, c->d_work, c->order[0], c->order[1],
                   c->order[2], c->order[3], c->x16, c->w16, c->a16, c->b16, c->x48, c->w48, c->a48, c->b48};
+    if (c->step_exec) hipGraphExecDestroy(c->step_exec);
     for (void *p : ps)
         if (p) hipFree(p);
     for (auto e : c->evpool) hipEventDestroy(e);
@@ -1881,6 +1906,57 @@ static int fermion_half(ovl_ctx *c, const ovl_step_opts *o, int src, int tgt, vo
     return sign_driver(c, &o->sign, c->field[src], c->field[tgt], FIN_OVERLAP_AXPY, o->dt, c->step_xmax, hpsi);
 }
 
+// every field of the step options; a new option must be added here (the sizes below catch it)
+static_assert(sizeof(ovl_sign_opts) == 80 && sizeof(ovl_step_opts) == 112, "compare every option in same_step_opts");
+static bool same_step_opts(const ovl_step_opts &a, const ovl_step_opts &b)
+{
+    const ovl_sign_opts &x = a.sign, &y = b.sign;
+    return a.kind == b.kind && a.dt == b.dt && a.gauge == b.gauge && a.use_graph == b.use_graph &&
+           x.method == y.method && x.precision == y.precision && x.degree == y.degree && x.cut == y.cut &&
+           x.tol == y.tol && x.head == y.head && x.lmin == y.lmin && x.lmax == y.lmax &&
+           x.mode_chunk == y.mode_chunk && x.fp16_tail == y.fp16_tail && x.head16 == y.head16 &&
+           x.fp48 == y.fp48 && x.head48 == y.head48;
+}
+
+static StepKey make_step_key(const ovl_ctx *c, const ovl_step_opts *o, const SignPlan *pl)
+{
+    StepKey k;
+    k.o = *o;
+    if (pl) {
+        k.head48 = pl->head48;
+        k.head = pl->head;
+        k.head16 = pl->head16;
+        k.sx = pl->sx;
+        k.sk = pl->sk;
+        k.sw = pl->sw;
+    }
+    k.ptrs = {c->links, c->efield, c->ul64, c->ul32, c->phase, c->field[0], c->field[1], c->field[2],
+              c->w64, c->a64, c->b64, c->x32, c->w32, c->a32, c->b32, c->x16, c->w16, c->a16, c->b16,
+              c->x48, c->w48, c->a48, c->b48, c->partial, c->d_sys, c->d_work, c->order[0], c->order[1],
+              c->order[2], c->order[3]};
+    k.ints = {c->work_blocks, c->ens_grid};
+    for (int nb : c->sys_nb) k.ints.push_back(nb != 0);  // systems whose links take the phases
+    for (int p = 0; p < NUM_PREC; ++p)
+        for (int v : {c->trav[p], c->slabs[p], c->nt_out[p], c->wpe[p]}) k.ints.push_back(v);
+    return k;
+}
+
+static bool same_step_key(const StepKey &a, const StepKey &b)
+{
+    return same_step_opts(a.o, b.o) && a.head48 == b.head48 && a.head == b.head && a.head16 == b.head16 &&
+           a.sx == b.sx && a.sk == b.sk && a.sw == b.sw && a.ptrs == b.ptrs && a.ints == b.ints;
+}
+
+static void drop_step_graph(ovl_ctx *c)
+{
+    if (c->step_exec) {
+        hipStreamSynchronize(c->st);
+        hipGraphExecDestroy(c->step_exec);
+    }
+    c->step_exec = nullptr;
+    c->step_key = StepKey{};
+}
+
 // observe: the second half step also leaves -i h_ov[U(t + dt/2)] Psi(t + dt) in the scratch field
 static int one_step(ovl_ctx *c, const ovl_step_opts *o, bool observe = false)
 {
@@ -1907,14 +1983,20 @@ static int update_impl(ovl_ctx *c, const ovl_step_opts *o, int nsteps, bool obse
     if (o->gauge < OVL_GAUGE_FROZEN || o->gauge > OVL_GAUGE_FULL) return fail(c, OVL_E_ARG, "unknown gauge mode");
     if (o->kind == OVL_OVERLAP) RETCHK(check_sign_opts(c, &o->sign));
     c->step_xmax = 0.0;
-    if (o->kind == OVL_OVERLAP && o->sign.method == OVL_SIGN_CHEBYSHEV) {
+    SignPlan pl;
+    const bool cheb = o->kind == OVL_OVERLAP && o->sign.method == OVL_SIGN_CHEBYSHEV;
+    if (cheb) {
         // bound of the mode norms for the FP16 scales: the staggered leapfrog conserves the norms
-        // for dt |omega| < 2, so twice the larger of the two fields at the start is a safe bound
+        // for dt |omega| < 2, so twice the larger of the two fields at the start is a safe bound.
+        // Rounded up to a power of two: the norms of the two fields oscillate by O(dt omega) from
+        // step to step, and the FP16 scales (and with them a kept step graph) should not follow.
         double b0, b1;
         RETCHK(input_bound(c, &o->sign, c->field[OVL_PSI], &b0));
         RETCHK(input_bound(c, &o->sign, c->field[OVL_PSIMID], &b1));
-        c->step_xmax = 2.0 * std::max(b0, b1);
-        SignPlan pl;
+        const double bnd = 2.0 * std::max(b0, b1);
+        int ex = 0;
+        const double mant = std::frexp(bnd, &ex);  // bnd = mant 2^ex, mant in [1/2, 1)
+        c->step_xmax = (bnd > 0.0 && std::isfinite(bnd)) ? std::ldexp(1.0, mant == 0.5 ? ex - 1 : ex) : bnd;
         RETCHK(make_plan(c, &o->sign, pl, c->step_xmax));
     }
     RETCHK(ensure_stencil(c));
@@ -1930,59 +2012,82 @@ static int update_impl(ovl_ctx *c, const ovl_step_opts *o, int nsteps, bool obse
         for (int s = 0; s < nsteps; ++s) RETCHK(one_step(c, o));
         return end_timing(c);
     }
-    // capture one full step (both half steps and both gauge half steps) and replay it
-    // every buffer the captured step touches is allocated before the capture
+    // one full step (both half steps and both gauge half steps) as a graph, replayed nsteps times;
+    // every buffer the captured step touches is allocated before the capture.  The instantiated
+    // graph stays in the context and later calls replay it while the step's key is unchanged.
     const bool mixed = o->kind == OVL_OVERLAP && o->sign.precision == OVL_PREC_MIXED;
     RETCHK(ensure_work(c, (o->sign.mode_chunk > 0) ? std::min(o->sign.mode_chunk, c->g.nblk) : c->g.nblk,
                        o->kind == OVL_OVERLAP && o->sign.precision != OVL_PREC_FP64, mixed && o->sign.fp16_tail,
                        mixed && o->sign.fp48));
-    hipGraph_t graph_h = nullptr;
-    hipGraphExec_t exec = nullptr;
+    const StepKey key = make_step_key(c, o, cheb ? &pl : nullptr);
     HIPCHK(c, hipStreamSynchronize(c->st));
     HIPCHK(c, hipEventRecord(c->t0, c->st));
-    c->capturing = true;
-    hipError_t e = hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal);
-    if (e != hipSuccess) {
+    if (!(c->step_exec && same_step_key(key, c->step_key))) {
+        drop_step_graph(c);
+        hipGraph_t graph_h = nullptr;
+        hipGraphExec_t exec = nullptr;
+        c->capturing = true;
+        hipError_t e = hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) {
+            c->capturing = false;
+            return fail(c, OVL_E_HIP, std::string("hipStreamBeginCapture: ") + hipGetErrorString(e));
+        }
+        const long long l0 = c->hop_launches;
+        const double b0 = c->hop_bytes;
+        long long vl0[NVAR];
+        double vb0[NVAR];
+        for (int v = 0; v < NVAR; ++v) { vl0[v] = c->var_launches[v]; vb0[v] = c->var_bytes[v]; }
+        const int s0[4] = {c->fp64_steps, c->fp32_steps, c->fp16_steps, c->fp48_steps};
+        int r = one_step(c, o);
+        hipError_t ee = hipStreamEndCapture(c->st, &graph_h);
         c->capturing = false;
-        return fail(c, OVL_E_HIP, std::string("hipStreamBeginCapture: ") + hipGetErrorString(e));
-    }
-    const long long l0 = c->hop_launches;
-    const double b0 = c->hop_bytes;
-    long long vl0[NVAR];
-    double vb0[NVAR];
-    for (int v = 0; v < NVAR; ++v) { vl0[v] = c->var_launches[v]; vb0[v] = c->var_bytes[v]; }
-    const int s64 = c->fp64_steps, s32 = c->fp32_steps, s16 = c->fp16_steps, s48 = c->fp48_steps;
-    int r = one_step(c, o);
-    hipError_t ee = hipStreamEndCapture(c->st, &graph_h);
-    c->capturing = false;
-    if (r != OVL_OK) {
-        if (graph_h) hipGraphDestroy(graph_h);
-        return r;
-    }
-    if (ee != hipSuccess) return fail(c, OVL_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ee));
-    const long long per_l = c->hop_launches - l0;
-    const double per_b = c->hop_bytes - b0;
-    const int p64 = c->fp64_steps - s64, p32 = c->fp32_steps - s32, p16 = c->fp16_steps - s16;
-    const int p48 = c->fp48_steps - s48;
-    e = hipGraphInstantiate(&exec, graph_h, nullptr, nullptr, 0);
-    if (e != hipSuccess) {
+        // the capture only recorded: the counters advance with the replays below
+        c->step_dl = c->hop_launches - l0;
+        c->step_db = c->hop_bytes - b0;
+        for (int v = 0; v < NVAR; ++v) {
+            c->step_dvl[v] = c->var_launches[v] - vl0[v];
+            c->step_dvb[v] = c->var_bytes[v] - vb0[v];
+            c->var_launches[v] = vl0[v];
+            c->var_bytes[v] = vb0[v];
+        }
+        c->hop_launches = l0;
+        c->hop_bytes = b0;
+        c->step_dp[0] = c->fp64_steps - s0[0];
+        c->step_dp[1] = c->fp32_steps - s0[1];
+        c->step_dp[2] = c->fp16_steps - s0[2];
+        c->step_dp[3] = c->fp48_steps - s0[3];
+        c->fp64_steps = s0[0];
+        c->fp32_steps = s0[1];
+        c->fp16_steps = s0[2];
+        c->fp48_steps = s0[3];
+        if (r != OVL_OK) {
+            if (graph_h) hipGraphDestroy(graph_h);
+            return r;
+        }
+        if (ee != hipSuccess) return fail(c, OVL_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ee));
+        e = hipGraphInstantiate(&exec, graph_h, nullptr, nullptr, 0);
         hipGraphDestroy(graph_h);
-        return fail(c, OVL_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        if (e != hipSuccess) return fail(c, OVL_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        c->step_exec = exec;
+        c->step_key = key;
+        c->captures++;
     }
-    for (int s = 0; s < nsteps && e == hipSuccess; ++s) e = hipGraphLaunch(exec, c->st);
-    hipGraphExecDestroy(exec);
-    hipGraphDestroy(graph_h);
-    if (e != hipSuccess) return fail(c, OVL_E_HIP, std::string("hipGraphLaunch: ") + hipGetErrorString(e));
-    c->hop_launches = l0 + per_l * nsteps;
-    c->hop_bytes = b0 + per_b * nsteps;
+    hipError_t e = hipSuccess;
+    for (int s = 0; s < nsteps && e == hipSuccess; ++s) e = hipGraphLaunch(c->step_exec, c->st);
+    if (e != hipSuccess) {
+        drop_step_graph(c);
+        return fail(c, OVL_E_HIP, std::string("hipGraphLaunch: ") + hipGetErrorString(e));
+    }
+    c->hop_launches += c->step_dl * nsteps;
+    c->hop_bytes += c->step_db * nsteps;
     for (int v = 0; v < NVAR; ++v) {
-        c->var_launches[v] = vl0[v] + (c->var_launches[v] - vl0[v]) * nsteps;
-        c->var_bytes[v] = vb0[v] + (c->var_bytes[v] - vb0[v]) * nsteps;
+        c->var_launches[v] += c->step_dvl[v] * nsteps;
+        c->var_bytes[v] += c->step_dvb[v] * nsteps;
     }
-    c->fp64_steps = s64 + p64 * nsteps;
-    c->fp32_steps = s32 + p32 * nsteps;
-    c->fp16_steps = s16 + p16 * nsteps;
-    c->fp48_steps = s48 + p48 * nsteps;
+    c->fp64_steps += c->step_dp[0] * nsteps;
+    c->fp32_steps += c->step_dp[1] * nsteps;
+    c->fp16_steps += c->step_dp[2] * nsteps;
+    c->fp48_steps += c->step_dp[3] * nsteps;
     return end_timing(c);
 }
 

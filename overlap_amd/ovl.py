@@ -76,7 +76,7 @@ class Timing(C.Structure):
     _fields_ = [("total_ms", C.c_double), ("hop_ms", C.c_double), ("hop_launches", C.c_longlong),
                 ("hop_bytes", C.c_double), ("fp64_steps", C.c_int), ("fp32_steps", C.c_int),
                 ("fp16_steps", C.c_int), ("fp48_steps", C.c_int), ("var_ms", C.c_double * 5),
-                ("var_launches", C.c_longlong * 5), ("var_bytes", C.c_double * 5)]
+                ("var_launches", C.c_longlong * 5), ("var_bytes", C.c_double * 5), ("graph_captures", C.c_int)]
 
 
 _lib = None

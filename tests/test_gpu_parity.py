@@ -308,6 +308,45 @@ def test_graph_replay_matches_stream(gpu):
             assert np.array_equal(a, b)
 
 
+def test_kept_graph_follows_the_context(gpu):
+    # a graph-mode update keeps its captured step: a later call with nothing changed replays it
+    # without capturing, a change of options, of tuning or of the FP16 scales (larger modes)
+    # captures anew, and every sequence gives the bits of the same calls on the stream
+    print("[case] kept step graph", flush=True)
+    n = (6, 4, 4)
+
+    def opts(dt, graph):
+        return ovl.StepOpts(kind=ovl.OVERLAP, dt=dt, gauge=ovl.GAUGE_LINKS, use_graph=graph,
+                            sign=ovl.sign_opts(degree=64, cut=0.1, precision=ovl.PREC_MIXED, head48=4, head=12,
+                                               head16=30))
+    runs = []
+    for graph in (0, 1):
+        ctx, L, ucc, links, e, psi = make(3, n, 32, 57, sigma=0.3, M=1.2, nb=3)
+        ctx.set_modes(ovl.PSIMID, psi)
+        caps = []
+
+        def step(dt, k):
+            ctx.fermion_update(opts(dt, graph), k)
+            caps.append(ctx.timing().graph_captures)
+        step(0.02, 2)  # first call: capture
+        step(0.02, 3)  # unchanged: replay
+        step(0.01, 1)  # other options
+        t = ctx.get_tuning(3)
+        ctx.set_tuning(3, t.traversal, t.slabs, t.nt_store, 3 if t.waves_per_eu != 3 else 2)
+        step(0.01, 1)  # other tuning of the 48-bit stencil
+        big = ctx.get_modes(ovl.PSI) * 1024.0
+        ctx.set_modes(ovl.PSI, big)
+        ctx.set_modes(ovl.PSIMID, big)
+        step(0.01, 1)  # FP16 scales 2^-10 smaller
+        step(0.01, 2)  # unchanged: replay
+        runs.append((caps, ctx.get_modes(ovl.PSI), ctx.get_modes(ovl.PSIMID), ctx.get_links()))
+        ctx.close()
+    assert runs[0][0] == [0] * 6
+    assert runs[1][0] == [1, 0, 1, 1, 1, 0]
+    for a, b in zip(runs[1][1:], runs[0][1:]):
+        assert np.isfinite(b).all() and np.array_equal(a, b)
+
+
 @pytest.mark.parametrize("nc", [2, 3])
 def test_observables_vs_oracle(gpu, nc):
     n = (5, 4, 4)
