@@ -150,3 +150,32 @@ def test_bench_refuses_more_than_one_gpu():
                        text=True, timeout=120)
     assert r.returncode != 0
     assert "one GPU" in json.loads(r.stdout.strip().splitlines()[-1])["error"]
+
+
+def test_python_structs_match_the_header(tmp_path):
+    # the ctypes mirrors of the parameter, option and result structs have the size and field
+    # offsets of include/ovl.h as a C compiler lays them out
+    import ctypes as C
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    pairs = [("ovl_params", ovl.Params), ("ovl_ensemble_params", ovl.EnsembleParams),
+             ("ovl_sign_opts", ovl.SignOpts), ("ovl_step_opts", ovl.StepOpts), ("ovl_tuning", ovl.Tuning),
+             ("ovl_observe", ovl.Observe), ("ovl_timing", ovl.Timing)]
+    lines, want = [], []
+    for cname, py in pairs:
+        lines.append(f'printf("%zu\\n", sizeof({cname}));')
+        want.append(C.sizeof(py))
+        for f in py._fields_:
+            lines.append(f'printf("%zu\\n", offsetof({cname}, {f[0]}));')
+            want.append(getattr(py, f[0]).offset)
+    src = tmp_path / "layout.c"
+    src.write_text('#include <stddef.h>\n#include <stdio.h>\n#include "ovl.h"\nint main(void)\n{\n    '
+                   + "\n    ".join(lines) + "\n    return 0;\n}\n")
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-I", os.path.join(root, "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    assert got == want
