@@ -129,7 +129,10 @@ def test_ensemble_wilson_leapfrog_full_gauge(gpu):
     for k, s in enumerate(sy):
         ens.set_system_modes(k, ovl.PSIMID, s[3])
     so = ovl.StepOpts(kind=ovl.WILSON, dt=dt, gauge=ovl.GAUGE_FULL, sign=ovl.sign_opts(), use_graph=1)
-    ens.fermion_update(so, steps)
+    ens.fermion_update(so, 1)
+    assert ens.timing().graph_captures == 1
+    ens.fermion_update(so, steps - 1)
+    assert ens.timing().graph_captures == 0  # the kept graph of the first call replays
     for k, (n, links, e, psi) in enumerate(sy):
         L = orc.lat(nc, n, 1.0, 0.1, 0.0)
         oo = orc.StepOpts(orc.WILSON, dt, 2, FLUX[k], 1.0, 0, 0.0, 0.0, 0.0, 8)
