@@ -316,6 +316,22 @@ def bench_next_rows(c: cf.Config, links, e):
             shutil.rmtree(d, ignore_errors=True)
         out["configuration_files"] = {"bytes": nbytes, "save_s": round(ws, 3), "load_s": round(wl, 3),
                                       "save_MBps": round(nbytes / ws / 1e6, 1), "load_MBps": round(nbytes / wl / 1e6, 1)}
+        # the single-mode drop-in with the reference's host arguments (WilsonEvoOperator::Compute:
+        # u_cc and phi in, psi out, pageable host arrays): every call copies them over PCIe
+        g = np.random.default_rng(11)
+        shp = (3 * c.volume, c.nc, c.nc)
+        ucc = (g.standard_normal(shp) + 1j * g.standard_normal(shp)) / c.nc
+        phi = g.standard_normal(c.volume * 4 * c.nc) + 1j * g.standard_normal(c.volume * 4 * c.nc)
+        ctx.wilson_evo_compute(ucc, phi)
+        reps = 20
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ctx.wilson_evo_compute(ucc, phi)
+        w = (time.perf_counter() - t0) / reps
+        hb = ucc.nbytes + 2 * phi.nbytes
+        out["host_buffer_wilson"] = {"lattice": list(c.n), "nc": c.nc, "modes_per_call": 1,
+                                     "ms_per_call": round(1e3 * w, 3), "site_updates_per_s": c.volume / w,
+                                     "host_bytes_per_call": hb, "host_GBps": round(hb / w / 1e9, 2)}
     with ovl.Context(c.nc, c.n, 256, m_fermion=c.m_fermion, m_dw=c.m_dw) as ctx:
         o = ovl.sign_opts(degree=c.cheb_n, cut=c.cheb_cut, precision=ovl.PREC_FP64)
         t0 = time.perf_counter()
