@@ -368,6 +368,7 @@ def main():
     ap.add_argument("--ensemble-only", action="store_true")
     ap.add_argument("--no-other", action="store_true", help="skip the other three single-lattice workloads")
     ap.add_argument("--no-next", action="store_true", help="skip the Gauss restore / files / eigenbasis timings")
+    ap.add_argument("--next-only", action="store_true", help="only the Gauss restore / files / eigenbasis timings")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -386,6 +387,10 @@ def main():
     t0 = time.perf_counter()
     links, e, psi = c.generate()
     log(f"inputs generated in {time.perf_counter() - t0:.1f}s")
+    if args.next_only:
+        del psi
+        print(json.dumps({"workload": c.name, "next_rows": bench_next_rows(c, links, e)}))
+        return
     ctx = ovl.Context(c.nc, c.n, c.nmodes, r_wilson=c.r_wilson, m_fermion=c.m_fermion, m_dw=c.m_dw,
                       mode_block=args.mb, traversal=args.traversal)
     ctx.set_links(links)
