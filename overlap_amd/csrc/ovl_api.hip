@@ -89,6 +89,7 @@ struct ovl_ctx {
     int slabs[NUM_PREC] = {1, 1, 1, 1};         // y slabs (TRAV_MODES) or sub-slabs per box (TRAV_BOXES)
     int nt_out[NUM_PREC] = {0, 0, 0, 0};        // streaming output stores
     int wpe[NUM_PREC] = {1, 1, 1, 1};           // occupancy variant
+    bool tuned = false;                         // ovl_set_tuning or ovl_autotune has run
     double step_xmax = 0.0;  // bound of the mode norms during ovl_fermion_update (FP16 scales)
     // the last sign polynomial (a function of degree and cut only): its Chebyshev coefficients and
     // the Clenshaw bounds of the ladder (t0 = -1) and of the FP16 scales (t0 = beta), empty until
@@ -2475,6 +2476,31 @@ int ovl_qed_eigenbasis(ovl_ctx *c, const ovl_sign_opts *o, int nb, double q, dou
         ovl_ctx *c;
         ~Restore() { c->stencil_custom = false; }
     } restore{c};
+    // a context never tuned (automatic traversal, no ovl_set_tuning / ovl_autotune) runs the
+    // eigenbasis's FP64 stencil as the autotuner picks it for large lattices: XCD boxes, streaming
+    // stores, two waves per SIMD (bit-identical to every other variant); its own tuning comes back
+    // afterwards
+    struct Retune {
+        ovl_ctx *c;
+        bool on;
+        ovl_tuning saved;
+        ~Retune()
+        {
+            if (!on) return;
+            ovl_set_tuning(c, P64, &saved);
+            c->tuned = false;
+        }
+    } retune{c, false, {}};
+    if (c->prm.traversal == 0 && c->boxes_ok && !c->tuned) {
+        RETCHK(ovl_get_tuning(c, P64, &retune.saved));
+        ovl_tuning t = retune.saved;
+        t.traversal = 2;
+        t.slabs = 1;
+        t.nt_store = 1;
+        t.waves_per_eu = 2;
+        RETCHK(ovl_set_tuning(c, P64, &t));
+        retune.on = true;
+    }
 
     // Structure of H = i (-i h_ov) on this background (exact, not an approximation):
     //  * colours do not mix (the link matrices are diag(phase, 0, ..)); colours >= 1 have no hopping,
@@ -2748,6 +2774,7 @@ int ovl_set_tuning(ovl_ctx *c, int precision, const ovl_tuning *t)
     RETCHK(set_traversal(c, precision, trav, t->slabs));
     c->nt_out[precision] = t->nt_store;
     c->wpe[precision] = t->waves_per_eu;
+    c->tuned = true;
     return OVL_OK;
 }
 
@@ -2856,6 +2883,7 @@ int ovl_autotune(ovl_ctx *c, char *report, int len)
         c->nt_out[prec] = best_nt;
     }
     c->profiling = prof;
+    c->tuned = true;
     rep += "chosen";
     for (int prec = 0; prec < NUM_PREC; ++prec) {
         char buf[96];
