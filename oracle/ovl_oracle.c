@@ -13,6 +13,7 @@
  *   orc_su2_iexp          compiled reference MatrixIExp; orc_su3_iexp: SU(2) embedding,
  *                         unitarity, det = 1, exp(A)exp(-A) = 1, small-angle series
  *   orc_update_links      exact constant-E solution, unitarity
+ *   orc_reunitarize       SVD polar factor with the determinant phase removed (numpy)
  *   orc_efield_update     Gauss-law conservation and energy conservation  (the reference's
  *                         plaquette macros are absent: formula restated from the Hamiltonian)
  *   orc_edotb             gauge invariance, E=0, linearity in E
@@ -306,6 +307,52 @@ void orc_update_links(const orc_lat *L, double *links, const double *efield, dou
                 m_mul(3, m, u, u);
                 link_store(3, links, s, mu, u);
             }
+        }
+}
+
+/* Projection of every link back onto SU(Nc).  SU(2): V/|V| in the 4-real format, the
+ * reference's MaxTraceProjection (SU2/GroupOperations.cpp:261-270).  SU(3): the unitary polar
+ * factor P = V (V^+ V)^{-1/2} (the U(3) matrix of largest Re Tr(P^+ V)) by the Newton iteration
+ * X <- (X + X^{-+})/2 (Higham, "Computing the polar decomposition", 1986), then P det(P)^{-1/3}
+ * with the principal cube root.  The reference's SU(3) MaxTraceProjection
+ * (SU3/GroupOperations.cpp:510-533, seven Cabibbo-Marinari sweeps) maximises Re Tr over SU(3)
+ * instead; both agree to second order in the distance of V from the group. */
+static void inv3(const cplx *a, cplx *r)
+{
+    cplx c00 = a[4] * a[8] - a[5] * a[7], c01 = a[5] * a[6] - a[3] * a[8], c02 = a[3] * a[7] - a[4] * a[6];
+    cplx det = a[0] * c00 + a[1] * c01 + a[2] * c02;
+    r[0] = c00 / det; r[3] = c01 / det; r[6] = c02 / det;
+    r[1] = (a[2] * a[7] - a[1] * a[8]) / det;
+    r[4] = (a[0] * a[8] - a[2] * a[6]) / det;
+    r[7] = (a[1] * a[6] - a[0] * a[7]) / det;
+    r[2] = (a[1] * a[5] - a[2] * a[4]) / det;
+    r[5] = (a[2] * a[3] - a[0] * a[5]) / det;
+    r[8] = (a[0] * a[4] - a[1] * a[3]) / det;
+}
+
+void orc_reunitarize(const orc_lat *L, double *links)
+{
+    int V = lat_vol(L), nc = L->nc;
+    for (int s = 0; s < V; s++)
+        for (int mu = 0; mu < 3; mu++) {
+            if (nc == 2) {
+                double *u = links + 4 * (mu + 3 * s);
+                double nrm = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2] + u[3] * u[3]);
+                for (int k = 0; k < 4; k++) u[k] /= nrm;
+                continue;
+            }
+            cplx x[9], xi[9];
+            link_matrix(3, links, s, mu, x);
+            for (int it = 0; it < 6; it++) {
+                inv3(x, xi);
+                for (int i = 0; i < 3; i++)
+                    for (int j = 0; j < 3; j++) x[i * 3 + j] = 0.5 * (x[i * 3 + j] + conj(xi[j * 3 + i]));
+            }
+            cplx det = x[0] * (x[4] * x[8] - x[5] * x[7]) - x[1] * (x[3] * x[8] - x[5] * x[6]) +
+                       x[2] * (x[3] * x[7] - x[4] * x[6]);
+            cplx f = cexp(-I * carg(det) / 3.0);
+            for (int i = 0; i < 9; i++) x[i] *= f;
+            link_store(3, links, s, mu, x);
         }
 }
 

@@ -42,6 +42,8 @@ void orc_su2_iexp(double c, const double *alpha, double *out4);
 void orc_su3_iexp(double c, const double *alpha, double *out18);
 void orc_update_links(const orc_lat *L, double *links, const double *efield, double dt);
 void orc_efield_update(const orc_lat *L, const double *links, double *efield, double dt);
+/* every link projected back onto SU(Nc): SU(2) V/|V|, SU(3) polar factor with det 1 */
+void orc_reunitarize(const orc_lat *L, double *links);
 
 /* ---- operators (one mode, AoS) ---- */
 void orc_wilson_evo(const orc_lat *L, const double *ucc, const double *phi, double *psi);

@@ -162,6 +162,12 @@ def update_links(L: Lat, links, e, dt):
     return links
 
 
+def reunitarize(L: Lat, links):
+    links = np.array(links, np.float64)
+    lib().orc_reunitarize(C.byref(L), dp(links))
+    return links
+
+
 def leapfrog(L: Lat, links, e, psi, psimid, opts: StepOpts, nsteps):
     """Returns (links, psi, psimid, efield) after nsteps."""
     links = np.array(links, np.float64)

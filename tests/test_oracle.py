@@ -482,3 +482,354 @@ def test_tier_model_fp16_rounding():
     want = np.array([1.0, 1.0, 1.0 + 2.0 ** -9, 65504.0, 2.0 ** -24, 0.0, 0.0999755859375])
     assert np.array_equal(tiers.q16(v), want)  # ties to even, subnormal floor 2^-24
     assert tiers.pow2_scale(3.0) == 64.0 and tiers.pow2_scale(256.0) == 1.0 and tiers.pow2_scale(0.0) == 1.0
+
+
+# ---------------------------------------------------------------- closed forms of the free theory
+# Dirac basis of DiracAlgebra.cpp:5-9 (gamma_k anti-Hermitian, gamma_k^2 = -1)
+G0 = np.diag([1, 1, -1, -1]).astype(complex)
+GK = [np.array([[0, 0, 0, 1], [0, 0, 1, 0], [0, -1, 0, 0], [-1, 0, 0, 0]], complex),
+      np.array([[0, 0, 0, -1j], [0, 0, 1j, 0], [0, 1j, 0, 0], [-1j, 0, 0, 0]], complex),
+      np.array([[0, 0, 1, 0], [0, 0, 0, -1], [-1, 0, 0, 0], [0, 1, 0, 0]], complex)]
+G5 = np.block([[np.zeros((2, 2)), np.eye(2)], [np.eye(2), np.zeros((2, 2))]]).astype(complex)
+
+
+def dirac_spinor(pp, m_eff, lam):
+    """Positive-frequency eigenvector of gamma0 (m_eff + gamma.pp) with helicity lam = +-1, in the
+    helicity construction of FermionField.cpp:193-248 (generic momentum: pp_x, pp_y not both 0).
+    Returns (u, omega)."""
+    pabs = np.linalg.norm(pp)
+    om = np.sqrt(pabs ** 2 + m_eff ** 2)
+    q = -(pp[2] - lam * pabs) / (pp[0] - 1j * pp[1])
+    phi = np.array([1.0, q]) / np.sqrt(1 + abs(q) ** 2)
+    norm = np.sqrt(2 * om * (om - m_eff) / pabs ** 2)
+    return np.concatenate([phi, lam * (om - m_eff) / pabs * phi]) / norm, om
+
+
+def plane_wave(n, nc, colour, p_idx, u):
+    """psi(x) = u exp(2 pi i p.x / N) / sqrt(V) on one colour, AoS [site][colour][spin]."""
+    V = int(np.prod(n))
+    s = np.arange(V)
+    x, y, z = s % n[0], (s // n[0]) % n[1], s // (n[0] * n[1])
+    ph = np.exp(2j * np.pi * (p_idx[0] * x / n[0] + p_idx[1] * y / n[1] + p_idx[2] * z / n[2])) / np.sqrt(V)
+    psi = np.zeros((V, nc, 4), complex)
+    psi[:, colour, :] = ph[:, None] * u[None, :]
+    return psi.reshape(-1)
+
+
+FREE_MOMENTA = [(1, 2, 1), (3, 1, 2), (1, 4, 0), (2, 3, 1)]
+
+
+@pytest.mark.parametrize("nc", [2, 3])
+def test_free_wilson_plane_wave_has_positive_frequency(nc):
+    # GetWilsonDiracSpinor (FermionField.cpp:145-317): the particle spinor u of momentum p with
+    # m_eff = m + 2 r sum sin^2(p/2) satisfies h_W u e^{ipx} = +omega u e^{ipx} (particles go as
+    # e^{-i omega t}); the kernel without the overlap shift (M = 0) and r = 1.  A global sign of
+    # h_W, a swapped forward/backward hop or a transposed link fails it; the spectrum test above is
+    # blind to all three.
+    n = (4, 5, 3)
+    m = 0.1
+    L = orc.lat(nc, n, 1.0, m, 0.0)
+    V = int(np.prod(n))
+    g = cf.rng(71)
+    # random SU(Nc) gauge transform of unit links: the plane wave rotated with it stays an eigenvector
+    ucc = orc.links_to_ucc(L, unit_links(nc, V))
+    for p in FREE_MOMENTA:
+        k = 2 * np.pi * np.array(p) / np.array(n)
+        pp = np.sin(k)
+        m_eff = m + 2.0 * np.sum(np.sin(k / 2) ** 2)
+        for lam in (1.0, -1.0):
+            u, om = dirac_spinor(pp, m_eff, lam)
+            assert abs(np.linalg.norm(u) - 1) < 1e-14
+            psi = plane_wave(n, nc, int(g.integers(nc)), p, u)
+            h = orc.hw(L, ucc, psi)
+            assert np.abs(h - om * psi).max() < 1e-14
+            assert np.abs(orc.wilson_evo(L, ucc, psi) - (-1j) * om * psi).max() < 1e-14
+
+
+def overlap_free(p, n, M):
+    """FermionField.cpp:331-370: momentum pp and mass of the free overlap operator (m = 0)."""
+    k = 2 * np.pi * np.array(p) / np.array(n)
+    hp = np.sin(k)
+    hp5 = 3.0 - M - np.sum(np.cos(k))
+    s = np.sqrt(np.sum(hp ** 2) + hp5 ** 2)
+    return M * hp / s, M * (1 + hp5 / s)
+
+
+@pytest.mark.parametrize("nc,M", [(2, 1.4), (3, 1.2)])
+def test_free_overlap_plane_wave(nc, M):
+    # GetOverlapDiracSpinor (FermionField.cpp:331-517): at m = 0 the overlap Hamiltonian of momentum
+    # p is gamma0 (m_eff + gamma.pp) with pp = M hp/|s|, m_eff = M (1 + hp5/|s|); its particle
+    # spinor times e^{ipx} must come back as -i omega psi from the overlap apply.  Pins the sign of
+    # h_W inside eps(h_W) and the factor M of OverlapU.cpp:188-201.
+    n = (4, 5, 3)
+    L = orc.lat(nc, n, 1.0, 0.0, M)
+    V = int(np.prod(n))
+    ucc = orc.links_to_ucc(L, unit_links(nc, V))
+    lam_b = orc.norm_bound(L)
+    deg, cut = 400, 0.04
+    coef = orc.cheb_coeffs(deg, cut)
+    for p in FREE_MOMENTA:
+        pp, m_eff = overlap_free(p, n, M)
+        for lam in (1.0, -1.0):
+            u, om = dirac_spinor(pp, m_eff, lam)
+            psi = plane_wave(n, nc, nc - 1, p, u)
+            out = orc.overlap_evo_cheb(L, ucc, lam_b, cut, deg, coef, psi)
+            assert np.abs(out - (-1j) * om * psi).max() < 1e-10 * np.abs(psi).max()
+
+
+def test_free_overlap_spectrum_with_mass():
+    # m != 0: h_ov(p) = M gamma0 [(1 + m/2) + (1 - m/2)(hp5' + gamma.hp)/|s'|] with the kernel's
+    # hp5' = 3 + m - M - sum cos p, so its eigenvalues are +-M sqrt(A^2 + B^2),
+    # A = (1 + m/2) + (1 - m/2) hp5'/|s'|, B = (1 - m/2)|hp|/|s'| (gamma0 (a + gamma.b) squares to
+    # a^2 + b^2), each twice per colour.  A wrong M factor or mass term moves the spectrum (it is
+    # symmetric under exchanging (1 + m/2) and (1 - m/2): the dense test below tells them apart).
+    n = (3, 2, 4)
+    nc, m, M = 2, 0.3, 1.3
+    L = orc.lat(nc, n, 1.0, m, M)
+    V = int(np.prod(n))
+    ucc = orc.links_to_ucc(L, unit_links(nc, V))
+    deg, cut = 400, 0.04
+    coef = orc.cheb_coeffs(deg, cut)
+    lam_b = orc.norm_bound(L)
+    H = 1j * dense(L, ucc, lambda L_, u_, e: orc.overlap_evo_cheb(L_, u_, lam_b, cut, deg, coef, e))
+    assert np.abs(H - H.conj().T).max() < 1e-10
+    ev = np.sort(np.linalg.eigvalsh(0.5 * (H + H.conj().T)))
+    want = []
+    for px in range(n[0]):
+        for py in range(n[1]):
+            for pz in range(n[2]):
+                k = 2 * np.pi * np.array([px / n[0], py / n[1], pz / n[2]])
+                hp = np.sin(k)
+                hp5 = 3.0 + m - M - np.sum(np.cos(k))
+                s = np.sqrt(np.sum(hp ** 2) + hp5 ** 2)
+                A = (1 + m / 2) + (1 - m / 2) * hp5 / s
+                B = (1 - m / 2) * np.linalg.norm(hp) / s
+                w = M * np.sqrt(A ** 2 + B ** 2)
+                want += [w, -w] * (2 * nc)
+    assert np.abs(ev - np.sort(want)).max() < 1e-10
+
+
+def test_overlap_combination_dense_with_mass():
+    # OverlapU.cpp:188-201 at m != 0, M != 1 on random links: -i M [(1 + m/2) g0 phi + (1 - m/2) S phi]
+    # with S = sign(h_W) from a dense eigendecomposition of the kernel (which carries m and M too)
+    g = cf.rng(13)
+    n = (3, 3, 2)
+    m, M = 0.25, 1.35
+    L = orc.lat(2, n, 1.0, m, M)
+    V = int(np.prod(n))
+    ucc = orc.links_to_ucc(L, cf.su2_links(n, 0.4, g))
+    w, v = np.linalg.eigh(dense(L, ucc))
+    lam = orc.norm_bound(L)
+    cut = 0.8 * np.abs(w).min() / lam
+    deg = 500
+    coef = orc.cheb_coeffs(deg, cut)
+    S = v @ np.diag(np.sign(w)) @ v.conj().T
+    g0 = np.tile(np.array([1, 1, -1, -1]), V * 2)
+    for phi in cf.spinors(n, 2, 2, g):
+        want = -1j * M * ((1 + m / 2) * g0 * phi + (1 - m / 2) * (S @ phi))
+        assert np.abs(orc.overlap_evo_cheb(L, ucc, lam, cut, deg, coef, phi) - want).max() < 1e-10
+
+
+@pytest.mark.parametrize("nc", [2, 3])
+def test_currents_of_free_plane_waves(nc):
+    # For a normalised plane wave psi = u e^{ipx}/sqrt(V) with -i h_ov psi = -i omega psi, where u
+    # is the helicity-lam eigenvector of gamma0 (m_eff + gamma.pp), the currents of
+    # OverlapCurrents.cpp:137-228 are fixed by the Dirac algebra: u^+u = 1, u^+ g0 u = m_eff/omega,
+    # u^+ g0 g_k u = pp_k/omega (Feynman-Hellmann), u^+ g5 u = lam |pp|/omega, u^+ g0 g_k g5 u =
+    # lam pp_k/|pp|, while u^+ g_k u and u^+ g5 g0 u are imaginary.  Per site and mode of weight w
+    # (+1 for an even global mode index, -1 for odd, FermionField.cpp:109-122):
+    #   j0_V = w (1 - m_eff/2) / 2V^2     jk_V = w pp_k / (omega 2V^2)
+    #   j0_A = w lam |pp| / (omega 2V^2)  jk_A = w lam pp_k/|pp| (1 - m_eff/2) / 2V^2
+    n = (4, 5, 3)
+    M = 1.3
+    V = int(np.prod(n))
+    L = orc.lat(nc, n, 1.0, 0.0, M)
+    modes, hmodes = [], []
+    wjv, wja = np.zeros(4), np.zeros(4)
+    first = 3
+    for k, (p, lam) in enumerate([((1, 2, 1), 1.0), ((3, 1, 2), -1.0), ((2, 3, 1), -1.0)]):
+        pp, m_eff = overlap_free(p, n, M)
+        u, om = dirac_spinor(pp, m_eff, lam)
+        psi = plane_wave(n, nc, k % nc, p, u)
+        modes.append(psi)
+        hmodes.append(-1j * om * psi)
+        w = 1.0 if (first + k) % 2 == 0 else -1.0
+        pa = np.linalg.norm(pp)
+        wjv += w * np.array([1 - m_eff / 2, *(pp / om)])
+        wja += w * np.array([lam * pa / om, *(lam * pp / pa * (1 - m_eff / 2))])
+    jv, ja = orc.currents(L, np.array(modes), np.array(hmodes), first_mode=first)
+    assert np.abs(jv.reshape(V, 4) - wjv / (2 * V * V)).max() < 1e-15
+    assert np.abs(ja.reshape(V, 4) - wja / (2 * V * V)).max() < 1e-15
+    # the same with -i h_ov psi from the overlap operator itself
+    ucc = orc.links_to_ucc(L, unit_links(nc, V))
+    coef = orc.cheb_coeffs(400, 0.04)
+    h2 = np.array([orc.overlap_evo_cheb(L, ucc, orc.norm_bound(L), 0.04, 400, coef, x) for x in modes])
+    jv2, ja2 = orc.currents(L, np.array(modes), h2, first_mode=first)
+    assert np.abs(jv2 - jv).max() < 1e-12 / V and np.abs(ja2 - ja).max() < 1e-12 / V
+
+
+@pytest.mark.parametrize("nc", [2, 3])
+def test_ritz_bounds_are_extreme_eigenvalues(nc):
+    # RitzSolver.cpp:92-297: sign +1 converges to the smallest eigenvalue of h_W^2, sign -1 to minus
+    # the largest (its stopping test never fires, so it runs all restarts)
+    g = cf.rng(17 + nc)
+    n = (3, 3, 3) if nc == 2 else (2, 3, 2)
+    L = orc.lat(nc, n, 1.0, 0.1, 1.0)
+    links = cf.su2_links(n, 0.4, g) if nc == 2 else cf.su3_links(n, 0.4, g)
+    ucc = orc.links_to_ucc(L, links)
+    H = dense(L, ucc)
+    ev = np.linalg.eigvalsh(H @ H)
+    start = g.uniform(size=H.shape[0]).astype(complex)
+    start /= np.linalg.norm(start)
+    lo = orc.ritz(L, ucc, start, sign=1.0, nrestart=20)
+    hi = orc.ritz(L, ucc, start, sign=-1.0, nrestart=4, nitlz=300)
+    assert abs(lo - ev[0]) < 1e-9 * ev[-1]
+    assert abs(-hi - ev[-1]) < 1e-9 * ev[-1]
+
+
+def test_su2_exponential_against_matrix_exponential():
+    # MatrixIExp (SU2/AlgebraOperations.cpp:37-55): exp(i c alpha.sigma/2) in the 4-real format of
+    # GetMatrix (SU2/GroupOperations.cpp:213-225), against the eigendecomposition exponential
+    g = cf.rng(23)
+    s = cf.pauli_generators()
+    for k in range(30):
+        a = g.normal(size=3) * (0.0 if k == 0 else 1.0)
+        c = g.normal() * 3.0
+        u = orc.su2_iexp(c, a)
+        m = cf.links_to_matrices(2, np.tile(u, 3))[0, 0]
+        h = c * np.einsum("a,aij->ij", a, s)
+        w, v = np.linalg.eigh(h)
+        want = v @ np.diag(np.exp(1j * w)) @ v.conj().T
+        assert np.abs(m - want).max() < 1e-14
+
+
+def diag_links(nc, theta):
+    """links diag(e^{i theta}, e^{-i theta}[, 1]) = exp(2 i theta t^3) per (site, mu), reference format."""
+    V = theta.shape[0]
+    if nc == 2:
+        u = np.zeros((V, 3, 4))
+        u[..., 2], u[..., 3] = np.sin(theta), np.cos(theta)
+        return u.reshape(-1)
+    m = np.zeros((V, 3, 3, 3), complex)
+    m[..., 0, 0], m[..., 1, 1], m[..., 2, 2] = np.exp(1j * theta), np.exp(-1j * theta), 1
+    cm = np.swapaxes(m, -1, -2).reshape(V, 3, 9)
+    out = np.empty((V, 3, 18))
+    out[..., 0::2], out[..., 1::2] = cm.real, cm.imag
+    return out.reshape(-1)
+
+
+@pytest.mark.parametrize("nc", [2, 3])
+def test_edotb_of_abelian_fields(nc):
+    # Abelian background along t^3: plane z carries a uniform magnetic flux Phi_z per xy plaquette
+    # (the QED construction of QED.cpp:62-84 with its own flux count), E_z = E(z) t^3.  Links commute,
+    # so transports are trivial: the clover leaves are exp(2i Phi t^3), the 1x2 rectangles
+    # exp(4i Phi t^3), and B^3_z = 2 ReTr(i t^3 Ubar) = -2 [(5/3) sin Phi - (1/3) sin 2 Phi]
+    # (ChernSimonsOperators.cpp:415-433: 5/12 of 4 leaves, -1/24 of 8 rectangles); the E average is
+    # -E(z+1)/12 + 7E(z)/12 + 7E(z-1)/12 - E(z-2)/12.  So sum_x E.B = n0 n1 sum_z Eavg(z) B(z):
+    # swapped or wrong improvement weights fail it.
+    n = (6, 5, 6)
+    V = int(np.prod(n))
+    ng = nc * nc - 1
+    flux = [0, 1, 3, 2, 5, 1]
+    s = np.arange(V)
+    x, y, z = s % n[0], (s // n[0]) % n[1], s // (n[0] * n[1])
+    theta = np.zeros((V, 3))
+    nbz = np.array(flux)[z]
+    theta[:, 1] = -2 * np.pi * nbz * x / (n[0] * n[1])                      # U_y(x, y)
+    theta[:, 0] = np.where(x == n[0] - 1, 2 * np.pi * nbz * y / n[1], 0.0)  # boundary twist of U_x
+    L = orc.lat(nc, n)
+    links = diag_links(nc, theta)
+    g = cf.rng(29)
+    Ez = g.normal(size=n[2])
+    e = np.zeros((V, 3, ng))
+    e[:, 2, 2] = Ez[z]
+    # flux per plaquette from the links themselves (the (0,0) entry), uniform in each plane
+    sx = (x + 1) % n[0] + n[0] * (y + n[1] * z)
+    sy = x + n[0] * ((y + 1) % n[1] + n[1] * z)
+    ang = theta[s, 0] + theta[sx, 1] - theta[sy, 0] - theta[s, 1]
+    Phi = np.angle(np.exp(1j * ang))
+    Phi_z = np.array([Phi[z == k][0] for k in range(n[2])])
+    assert np.abs(np.angle(np.exp(1j * (Phi - Phi_z[z])))).max() < 1e-12
+    B = -2 * ((5 / 3) * np.sin(Phi_z) - (1 / 3) * np.sin(2 * Phi_z))
+    Eavg = -np.roll(Ez, -1) / 12 + 7 * Ez / 12 + 7 * np.roll(Ez, 1) / 12 - np.roll(Ez, 2) / 12
+    want = n[0] * n[1] * np.sum(Eavg * B)
+    got = orc.edotb(L, links, e.reshape(-1))
+    assert abs(want) > 1.0
+    assert abs(got - want) < 1e-12 * abs(want)
+
+
+@pytest.mark.parametrize("nc", [2, 3])
+def test_unitarity_violation_of_scaled_links(nc):
+    # UnitarityNorm (SU2/GroupOperations.cpp:172-184, SU3/GroupOperations.cpp:351-363) is
+    # |V V^+ - 1|_F: for V = lam U with U unitary it is |lam^2 - 1| (SU(2), V V^+ in the 4-real format
+    # is |V|^2 times the unit) and sqrt(3)|lam^2 - 1| (SU(3)); UnitarityViolation.cpp:13-51 takes the
+    # max and the rms over all 3V links
+    g = cf.rng(37)
+    n = (3, 4, 2)
+    V = int(np.prod(n))
+    L = orc.lat(nc, n)
+    links = cf.su2_links(n, 0.6, g) if nc == 2 else cf.su3_links(n, 0.6, g)
+    lam = 1.0 + g.normal(0, 0.01, size=3 * V)
+    per = links.shape[0] // (3 * V)
+    scaled = (links.reshape(3 * V, per) * lam[:, None]).reshape(-1)
+    f = 1.0 if nc == 2 else np.sqrt(3.0)
+    viol = f * np.abs(lam ** 2 - 1)
+    mx, rms = orc.unitarity_violation(L, scaled)
+    assert abs(mx - viol.max()) < 1e-14 and abs(rms - np.sqrt(np.mean(viol ** 2))) < 1e-14
+
+
+@pytest.mark.parametrize("nc", [2, 3])
+def test_link_update_against_matrix_exponential(nc):
+    # Dynamics.cpp:97-138: U_mu(x) <- exp(i cU E^a_mu(x) t^a) U_mu(x) with cU = -dt, against the
+    # eigendecomposition exponential of the Hermitian generator (t = sigma/2, lambda/2); a wrong sign
+    # of dt, a generator normalisation or a product order fails it
+    g = cf.rng(43 + nc)
+    n = (3, 2, 3)
+    V = int(np.prod(n))
+    L = orc.lat(nc, n)
+    links = cf.su2_links(n, 0.5, g) if nc == 2 else cf.su3_links(n, 0.5, g)
+    e = cf.efield(n, nc, 0.8, g)
+    dt = 0.37
+    out = cf.links_to_matrices(nc, orc.update_links(L, links, e, dt))
+    U = cf.links_to_matrices(nc, links)
+    t = cf.pauli_generators() if nc == 2 else cf.gell_mann_generators()
+    h = -dt * np.einsum("xma,aij->xmij", e.reshape(V, 3, nc * nc - 1), t)
+    w, v = np.linalg.eigh(h)
+    ex = np.einsum("xmij,xmj,xmkj->xmik", v, np.exp(1j * w), v.conj())
+    assert np.abs(out - ex @ U).max() < 1e-14
+
+
+@pytest.mark.parametrize("nc", [2, 3])
+def test_reunitarize_is_the_polar_projection(nc):
+    # links pushed off the group, V = U (1 + eps X) (SU(3)) or scaled quaternions (SU(2)): the
+    # projection is V/|V| (SU2/GroupOperations.cpp:261-270) and, for SU(3), the unitary polar factor
+    # W Z^+ of the SVD V = W S Z^+ with its determinant phase divided out (principal cube root)
+    g = cf.rng(47 + nc)
+    n = (3, 2, 2)
+    V = int(np.prod(n))
+    L = orc.lat(nc, n)
+    links = cf.su2_links(n, 0.6, g) if nc == 2 else cf.su3_links(n, 0.6, g)
+    if nc == 2:
+        off = (links.reshape(-1, 4) * (1 + g.normal(0, 1e-3, size=(3 * V, 1)))).reshape(-1)
+        want = links
+    else:
+        U = cf.links_to_matrices(3, links)
+        X = g.normal(size=U.shape) + 1j * g.normal(size=U.shape)
+        Voff = U @ (np.eye(3) + 1e-3 * X)
+        w, _, zh = np.linalg.svd(Voff)
+        P = w @ zh
+        P = P * np.exp(-1j * np.angle(np.linalg.det(P)) / 3)[..., None, None]
+        cm = np.swapaxes(Voff, -1, -2).reshape(V, 3, 9)
+        off = np.empty((V, 3, 18))
+        off[..., 0::2], off[..., 1::2] = cm.real, cm.imag
+        off = off.reshape(-1)
+        cm = np.swapaxes(P, -1, -2).reshape(V, 3, 9)
+        want = np.empty((V, 3, 18))
+        want[..., 0::2], want[..., 1::2] = cm.real, cm.imag
+        want = want.reshape(-1)
+    assert orc.unitarity_violation(L, off)[0] > 1e-4
+    out = orc.reunitarize(L, off)
+    assert np.abs(out - want).max() < 1e-14
+    assert orc.unitarity_violation(L, out)[0] < 1e-14
+    if nc == 3:
+        assert np.abs(np.linalg.det(cf.links_to_matrices(3, out)) - 1).max() < 1e-14
