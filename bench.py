@@ -45,6 +45,40 @@ def step_opts(c: cf.Config, prec, graph, tol=1e-10, chunk=0):
     return ovl.StepOpts(kind=kind, dt=c.dt, gauge=ovl.GAUGE_LINKS, sign=so, use_graph=int(graph))
 
 
+def host_description() -> dict:
+    """nproc, the CPUs this process may run on, OMP_NUM_THREADS and the CPU model name."""
+    model = ""
+    try:
+        r = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10)
+        for line in r.stdout.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except (OSError, subprocess.SubprocessError):
+        pass
+    if not model and os.path.exists("/proc/cpuinfo"):
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return {"nproc": os.cpu_count(), "affinity_cpus": aff, "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
+            "model": model or platform.processor() or platform.machine()}
+
+
+def default_cpu_threads() -> int:
+    """The CPU share of this process: OMP_NUM_THREADS when set (the GPU box sets it to its 16-core
+    share per GPU), else the CPUs it may run on."""
+    try:
+        n = int(os.environ.get("OMP_NUM_THREADS", "0"))
+    except ValueError:
+        n = 0
+    if n > 0:
+        return n
+    return len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+
+
 def cpu_baseline(c: cf.Config, links, e, psi, threads: int, modes: int):
     """The oracle (plain C FP64 restatement, OpenMP over modes) on a bounded sample:
     `modes` mode functions, one full leapfrog step, on `threads` host threads."""
@@ -177,6 +211,9 @@ def bench_ensemble(ec: cf.EnsembleConfig, steps: int, warmup: int, tol: float):
     out = {}
     for name, prec in (("mixed", ovl.PREC_MIXED), ("fp64", ovl.PREC_FP64)):
         load()
+        # new links make the mixed sign's accuracy guard due: its check runs here, untimed (in a
+        # run it recurs every 100 steps)
+        ens.apply_overlap(ovl.PSI, ovl.SCRATCH, opts(prec).sign)
         ens.set_profiling(True)
         t0 = time.perf_counter()
         ens.fermion_update(opts(prec), steps)
@@ -210,15 +247,27 @@ def bench_ensemble(ec: cf.EnsembleConfig, steps: int, warmup: int, tol: float):
 
 
 VARIANT_NAMES = ["k_hop FP64", "k_hop FP32", "k_hop FP32->FP64", "k_hop FP16", "k_hop 48-bit"]
+# storage bytes per real of the spinor fields and of the links each stencil variant reads (FP16 runs
+# on the FP32 links, the 48-bit layout on the FP64 links)
+DEGREE_BYTES_PER_REAL = {0: (8, 8), 1: (4, 4), 3: (2, 4), 4: (6, 8)}
+
+
+def wilson_step_alg_bytes(c: cf.Config, real_bytes: int) -> float:
+    """Algorithmic bytes of one Wilson leapfrog step of all modes: two half steps of 3S + L/M per
+    site-mode (read the stencil input and the target, write the target; links once)."""
+    S = 4 * c.nc * 2 * real_bytes
+    Lb = 3 * (4 if c.nc == 2 else 18) * real_bytes
+    return 2 * (3 * S + Lb / c.nmodes) * c.volume * c.nmodes
 
 
 def bench_other_workloads(tol: float):
     """The other three single-lattice workloads, timed the same way as the main one (inputs
-    resident, autotuned stencil, HIP events around every stencil launch): Wilson leapfrogs in
-    FP64, the SU(2) overlap in the mixed form.  Their cost does not depend on the mode values, so
-    only the first 64 modes are drawn and repeated to fill the mode count."""
+    resident, autotuned stencil, HIP events around every stencil launch): the SU(2) Wilson
+    leapfrog in FP64, the SU(2) overlap in the mixed form, the SU(3) 32^3 Wilson leapfrog with FP32
+    spinor arithmetic (as that workload specifies; FP64 beside it).  Their cost does not depend on
+    the mode values, so only the first 64 modes are drawn and repeated to fill the mode count."""
     out = {}
-    for name, steps in (("su2_wilson_8", 20), ("su2_overlap_16", 2), ("su3_wilson_32", 3)):
+    for name, steps in (("su2_wilson_8", 20), ("su2_overlap_16", 2), ("su3_wilson_32", 10)):
         c = cf.CONFIGS[name]
         nm = min(c.nmodes, 64)
         links, e, psi = c.generate(nm)
@@ -234,26 +283,40 @@ def bench_other_workloads(tol: float):
             ctx.set_modes(ovl.PSIMID, psi[:k], first=f)
         del psi
         ctx.autotune()
-        prec = ovl.PREC_MIXED if c.kind == "overlap" else ovl.PREC_FP64
-        so = step_opts(c, prec, False, tol)
-        ctx.fermion_update(so, 1)
-        ctx.set_profiling(True)
-        t0 = time.perf_counter()
-        ctx.fermion_update(so, steps)
-        wall = time.perf_counter() - t0
-        ctx.set_profiling(False)
-        t = ctx.timing()
-        var = int(np.argmax([t.var_ms[i] for i in range(5)]))
-        gbps = t.var_bytes[var] / max(t.var_ms[var], 1e-9) / 1e6
-        out[name] = {"lattice": list(c.n), "nc": c.nc, "modes": c.nmodes, "kind": c.kind, "steps": steps,
-                     "precision": "mixed" if prec == ovl.PREC_MIXED else "fp64",
-                     "ms_per_step": round(1e3 * wall / steps, 3),
-                     "value": c.volume * c.nmodes * steps / wall, "unit": "site*mode updates/s",
-                     "kernel": VARIANT_NAMES[var], "kernel_GBps": round(gbps, 1),
-                     "kernel_frac_of_8TBps": round(gbps / HBM_PEAK_GBS, 4),
-                     "kernel_share_of_step": round(t.var_ms[var] / max(t.total_ms, 1e-9), 3)}
-        log(f"{name}: {out[name]['ms_per_step']:.2f} ms/step {out[name]['value']:.4e} site*mode/s, "
-            f"{VARIANT_NAMES[var]} {gbps:.0f} GB/s")
+        if c.kind == "overlap":
+            precs = [ovl.PREC_MIXED]
+        elif name == "su3_wilson_32":
+            precs = [ovl.PREC_FP32, ovl.PREC_FP64]
+        else:
+            precs = [ovl.PREC_FP64]
+        pname = {ovl.PREC_MIXED: "mixed", ovl.PREC_FP64: "fp64", ovl.PREC_FP32: "fp32"}
+        for prec in precs:
+            so = step_opts(c, prec, False, tol)
+            ctx.fermion_update(so, 1)
+            ctx.set_profiling(True)
+            t0 = time.perf_counter()
+            ctx.fermion_update(so, steps)
+            wall = time.perf_counter() - t0
+            ctx.set_profiling(False)
+            t = ctx.timing()
+            var = int(np.argmax([t.var_ms[i] for i in range(5)]))
+            gbps = t.var_bytes[var] / max(t.var_ms[var], 1e-9) / 1e6
+            r = {"lattice": list(c.n), "nc": c.nc, "modes": c.nmodes, "kind": c.kind, "steps": steps,
+                 "precision": pname[prec], "ms_per_step": round(1e3 * wall / steps, 3),
+                 "value": c.volume * c.nmodes * steps / wall, "unit": "site*mode updates/s",
+                 "kernel": VARIANT_NAMES[var], "kernel_GBps": round(gbps, 1),
+                 "kernel_frac_of_8TBps": round(gbps / HBM_PEAK_GBS, 4),
+                 "kernel_share_of_step": round(t.var_ms[var] / max(t.total_ms, 1e-9), 3)}
+            if c.kind == "wilson":
+                # the whole call: the FP32 form converts Psi and PsiMid to FP32 when it starts and back
+                # when it ends (in the timed call, amortised over its steps)
+                ab = wilson_step_alg_bytes(c, 4 if prec == ovl.PREC_FP32 else 8)
+                r["step_alg_bytes_per_site_mode"] = round(ab / (c.volume * c.nmodes), 2)
+                r["step_alg_frac"] = round(ab / (1e-3 * r["ms_per_step"]) / 1e9 / HBM_PEAK_GBS, 4)
+            key = name if prec == precs[0] else f"{name}_{pname[prec]}"
+            out[key] = r
+            log(f"{key}: {r['ms_per_step']:.2f} ms/step {r['value']:.4e} site*mode/s, "
+                f"{VARIANT_NAMES[var]} {gbps:.0f} GB/s")
         ctx.close()
         if c.nc == 2:
             out[name]["cpu_reference"] = reference_cpu(c, links, psi0)
@@ -359,8 +422,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="timed region only (layout / tile sweeps)")
-    ap.add_argument("--cpu-threads", type=int, default=16)
-    ap.add_argument("--cpu-modes", type=int, default=16)
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = OMP_NUM_THREADS, else the CPUs allowed")
+    ap.add_argument("--cpu-modes", type=int, default=0, help="0 = two per thread")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the ensemble workload")
     ap.add_argument("--no-traffic", action="store_true", help="skip the two rocprofv3 counter passes")
     ap.add_argument("--tuning", default="", help="fixed stencil tuning instead of the autotune sweep, e.g. "
@@ -399,7 +462,6 @@ def main():
         ctx.set_u1_flux(c.nb, c.q)
     ctx.set_modes(ovl.PSI, psi)
     ctx.set_modes(ovl.PSIMID, psi)
-    del psi
     # a plain copy of one FP64 field (k_lincomb<double>, known bytes): the calibration point of the
     # FETCH_SIZE / WRITE_SIZE counters in profiles/
     ctx.copy_field(ovl.PSI, ovl.SCRATCH)
@@ -459,7 +521,20 @@ def main():
     var = int(np.argmax([t.var_ms[i] for i in range(5)]))
     names = VARIANT_NAMES
     avg_ms = t.var_ms[var] / max(1, t.var_launches[var])
-    bytes_per_launch = t.var_bytes[var] / max(1, t.var_launches[var])
+    unfused_per_launch = t.var_bytes[var] / max(1, t.var_launches[var])
+    bytes_per_launch = unfused_per_launch
+    deg = None
+    if c.kind == "overlap" and var in DEGREE_BYTES_PER_REAL:
+        # the fused Clenshaw degree of the survey's roofline, 4S + 2L/M bytes per site-mode (read
+        # b_{k+1}, b_{k+2}, x, write b_k; links once per launch over all modes), is two launches here
+        # (w = h b_{k+1}, then the epilogue): per launch, half of it
+        rb, lb = DEGREE_BYTES_PER_REAL[var]
+        S = 4 * c.nc * 2 * rb
+        Lb = 3 * (4 if c.nc == 2 else 18) * lb
+        per_site_mode = 4 * S + 2 * Lb / c.nmodes
+        bytes_per_launch = per_site_mode * c.volume * c.nmodes / 2
+        deg = {"bytes_per_site_mode": per_site_mode, "S": S, "L": Lb, "launches_per_degree": 2,
+               "us_per_degree": round(2 * avg_ms * 1e3, 2), "bytes_per_degree": 2 * bytes_per_launch}
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     alg_step_bytes = None
     if c.kind == "overlap":
@@ -473,7 +548,15 @@ def main():
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                 "avg_launch_us": round(avg_ms * 1e3, 2), "alg_bytes_per_launch": bytes_per_launch,
                 "launches": int(t.var_launches[var]),
-                "kernel_share_of_step": round(t.var_ms[var] / max(t.total_ms, 1e-9), 3)}
+                "kernel_share_of_step": round(t.var_ms[var] / max(t.total_ms, 1e-9), 3),
+                "algorithmic_bytes": ("fused Clenshaw degree 4S + 2L/M per site-mode, two launches per degree"
+                                      if deg else "stencil input once, pointwise operands and output once"),
+                "degree": deg,
+                # the same launches counted as they run (unfused h_W^2: stencil input, each pointwise
+                # operand and the output once, 7S per degree)
+                "unfused_alg_bytes_per_launch": unfused_per_launch,
+                "unfused_achieved": round(unfused_per_launch / (avg_ms * 1e-3) / 1e9, 1),
+                "unfused_frac": round(unfused_per_launch / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     if alg_step_bytes:
         roofline["step_alg_bytes"] = alg_step_bytes
         roofline["step_alg_frac"] = round(alg_step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
@@ -533,6 +616,7 @@ def main():
         res["sign_residual_mixed"] = ctx.sign_residual(ovl.PSI, som)
         log(f"sign residual (mixed) {res['sign_residual_mixed']:.3e}")
     ctx.close()
+    del psi
 
     # HBM traffic of the dominant stencil variant, from two counter passes (child processes)
     if not args.no_traffic:
@@ -550,12 +634,13 @@ def main():
                 "write_calibration": round(tr["WRITE_SIZE"]["factor"], 4),
                 "kernel": tr["FETCH_SIZE"]["kernel"],
                 "traffic_over_algorithmic": round(traffic / bytes_per_launch, 3),
+                "traffic_over_unfused": round(traffic / unfused_per_launch, 3),
                 "traffic_GBps": round(traffic / (avg_ms * 1e-3) / 1e9, 1)}
             log("traffic", json.dumps(roofline["traffic_detail"]))
 
     ens = None
     if not args.no_ensemble:
-        ens = bench_ensemble(cf.ENSEMBLES["su2_overlap_ensemble"], 2, 1, args.tol)
+        ens = bench_ensemble(cf.ENSEMBLES["su2_overlap_ensemble"], 10, 1, args.tol)
 
     other = None
     if not args.no_other:
@@ -568,13 +653,16 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        links, e, psi = c.generate(nmodes=args.cpu_modes)
-        v, secs = cpu_baseline(c, links, e, psi, args.cpu_threads, args.cpu_modes)
-        cpu = {"value": v, "unit": "site*mode updates/s", "cores": args.cpu_threads, "kind": "port",
-               "sample": f"{args.cpu_modes} modes x 1 full step of {c.name} ({secs:.1f} s), "
-                         f"oracle/liboracle.so FP64 C restatement, OpenMP over modes",
-               "host": platform.processor() or platform.machine()}
-        log(f"cpu baseline {v:.4e} site*mode/s in {secs:.1f}s")
+        threads = args.cpu_threads or default_cpu_threads()
+        nm = args.cpu_modes or 2 * threads
+        links, e, psi = c.generate(nmodes=nm)
+        v, secs = cpu_baseline(c, links, e, psi, threads, nm)
+        cpu = {"value": v, "unit": "site*mode updates/s", "cores": threads, "kind": "port",
+               "sample": f"{nm} modes x 1 full step of {c.name} ({secs:.1f} s; two modes per thread), "
+                         f"oracle/liboracle.so FP64 C restatement (the reference has no SU(3) path), "
+                         f"OpenMP over modes, {threads} threads",
+               "host": host_description()}
+        log(f"cpu baseline {v:.4e} site*mode/s in {secs:.1f}s on {threads} threads")
 
     out = {"metric": METRIC, "value": value, "unit": "site*mode updates/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
@@ -585,7 +673,8 @@ def main():
                       "traversal": args.traversal, "parallelism": "single GPU"},
            "roofline": roofline, "cpu_baseline": cpu, "fp64_gpu": fp64,
            "speedup_vs_fp64": (value / fp64["value"]) if fp64 else None,
-           "graph_ms_per_step": graph_ms, "eager": eager, "timing": timing, "accuracy": res, "observables": observables, "ensemble": ens,
+           "graph_ms_per_step": graph_ms, "eager": eager, "timing": timing, "accuracy": res, "guard": guard,
+           "observables": observables, "ensemble": ens,
            "other_workloads": other, "next_rows": nxt}
     print(json.dumps(out))
 

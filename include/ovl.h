@@ -81,11 +81,25 @@ typedef struct {
     int fp48;            /* MIXED: 1 = store the FP64 steps whose error bound allows it in 48 bits (the
                             top 48 bits of the double, rounded; FP64 arithmetic), 0 = plain FP64 */
     int head48;          /* MIXED with fp48: first 48-bit step (> 0 overrides tol; 0 = from tol) */
+    int guard;           /* MIXED: a-posteriori accuracy guard (0 = off).  A check takes one mode
+                            block of the input (Psi for the leapfrog; a different block each time)
+                            through the mixed sign and through the FP64 sign; while a mode's relative
+                            difference exceeds tol, the ladder is tightened a level (head overrides
+                            dropped, then tol divided by 16 per level, at last plain FP64) and
+                            checked again.  A check runs before the first step of an update call (or
+                            before an apply) when the links or these ladder options changed since
+                            the last check, or when at least `guard` leapfrog steps (applies) went
+                            by.  The tightening is kept by the context until new links are set.
+                            Default 100. */
 } ovl_sign_opts;
 
 typedef struct {
     int kind;            /* OVL_WILSON or OVL_OVERLAP */
     double dt;           /* full leapfrog step */
+    /* sign.precision also selects the spinor arithmetic of OVL_WILSON: OVL_PREC_FP64 steps the FP64
+     * fields, OVL_PREC_FP32 and OVL_PREC_MIXED step FP32 copies of Psi and PsiMid (converted when
+     * the call starts, written back to the FP64 fields when it ends); links, E and every
+     * observable stay FP64 */
     int gauge;           /* OVL_GAUGE_FROZEN / _LINKS (U <- exp(-i dt/2 E) U, E fixed) / _FULL */
     ovl_sign_opts sign;
     int use_graph;       /* capture one full step into a hipGraph and replay it; the context keeps
@@ -271,6 +285,10 @@ typedef struct {
     double var_bytes[5];
     int graph_captures;       /* hipGraph captures the call made (a graph-mode ovl_fermion_update
                                  whose step is unchanged since the previous one replays that graph: 0) */
+    int guard_checks;         /* accuracy-guard checks the call made (each: one mode block, mixed and
+                                 FP64 sign; their kernels are not counted above) */
+    int guard_level;          /* tightening of the mixed ladder after the call (0 = as requested) */
+    double guard_err;         /* largest relative mixed-vs-FP64 difference of the last check */
 } ovl_timing;
 int ovl_get_timing(ovl_ctx *ctx, ovl_timing *t);
 int ovl_set_profiling(ovl_ctx *ctx, int on);

@@ -70,6 +70,19 @@ struct ovl_ctx {
     bool stencil_custom = false;  // the stencil links hold a special operator (QED eigenbasis)
 
     void *field[3] = {nullptr, nullptr, nullptr};  // Psi, PsiMid, scratch (FP64)
+    // FP32 copies of Psi and PsiMid that the FP32 Wilson leapfrog steps in (allocated on first use;
+    // filled from the FP64 fields when a call starts and written back when it ends)
+    void *f32[2] = {nullptr, nullptr};
+
+    // a-posteriori accuracy guard of the mixed sign: tightening level of the ladder (kept until new
+    // links are set), the next mode block to sample, two FP64 one-block results, steps (or applies)
+    // since the last check, whether the links or the ladder options changed since it (guard_opts:
+    // the options it checked), the checks of the current call and the largest difference of the last
+    int guard_level = 0, guard_next = 0, guard_since = 0, guard_checks = 0;
+    bool guard_fresh = true;
+    ovl_sign_opts guard_opts{};
+    double guard_err = 0.0;
+    void *guard_buf[2] = {nullptr, nullptr};
 
     // Clenshaw work buffers (sized for `work_blocks` mode blocks)
     int work_blocks = 0;
@@ -468,6 +481,9 @@ int end_timing(ovl_ctx *c)
     c->last.fp16_steps = c->fp16_steps;
     c->last.fp48_steps = c->fp48_steps;
     c->last.graph_captures = c->captures;
+    c->last.guard_checks = c->guard_checks;
+    c->last.guard_level = c->guard_level;
+    c->last.guard_err = c->guard_err;
     return OVL_OK;
 }
 
@@ -753,19 +769,22 @@ enum FinalKind { FIN_SIGN = 0, FIN_OVERLAP = 1, FIN_OVERLAP_AXPY = 2 };
 
 // xmax: bound of the 2-norm of every input mode (used by the FP16 tier only)
 // hpsi (FIN_OVERLAP_AXPY only, optional): also store -i h_ov in (the term added to out times dt)
+// blocks [blk0, blk0 + nblk) of the fields (nblk < 0: all from blk0 on); out_rel: out64 is a buffer
+// whose first block receives block blk0
 int sign_driver(ovl_ctx *c, const ovl_sign_opts *o, const void *in64, void *out64, int kind, double dt,
-                double xmax, void *hpsi = nullptr)
+                double xmax, void *hpsi = nullptr, int blk0 = 0, int nblk = -1, bool out_rel = false)
 {
     RETCHK(ensure_stencil(c));
     SignPlan pl;
     RETCHK(make_plan(c, o, pl, xmax));
-    const int chunk = (o->mode_chunk > 0) ? std::min(o->mode_chunk, c->g.nblk) : c->g.nblk;
+    const int bend = (nblk < 0) ? c->g.nblk : std::min(c->g.nblk, blk0 + nblk);
+    const int chunk = (o->mode_chunk > 0) ? std::min(o->mode_chunk, bend - blk0) : bend - blk0;
     const double M = c->prm.m_dw, m = c->prm.m_fermion;
-    for (int b0 = 0; b0 < c->g.nblk; b0 += chunk) {
+    for (int b0 = blk0; b0 < bend; b0 += chunk) {
         Geom gc = c->g;
-        gc.nblk = std::min(chunk, c->g.nblk - b0);
+        gc.nblk = std::min(chunk, bend - b0);
         const void *x = blk_ptr((void *)in64, P64, b0, c);
-        void *y = blk_ptr(out64, P64, b0, c);
+        void *y = blk_ptr(out64, P64, out_rel ? b0 - blk0 : b0, c);
         void *p = nullptr;
         int pp = P64;
         RETCHK(clenshaw_chunk(c, gc, pl, x, &p, &pp));
@@ -920,6 +939,111 @@ int input_bound(ovl_ctx *c, const ovl_sign_opts *o, const void *f, double *xmax)
     *xmax = std::sqrt(m);
     if (!(*xmax > 0.0)) *xmax = 1.0;  // all-zero input: any scale works
     return OVL_OK;
+}
+
+// ---- a-posteriori accuracy guard of the mixed sign ----
+bool guarded(const ovl_sign_opts *o)
+{
+    return o->method == OVL_SIGN_CHEBYSHEV && o->precision == OVL_PREC_MIXED && o->guard > 0;
+}
+
+// a check is due when the links or the options that set the ladder changed since the last check, or
+// at least `guard` steps (applies) went by
+bool guard_due(const ovl_ctx *c, const ovl_sign_opts *o)
+{
+    if (!guarded(o)) return false;
+    const ovl_sign_opts &g = c->guard_opts;
+    const bool same = g.degree == o->degree && g.cut == o->cut && g.tol == o->tol && g.head == o->head &&
+                      g.fp16_tail == o->fp16_tail && g.head16 == o->head16 && g.fp48 == o->fp48 &&
+                      g.head48 == o->head48 && g.precision == o->precision && g.method == o->method;
+    return c->guard_fresh || !same || c->guard_since >= o->guard;
+}
+
+// the options a guarded mixed sign runs with at the context's tightening level: level 0 as
+// requested; level >= 1 without head overrides (the ladder follows tol); level l >= 2 planned for
+// tol / 16^(l - 1) (each tier starts about 26 steps earlier at the geometric decay of the Clenshaw
+// intermediates); the last level is plain FP64
+constexpr int GUARD_LEVELS = 6;
+ovl_sign_opts guarded_opts(const ovl_ctx *c, const ovl_sign_opts *o)
+{
+    ovl_sign_opts e = *o;
+    if (!guarded(o) || c->guard_level == 0) return e;
+    e.head = -1;
+    e.head16 = 0;
+    e.head48 = 0;
+    const double tol = o->tol > 0 ? o->tol : 1e-10;
+    e.tol = tol * std::pow(16.0, -(double)(c->guard_level - 1));
+    if (c->guard_level >= GUARD_LEVELS) e.precision = OVL_PREC_FP64;
+    return e;
+}
+
+// One mode block of in64 (the next in turn) through the mixed sign and through the FP64 sign: while
+// some mode's relative difference exceeds tol, the ladder is tightened a level and checked again.
+// *eff receives the options to run with.  The check's kernels stay out of the call's stencil
+// counters and events; its time is part of the call.  A non-finite FP64 result is an error (the
+// field itself has overflowed); a non-finite mixed one (FP16 range) is a failed check.
+int guard_check(ovl_ctx *c, const ovl_sign_opts *o, const void *in64, double xmax, ovl_sign_opts *eff)
+{
+    *eff = guarded_opts(c, o);
+    if (!guarded(o)) return OVL_OK;
+    const int blk = c->guard_next++ % c->g.nblk;
+    Geom g1 = c->g;
+    g1.nblk = 1;
+    for (int f = 0; f < 2; ++f)
+        if (!c->guard_buf[f]) RETCHK(dalloc(c, &c->guard_buf[f], c->blk64_elems * 16));
+    const long long hl = c->hop_launches;
+    const double hb = c->hop_bytes;
+    long long vl[NVAR];
+    double vb[NVAR];
+    for (int v = 0; v < NVAR; ++v) { vl[v] = c->var_launches[v]; vb[v] = c->var_bytes[v]; }
+    const int st[4] = {c->fp64_steps, c->fp32_steps, c->fp16_steps, c->fp48_steps};
+    const bool prof = c->profiling;
+    c->profiling = false;
+    const double tol = o->tol > 0 ? o->tol : 1e-10;
+    int r = OVL_OK;
+    for (;;) {
+        *eff = guarded_opts(c, o);
+        if (eff->precision != OVL_PREC_MIXED) break;  // plain FP64: nothing left to compare
+        ovl_sign_opts o64 = *eff;
+        o64.precision = OVL_PREC_FP64;
+        r = sign_driver(c, eff, in64, c->guard_buf[0], FIN_SIGN, 0.0, xmax, nullptr, blk, 1, true);
+        if (r == OVL_OK) r = sign_driver(c, &o64, in64, c->guard_buf[1], FIN_SIGN, 0.0, xmax, nullptr, blk, 1, true);
+        if (r == OVL_OK) {
+            const hipError_t e = ovl::launch_lincomb(g1, 1, P64, c->guard_buf[0], c->guard_buf[0], 1.0, 0.0,
+                                                     c->guard_buf[1], -1.0, 0.0, c->st);
+            if (e != hipSuccess) r = fail(c, OVL_E_HIP, std::string("guard: ") + hipGetErrorString(e));
+        }
+        std::vector<cplx> dd, yy;
+        if (r == OVL_OK) r = mode_dot(c, g1, c->guard_buf[0], c->guard_buf[0], dd);
+        if (r == OVL_OK) r = mode_dot(c, g1, c->guard_buf[1], c->guard_buf[1], yy);
+        if (r != OVL_OK) break;
+        double worst = 0.0;
+        bool ok = true;
+        for (size_t i = 0; i < dd.size() && r == OVL_OK; ++i) {
+            if (!std::isfinite(yy[i].real())) r = fail(c, OVL_E_ARG, "non-finite mode function (the leapfrog diverged)");
+            else if (!std::isfinite(dd[i].real())) ok = false;
+            else if (yy[i].real() > 0) worst = std::max(worst, std::sqrt(dd[i].real() / yy[i].real()));
+        }
+        if (r != OVL_OK) break;
+        c->guard_checks++;
+        c->guard_err = ok ? worst : INFINITY;
+        if (ok && worst <= tol) break;
+        c->guard_level++;
+    }
+    c->profiling = prof;
+    if (r == OVL_OK) {
+        c->guard_fresh = false;
+        c->guard_since = 0;
+        c->guard_opts = *o;
+    }
+    c->hop_launches = hl;
+    c->hop_bytes = hb;
+    for (int v = 0; v < NVAR; ++v) { c->var_launches[v] = vl[v]; c->var_bytes[v] = vb[v]; }
+    c->fp64_steps = st[0];
+    c->fp32_steps = st[1];
+    c->fp16_steps = st[2];
+    c->fp48_steps = st[3];
+    return r;
 }
 
 // (h_W^2 + d0) v  ->  q   via two stencil launches, FP64
@@ -1079,6 +1203,7 @@ void ovl_sign_defaults(ovl_sign_opts *o)
     o->head16 = 0;
     o->fp48 = 1;
     o->head48 = 0;
+    o->guard = 100;
 }
 
 int ovl_sign_plan(const ovl_sign_opts *o, int *head48, int *head, int *head16)
@@ -1301,7 +1426,8 @@ int ovl_destroy(ovl_ctx *c)
                   c->b64, c->x32, c->w32, c->a32, c->b32, c->partial, c->dout, c->gam, c->jv, c->ja, c->staging,
                   c->ms_ps, c->ms_r, c->ms_q, c->ms_w, c->ms_chi, c->ms_coef, c->ms_active, c->rz[0], c->rz[1],
                   c->rz[2], c->rz[3], c->rz[4], c->rz[5], c->d_sys, c->d_work, c->order[0], c->order[1],
-                  c->order[2], c->order[3], c->x16, c->w16, c->a16, c->b16, c->x48, c->w48, c->a48, c->b48};
+                  c->order[2], c->order[3], c->x16, c->w16, c->a16, c->b16, c->x48, c->w48, c->a48, c->b48,
+                  c->f32[0], c->f32[1], c->guard_buf[0], c->guard_buf[1]};
     if (c->step_exec) hipGraphExecDestroy(c->step_exec);
     for (void *p : ps)
         if (p) hipFree(p);
@@ -1357,6 +1483,10 @@ static int get_doubles(ovl_ctx *c, double *dst, const double *src, size_t n)
 
 int ovl_set_links(ovl_ctx *c, const double *links)
 {
+    if (c) {  // a new gauge configuration: the guard starts again from the requested ladder
+        c->guard_level = 0;
+        c->guard_fresh = true;
+    }
     if (!c || !links) return OVL_E_ARG;
     RETCHK(put_doubles(c, c->links, links, (size_t)link_reals(c) * 3 * c->link_sites, "gauge link"));
     std::fill(c->sys_links_set.begin(), c->sys_links_set.end(), 1);
@@ -1388,6 +1518,8 @@ int ovl_get_efield(ovl_ctx *c, double *e)
 int ovl_set_system_links(ovl_ctx *c, int sys, const double *links)
 {
     if (!c || !links) return OVL_E_ARG;
+    c->guard_level = 0;
+    c->guard_fresh = true;
     RETCHK(check_sys(c, sys));
     RETCHK(put_doubles(c, sys_links(c, sys), links, (size_t)link_reals(c) * 3 * c->sys[sys].V, "gauge link"));
     c->sys_links_set[sys] = 1;
@@ -1574,8 +1706,12 @@ int ovl_apply_overlap(ovl_ctx *c, int in, int out, const ovl_sign_opts *o)
     }
     double xmax;
     RETCHK(input_bound(c, o, c->field[in], &xmax));
+    c->guard_checks = 0;
     begin_timing(c);
-    RETCHK(sign_driver(c, o, c->field[in], c->field[out], FIN_OVERLAP, 0.0, xmax));
+    ovl_sign_opts e = guarded_opts(c, o);
+    if (guard_due(c, o)) RETCHK(guard_check(c, o, c->field[in], xmax, &e));
+    RETCHK(sign_driver(c, &e, c->field[in], c->field[out], FIN_OVERLAP, 0.0, xmax));
+    c->guard_since++;
     return end_timing(c);
 }
 
@@ -1589,8 +1725,12 @@ int ovl_apply_sign(ovl_ctx *c, int in, int out, const ovl_sign_opts *o)
     if (in == out) return fail(c, OVL_E_ARG, "in and out must be different fields");
     double xmax;
     RETCHK(input_bound(c, o, c->field[in], &xmax));
+    c->guard_checks = 0;
     begin_timing(c);
-    RETCHK(sign_driver(c, o, c->field[in], c->field[out], FIN_SIGN, 0.0, xmax));
+    ovl_sign_opts e = guarded_opts(c, o);
+    if (guard_due(c, o)) RETCHK(guard_check(c, o, c->field[in], xmax, &e));
+    RETCHK(sign_driver(c, &e, c->field[in], c->field[out], FIN_SIGN, 0.0, xmax));
+    c->guard_since++;
     return end_timing(c);
 }
 
@@ -1890,17 +2030,23 @@ static int gauge_half(ovl_ctx *c, int mode, double h)
     return OVL_OK;
 }
 
+// the Wilson leapfrog steps FP32 spinor fields unless the options ask for FP64
+static bool wilson_fp32(const ovl_step_opts *o) { return o->kind == OVL_WILSON && o->sign.precision != OVL_PREC_FP64; }
+
 // target += dt * (-i h src)   (OverlapUpdate.cpp:85-124, WILSON/Update.cpp:6-45)
 static int fermion_half(ovl_ctx *c, const ovl_step_opts *o, int src, int tgt, void *hpsi = nullptr)
 {
     if (o->kind == OVL_WILSON) {
-        HopArgs a = hop_args(c, P64);
-        a.in = c->field[src];
-        a.out = c->field[tgt];
-        a.tgt = c->field[tgt];
+        // one stencil launch with the axpy in its epilogue: FP64 fields, or the FP32 copies (3S + L/M
+        // bytes per site-mode at S = 96 B for SU(3) instead of 192)
+        const int pr = wilson_fp32(o) ? P32 : P64;
+        HopArgs a = hop_args(c, pr);
+        a.in = pr == P32 ? c->f32[src] : c->field[src];
+        a.out = pr == P32 ? c->f32[tgt] : c->field[tgt];
+        a.tgt = a.out;
         a.st_r = 1.0;
         a.sh_i = -o->dt;
-        return hop(c, c->g, P64, P64, a);
+        return hop(c, c->g, pr, pr, a);
     }
     if (o->sign.method == OVL_SIGN_ZOLOTAREV)
         return zolo_overlap(c, c->g, c->field[src], c->field[tgt], o->sign.lmin, o->sign.lmax, 1, o->dt);
@@ -1916,7 +2062,7 @@ static bool same_step_opts(const ovl_step_opts &a, const ovl_step_opts &b)
            x.method == y.method && x.precision == y.precision && x.degree == y.degree && x.cut == y.cut &&
            x.tol == y.tol && x.head == y.head && x.lmin == y.lmin && x.lmax == y.lmax &&
            x.mode_chunk == y.mode_chunk && x.fp16_tail == y.fp16_tail && x.head16 == y.head16 &&
-           x.fp48 == y.fp48 && x.head48 == y.head48;
+           x.fp48 == y.fp48 && x.head48 == y.head48 && x.guard == y.guard;
 }
 
 static StepKey make_step_key(const ovl_ctx *c, const ovl_step_opts *o, const SignPlan *pl)
@@ -1934,7 +2080,7 @@ static StepKey make_step_key(const ovl_ctx *c, const ovl_step_opts *o, const Sig
     k.ptrs = {c->links, c->efield, c->ul64, c->ul32, c->phase, c->field[0], c->field[1], c->field[2],
               c->w64, c->a64, c->b64, c->x32, c->w32, c->a32, c->b32, c->x16, c->w16, c->a16, c->b16,
               c->x48, c->w48, c->a48, c->b48, c->partial, c->d_sys, c->d_work, c->order[0], c->order[1],
-              c->order[2], c->order[3]};
+              c->order[2], c->order[3], c->f32[0], c->f32[1]};
     k.ints = {c->work_blocks, c->ens_grid};
     for (int nb : c->sys_nb) k.ints.push_back(nb != 0);  // systems whose links take the phases
     for (int p = 0; p < NUM_PREC; ++p)
@@ -1972,6 +2118,8 @@ static int update_impl(ovl_ctx *c, const ovl_step_opts *o, int nsteps, bool obse
 
 int ovl_fermion_update(ovl_ctx *c, const ovl_step_opts *o, int nsteps)
 {
+    if (!c || !o || nsteps < 0) return OVL_E_ARG;
+    c->guard_checks = 0;
     return update_impl(c, o, nsteps, false);
 }
 
@@ -1985,6 +2133,8 @@ static int update_impl(ovl_ctx *c, const ovl_step_opts *o, int nsteps, bool obse
     if (o->kind == OVL_OVERLAP) RETCHK(check_sign_opts(c, &o->sign));
     c->step_xmax = 0.0;
     SignPlan pl;
+    const ovl_step_opts *const req = o;  // the options as requested (o becomes the guarded ones)
+    ovl_step_opts oe = *o;
     const bool cheb = o->kind == OVL_OVERLAP && o->sign.method == OVL_SIGN_CHEBYSHEV;
     if (cheb) {
         // bound of the mode norms for the FP16 scales: the staggered leapfrog conserves the norms
@@ -1998,20 +2148,42 @@ static int update_impl(ovl_ctx *c, const ovl_step_opts *o, int nsteps, bool obse
         int ex = 0;
         const double mant = std::frexp(bnd, &ex);  // bnd = mant 2^ex, mant in [1/2, 1)
         c->step_xmax = (bnd > 0.0 && std::isfinite(bnd)) ? std::ldexp(1.0, mant == 0.5 ? ex - 1 : ex) : bnd;
+        // when a check is due (new links or ladder options, or `guard` steps since the last) the
+        // guard takes Psi's next mode block through both signs before the first step; the steps
+        // run with the ladder as tightened
+        oe.sign = guarded_opts(c, &o->sign);
+        if (guard_due(c, &o->sign)) RETCHK(guard_check(c, &o->sign, c->field[OVL_PSI], c->step_xmax, &oe.sign));
+        o = &oe;
         RETCHK(make_plan(c, &o->sign, pl, c->step_xmax));
     }
     RETCHK(ensure_stencil(c));
+    const bool w32 = wilson_fp32(o) && nsteps > 0;
+    if (w32)
+        for (int f = 0; f < 2; ++f)
+            if (!c->f32[f]) RETCHK(dalloc(c, &c->f32[f], c->field32_bytes));
     begin_timing(c);
     if (observe_last && nsteps > 0) {
-        RETCHK(update_impl(c, o, nsteps - 1, false));
+        RETCHK(update_impl(c, req, nsteps - 1, false));
         begin_timing(c);
         RETCHK(one_step(c, o, true));
+        c->guard_since += 1;
         return end_timing(c);
     }
+    // FP32 Wilson: the call steps the FP32 copies of Psi and PsiMid and writes them back at the end
+    if (w32)
+        for (int f = 0; f < 2; ++f)
+            HIPCHK(c, ovl::launch_convert(c->g, c->g.nblk, c->field[f], P64, c->f32[f], P32, c->st));
+    auto finish = [&]() -> int {
+        c->guard_since += nsteps;
+        if (w32)
+            for (int f = 0; f < 2; ++f)
+                HIPCHK(c, ovl::launch_convert(c->g, c->g.nblk, c->f32[f], P32, c->field[f], P64, c->st));
+        return end_timing(c);
+    };
     const bool graph = o->use_graph && !(o->kind == OVL_OVERLAP && o->sign.method == OVL_SIGN_ZOLOTAREV) && nsteps > 0;
     if (!graph) {
         for (int s = 0; s < nsteps; ++s) RETCHK(one_step(c, o));
-        return end_timing(c);
+        return finish();
     }
     // one full step (both half steps and both gauge half steps) as a graph, replayed nsteps times;
     // every buffer the captured step touches is allocated before the capture.  The instantiated
@@ -2089,7 +2261,7 @@ static int update_impl(ovl_ctx *c, const ovl_step_opts *o, int nsteps, bool obse
     c->fp32_steps += c->step_dp[1] * nsteps;
     c->fp16_steps += c->step_dp[2] * nsteps;
     c->fp48_steps += c->step_dp[3] * nsteps;
-    return end_timing(c);
+    return finish();
 }
 
 // ---- observables ----
@@ -2323,6 +2495,9 @@ int ovl_fermion_update_observed(ovl_ctx *c, const ovl_step_opts *o, int nsteps, 
     if (obs->every <= 0) return ovl_fermion_update(c, o, nsteps);
     if (o->kind != OVL_OVERLAP || o->sign.method != OVL_SIGN_CHEBYSHEV)
         return fail(c, OVL_E_ARG, "observed updates take the currents from the Chebyshev overlap apply");
+    // the step keeps -i h_ov Psi as (dt term) / dt: dt = 0 leaves nothing to recover it from
+    if (!(o->dt != 0.0)) return fail(c, OVL_E_ARG, "observed updates need dt != 0 (the currents reuse the step's dt-scaled overlap term)");
+    c->guard_checks = 0;
     int done = 0;
     while (done < nsteps) {
         const int chunk = std::min(obs->every, nsteps - done);
@@ -2419,6 +2594,8 @@ static int read_link_file(ovl_ctx *c, const SysDesc &d, const char *path, int nv
 int ovl_load_configuration(ovl_ctx *c, int sys, const char *ufile, const char *efile)
 {
     if (!c || !ufile) return OVL_E_ARG;
+    c->guard_level = 0;
+    c->guard_fresh = true;
     RETCHK(check_sys(c, sys));
     const SysDesc d = c->sys[sys];
     std::vector<double> u, e;
@@ -2447,11 +2624,11 @@ int ovl_qed_eigenbasis(ovl_ctx *c, const ovl_sign_opts *o, int nb, double q, dou
     if (Dz > 16384) return fail(c, OVL_E_ARG, "4 n0 n1 above 16384: the xy blocks of the eigenproblem are too large");
     if ((double)n2 * Dz * Dz * 16.0 > 32e9) return fail(c, OVL_E_ARG, "the n2 blocks of 4 n0 n1 squared exceed 32 GB");
     const int nm = c->prm.n_modes;
-    // every requested mode must name an eigenvector
-    for (int s = 0; s < nm; ++s) {
-        const long g = (long)first_mode + s, level = (g / 2) % (2L * V * nc);
-        if (level >= D / 2) return fail(c, OVL_E_ARG, "mode index beyond the D/2 levels of the eigenbasis");
-    }
+    // every requested mode must name an eigenvector: global mode g = particle + 2 level with
+    // level < 2 Nc V = D/2 (GetLocalModeQuantumNumbers, FermionField.cpp:63-74, which reports an
+    // indexing error when g / 2 carries past the last level)
+    if ((long)first_mode + nm > D)
+        return fail(c, OVL_E_ARG, "mode index beyond the D/2 levels of the eigenbasis (first_mode + n_modes > 4 Nc V)");
     // U(1)-only stencil links (ConvertU1GaugeLinks, OverlapQEDEigenfunctions.cpp:20-48): colour (0,0)
     // carries the phase of QED::SetConstantMagneticField (QED.cpp:136-143), every other entry is 0
     std::vector<double> ucc((size_t)3 * V * nc * nc * 2, 0.0);

@@ -928,31 +928,53 @@ __device__ __forceinline__ void store_link3(double *links, int s, int mu, const 
         }
 }
 
-// project onto SU(3): Gram-Schmidt of rows 0,1, row 2 = conj(row0 x row1)
-__device__ Mat<3> reunit3(Mat<3> m)
+// project onto SU(3): the unitary polar factor P = V (V^+ V)^{-1/2} (the U(3) matrix closest to V,
+// of largest Re Tr(P^+ V)) by the Newton iteration X <- (X + X^{-+})/2, quadratically convergent
+// from any nonsingular V (Higham 1986; six steps take a drift of 1e-3 far below rounding), then
+// P det(P)^{-1/3} with the principal cube root so that det = 1.  The reference's SU(3)
+// MaxTraceProjection (SU3/GroupOperations.cpp:510-533, Cabibbo-Marinari sweeps over SU(3)) agrees
+// to second order in the distance of V from the group.
+__device__ __forceinline__ cd cdiv(cd a, cd b)
 {
-    double n0 = 0;
-    for (int j = 0; j < 3; ++j) n0 += m.e[j].r * m.e[j].r + m.e[j].i * m.e[j].i;
-    n0 = 1.0 / sqrt(n0);
-    for (int j = 0; j < 3; ++j) { m.e[j].r *= n0; m.e[j].i *= n0; }
-    cd d = {0, 0};
-    for (int j = 0; j < 3; ++j) d = cadd(d, cmulc(m.e[3 + j], m.e[j]));  // <r0, r1> = sum r1 conj(r0)
-    for (int j = 0; j < 3; ++j) {
-        cd t = cmul(d, m.e[j]);
-        m.e[3 + j].r -= t.r;
-        m.e[3 + j].i -= t.i;
+    const double d = 1.0 / (b.r * b.r + b.i * b.i);
+    return {(a.r * b.r + a.i * b.i) * d, (a.i * b.r - a.r * b.i) * d};
+}
+__device__ __forceinline__ cd csub(cd a, cd b) { return {a.r - b.r, a.i - b.i}; }
+
+__device__ Mat<3> reunit3(Mat<3> x)
+{
+    for (int it = 0; it < 6; ++it) {
+        const cd *a = x.e;
+        const cd c00 = csub(cmul(a[4], a[8]), cmul(a[5], a[7]));
+        const cd c01 = csub(cmul(a[5], a[6]), cmul(a[3], a[8]));
+        const cd c02 = csub(cmul(a[3], a[7]), cmul(a[4], a[6]));
+        const cd det = cadd(cadd(cmul(a[0], c00), cmul(a[1], c01)), cmul(a[2], c02));
+        Mat<3> inv;  // row-major inverse: adjugate / det
+        inv.e[0] = cdiv(c00, det);
+        inv.e[3] = cdiv(c01, det);
+        inv.e[6] = cdiv(c02, det);
+        inv.e[1] = cdiv(csub(cmul(a[2], a[7]), cmul(a[1], a[8])), det);
+        inv.e[4] = cdiv(csub(cmul(a[0], a[8]), cmul(a[2], a[6])), det);
+        inv.e[7] = cdiv(csub(cmul(a[1], a[6]), cmul(a[0], a[7])), det);
+        inv.e[2] = cdiv(csub(cmul(a[1], a[5]), cmul(a[2], a[4])), det);
+        inv.e[5] = cdiv(csub(cmul(a[2], a[3]), cmul(a[0], a[5])), det);
+        inv.e[8] = cdiv(csub(cmul(a[0], a[4]), cmul(a[1], a[3])), det);
+        Mat<3> nx;
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {  // (X^{-+})_{ij} = conj((X^{-1})_{ji})
+                const cd v = x.e[i * 3 + j], w = inv.e[j * 3 + i];
+                nx.e[i * 3 + j] = {0.5 * (v.r + w.r), 0.5 * (v.i - w.i)};
+            }
+        x = nx;
     }
-    double n1 = 0;
-    for (int j = 0; j < 3; ++j) n1 += m.e[3 + j].r * m.e[3 + j].r + m.e[3 + j].i * m.e[3 + j].i;
-    n1 = 1.0 / sqrt(n1);
-    for (int j = 0; j < 3; ++j) { m.e[3 + j].r *= n1; m.e[3 + j].i *= n1; }
-    // r2 = conj(r0 x r1)
-    for (int j = 0; j < 3; ++j) {
-        const int a = (j + 1) % 3, b = (j + 2) % 3;
-        cd c = cadd(cmul(m.e[a], m.e[3 + b]), {-cmul(m.e[b], m.e[3 + a]).r, -cmul(m.e[b], m.e[3 + a]).i});
-        m.e[6 + j] = {c.r, -c.i};
-    }
-    return m;
+    const cd *a = x.e;
+    const cd det = cadd(cadd(cmul(a[0], csub(cmul(a[4], a[8]), cmul(a[5], a[7]))),
+                             cmul(a[1], csub(cmul(a[5], a[6]), cmul(a[3], a[8])))),
+                        cmul(a[2], csub(cmul(a[3], a[7]), cmul(a[4], a[6]))));
+    const double ph = -atan2(det.i, det.r) / 3.0;
+    const cd f = {cos(ph), sin(ph)};
+    for (int k = 0; k < 9; ++k) x.e[k] = cmul(x.e[k], f);
+    return x;
 }
 
 // stencil links from master links and U(1) phases (ConvertGaugeLinks,

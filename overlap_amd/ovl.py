@@ -54,7 +54,7 @@ class SignOpts(C.Structure):
     _fields_ = [("method", C.c_int), ("precision", C.c_int), ("degree", C.c_int), ("cut", C.c_double),
                 ("tol", C.c_double), ("head", C.c_int), ("lmin", C.c_double), ("lmax", C.c_double),
                 ("mode_chunk", C.c_int), ("fp16_tail", C.c_int), ("head16", C.c_int),
-                ("fp48", C.c_int), ("head48", C.c_int)]
+                ("fp48", C.c_int), ("head48", C.c_int), ("guard", C.c_int)]
 
 
 class StepOpts(C.Structure):
@@ -76,7 +76,8 @@ class Timing(C.Structure):
     _fields_ = [("total_ms", C.c_double), ("hop_ms", C.c_double), ("hop_launches", C.c_longlong),
                 ("hop_bytes", C.c_double), ("fp64_steps", C.c_int), ("fp32_steps", C.c_int),
                 ("fp16_steps", C.c_int), ("fp48_steps", C.c_int), ("var_ms", C.c_double * 5),
-                ("var_launches", C.c_longlong * 5), ("var_bytes", C.c_double * 5), ("graph_captures", C.c_int)]
+                ("var_launches", C.c_longlong * 5), ("var_bytes", C.c_double * 5), ("graph_captures", C.c_int),
+                ("guard_checks", C.c_int), ("guard_level", C.c_int), ("guard_err", C.c_double)]
 
 
 _lib = None

@@ -229,3 +229,71 @@ def test_config3_su3_overlap_24_ten_steps(gpu):
     print(f"  mixed vs fp64 after {steps} steps over 256 modes: {err:.2e}", flush=True)
     assert err < 1e-10
     assert np.abs(i64 - 1).max() < 1e-12 and np.abs(imx - 1).max() < 1e-10
+
+
+def test_config2_su3_wilson_32_fp32_all_200_steps(gpu):
+    # the workload as specified: FP32 spinor arithmetic (FP32 copies of Psi and PsiMid stepped by the
+    # FP32 stencil with the axpy in its epilogue), links and observables FP64.  All 512 modes after
+    # 200 steps against the FP64 run of the same inputs (itself pinned to the oracle above), the
+    # FP32 rounding allowed 2e-5 relative per mode; two modes over 3 steps against the oracle; the
+    # leapfrog invariant to FP32 accuracy.
+    c = cf.CONFIGS["su3_wilson_32"]
+    print(f"[case] {c.name} fp32 {c.steps} steps", flush=True)
+    ctx, links, e, psi = setup(c)
+    ctx.fermion_update(step_opts(c, ovl.PREC_FP32), 3)
+    sample = [0, 511]
+    lk, p, pm, _ = oracle_steps(c, links, e, psi[sample], 3)
+    for i, s in enumerate(sample):
+        assert rel(ctx.get_modes(ovl.PSI, s, 1)[0], p[i]) < 2e-5
+        assert rel(ctx.get_modes(ovl.PSIMID, s, 1)[0], pm[i]) < 2e-5
+    ctx.fermion_update(step_opts(c, ovl.PREC_FP32), c.steps - 3)
+    c64, *_ = setup(c)
+    c64.fermion_update(step_opts(c, ovl.PREC_FP64), c.steps)
+    assert np.array_equal(ctx.get_links(), c64.get_links())
+    worst = 0.0
+    for f in (ovl.PSI, ovl.PSIMID):
+        for m0 in range(0, 512, 64):
+            a, b = ctx.get_modes(f, m0, 64), c64.get_modes(f, m0, 64)
+            worst = max(worst, max(np.linalg.norm(x - y) / np.linalg.norm(y) for x, y in zip(a, b)))
+    inv = stagger_invariant(ctx)
+    print(f"  fp32 vs fp64 after {c.steps} steps over 512 modes: {worst:.2e}; invariant {np.abs(inv - 1).max():.2e}",
+          flush=True)
+    assert worst < 2e-5
+    assert np.abs(inv - 1).max() < 1e-5
+    t = ctx.timing()
+    assert t.var_launches[1] > 0 and t.var_launches[0] == 0  # FP32 stencil only
+
+
+def test_config4_ensemble_all_50_steps(gpu):
+    # the ensemble workload at its full length: 32 SU(2) overlap systems of uneven sizes, 50 leapfrog
+    # steps of the mixed form and of the FP64 form from the same state, every mode of every system
+    # compared (1e-10 relative per mode), and each system's links against the oracle's link update
+    ec = cf.ENSEMBLES["su2_overlap_ensemble"]
+    print(f"[case] {ec.name} {ec.steps} steps", flush=True)
+    sy = [ec.system(k) for k in range(ec.n_systems)]
+    ens = ovl.Ensemble(ec.nc, [s[0] for s in sy], ec.nmodes, m_dw=ec.m_dw)
+    runs = []
+    for prec in (ovl.PREC_MIXED, ovl.PREC_FP64):
+        for k, (n, nb, links, e, psi) in enumerate(sy):
+            ens.set_system_links(k, links)
+            ens.set_system_efield(k, e)
+            ens.set_system_u1_flux(k, nb, ec.q)
+            ens.set_system_modes(k, ovl.PSI, psi)
+            ens.set_system_modes(k, ovl.PSIMID, psi)
+        so = ovl.sign_opts(degree=ec.cheb_n, cut=ec.cheb_cut, precision=prec, tol=1e-10)
+        ens.fermion_update(ovl.StepOpts(ovl.OVERLAP, ec.dt, ovl.GAUGE_LINKS, so, 1), ec.steps)
+        runs.append([(ens.get_system_modes(k, ovl.PSI), ens.get_system_modes(k, ovl.PSIMID))
+                     for k in range(ec.n_systems)])
+    worst = []
+    for mix, ref in zip(*runs):
+        worst.append(max(max(np.linalg.norm(a - b) / np.linalg.norm(b) for a, b in zip(x, y))
+                         for x, y in zip(mix, ref)))
+    print(f"  per-system mixed vs fp64 after {ec.steps} steps: max {max(worst):.2e} min {min(worst):.2e}", flush=True)
+    assert max(worst) < 1e-10
+    for k in (0, ec.n_systems - 1):
+        n, nb, links, e, psi = sy[k]
+        L = orc.lat(ec.nc, n)
+        want = links
+        for _ in range(2 * ec.steps):
+            want = orc.update_links(L, want, e, 0.5 * ec.dt)
+        assert rel(ens.get_system_links(k), want) < 1e-12

@@ -354,6 +354,17 @@ def test_observables_vs_oracle(gpu, nc):
     assert np.allclose(ctx.energy(), orc.plaquette_energy(L, links, e), rtol=1e-13, atol=0)
     g_gpu, g_orc = ctx.gauss_violation(), orc.gauss_violation(L, links, e)
     assert np.allclose(g_gpu, g_orc, rtol=1e-12, atol=0)
+    # unitarity on links pushed off the group (each scaled by its own 1 + O(1e-2)): the oracle's
+    # value is the closed form |lam^2 - 1| (times sqrt 3 for SU(3)), tests/test_oracle.py
+    g = cf.rng(900 + nc)
+    per = 4 if nc == 2 else 18
+    lam = 1.0 + g.normal(0, 0.01, size=3 * int(np.prod(n)))
+    off = (links.reshape(-1, per) * lam[:, None]).reshape(-1)
+    ctx.set_links(off)
+    u_gpu, u_orc = ctx.unitarity_violation(), orc.unitarity_violation(L, off)
+    assert u_orc[0] > 1e-3
+    assert np.allclose(u_gpu, u_orc, rtol=1e-12, atol=0)
+    ctx.set_links(links)
     u_gpu, u_orc = ctx.unitarity_violation(), orc.unitarity_violation(L, links)
     assert abs(u_gpu[0] - u_orc[0]) < 1e-15 and abs(u_gpu[1] - u_orc[1]) < 1e-15
     assert abs(ctx.ncs_dot() - orc.edotb(L, links, e) / (8 * np.pi ** 2)) < 1e-12 * abs(orc.edotb(L, links, e))
@@ -361,6 +372,44 @@ def test_observables_vs_oracle(gpu, nc):
     ctx.update_efield(0.03)
     err = rel(ctx.get_efield(), orc.efield_update(L, links, e, 0.03))
     assert err < 1e-13
+
+
+@pytest.mark.parametrize("nc", [2, 3])
+def test_link_update_reunitarizes(gpu, nc):
+    # reunitarize = 1: every link update ends with the projection onto SU(Nc) (SU(2) V/|V|, SU(3)
+    # polar factor with det 1); from links pushed off the group by 1e-3 the update must equal the
+    # oracle's exponential update followed by its projection (pinned against numpy's SVD polar
+    # factor) and land on the group to rounding
+    print(f"[case] reunitarize nc={nc}", flush=True)
+    n = (4, 3, 5)
+    V = int(np.prod(n))
+    g = cf.rng(910 + nc)
+    links = cf.su2_links(n, 0.5, g) if nc == 2 else cf.su3_links(n, 0.5, g)
+    e = cf.efield(n, nc, 0.4, g)
+    if nc == 2:
+        off = (links.reshape(-1, 4) * (1 + g.normal(0, 1e-3, size=(3 * V, 1)))).reshape(-1)
+    else:
+        U = cf.links_to_matrices(3, links)
+        X = g.normal(size=U.shape) + 1j * g.normal(size=U.shape)
+        cm = np.swapaxes(U @ (np.eye(3) + 1e-3 * X), -1, -2).reshape(V, 3, 9)
+        o = np.empty((V, 3, 18))
+        o[..., 0::2], o[..., 1::2] = cm.real, cm.imag
+        off = o.reshape(-1)
+    L = orc.lat(nc, n)
+    for reunit in (0, 1):
+        ctx = ovl.Context(nc, n, 16, reunitarize=reunit)
+        ctx.set_links(off)
+        ctx.set_efield(e)
+        ctx.update_links(0.05)
+        got = ctx.get_links()
+        want = orc.update_links(L, off, e, 0.05)
+        if reunit:
+            want = orc.reunitarize(L, want)
+            assert ctx.unitarity_violation()[0] < 1e-14
+        else:
+            assert ctx.unitarity_violation()[0] > 1e-4
+        assert rel(got, want) < 1e-13
+        ctx.close()
 
 
 def test_currents_vs_oracle(gpu):
@@ -397,6 +446,15 @@ def test_error_paths(gpu):
     with pytest.raises(ovl.OvlError) as ex:
         ctx.apply_overlap(ovl.PSI, ovl.PSIMID, ovl.sign_opts(method=ovl.SIGN_ZOLOTAREV, lmin=2.0, lmax=1.0))
     assert ex.value.code == ovl.E_SPECTRUM
+    # the eigenbasis has D = 4 Nc V = 512 modes here: modes 510..513 run past it
+    with pytest.raises(ovl.OvlError) as ex:
+        ctx.qed_eigenbasis(ovl.sign_opts(degree=16, cut=0.1), 1, first_mode=510)
+    assert ex.value.code == ovl.E_ARG
+    # observed updates recover -i h_ov Psi from the dt-scaled step term: dt = 0 is refused up front
+    so = ovl.StepOpts(ovl.OVERLAP, 0.0, ovl.GAUGE_FROZEN, ovl.sign_opts(degree=16, cut=0.1), 0)
+    with pytest.raises(ovl.OvlError) as ex:
+        ctx.fermion_update_observed(so, 2, 1)
+    assert ex.value.code == ovl.E_ARG
 
 
 @pytest.mark.parametrize("nc,n,nm,mb", [(3, (4, 4, 4), 64, 64), (2, (8, 4, 6), 32, 16), (3, (6, 4, 4), 64, 32),
@@ -635,3 +693,47 @@ def test_qed_eigenbasis_vs_dense_oracle(gpu, nc, n, nb, nm, mb, first):
     ctx2.set_modes(ovl.PSI, psi2)
     ctx2.apply_wilson(ovl.PSI, ovl.PSIMID)
     assert rel(ctx2.get_modes(ovl.PSIMID)[0], orc.wilson_evo(L2, ucc2, psi2[0])) < 1e-13
+
+
+def test_guard_tightens_an_aggressive_ladder(gpu):
+    # a mixed ladder forced far too low (FP32 from step 6, FP16 from step 12 of degree 160): with the
+    # guard off it misses 1e-10 by orders of magnitude; with it on, the first check of the sampled
+    # mode block fails, the context drops the overrides (ladder from tol), the recheck passes, and the
+    # apply and a leapfrog run end within 1e-10 of FP64 and of the oracle
+    print("[case] guard", flush=True)
+    n, nc, nm = (6, 6, 6), 3, 128
+    ctx, L, ucc, links, e, psi = make(nc, n, nm, 777, sigma=0.6, M=1.2, nb=8, mb=64)
+    deg, cut = 160, 0.08
+    bad = dict(degree=deg, cut=cut, precision=ovl.PREC_MIXED, tol=1e-10, head=6, head16=12, head48=2)
+    ctx.apply_overlap(ovl.PSI, ovl.SCRATCH, ovl.sign_opts(degree=deg, cut=cut))
+    ref = ctx.get_modes(ovl.SCRATCH)
+    ctx.apply_overlap(ovl.PSI, ovl.SCRATCH, ovl.sign_opts(guard=0, **bad))
+    unguarded = max(np.linalg.norm(a - b) / np.linalg.norm(b) for a, b in zip(ctx.get_modes(ovl.SCRATCH), ref))
+    assert unguarded > 1e-8
+    ctx.apply_overlap(ovl.PSI, ovl.SCRATCH, ovl.sign_opts(**bad))
+    t = ctx.timing()
+    guarded = max(np.linalg.norm(a - b) / np.linalg.norm(b) for a, b in zip(ctx.get_modes(ovl.SCRATCH), ref))
+    print(f"  unguarded {unguarded:.2e}, guarded {guarded:.2e}, checks {t.guard_checks}, level {t.guard_level}, "
+          f"last check {t.guard_err:.2e}", flush=True)
+    assert t.guard_checks >= 2 and t.guard_level >= 1 and t.guard_err <= 1e-10
+    assert guarded < 1e-10
+    # a leapfrog run with the same bad options: the context keeps the tightened ladder (no new check
+    # is due: same links, same options, one apply since), and the run matches the oracle's FP64
+    # leapfrog; asked to check every 2 steps, the next call checks again
+    ctx.set_modes(ovl.PSIMID, psi)
+    so = ovl.StepOpts(ovl.OVERLAP, 0.02, ovl.GAUGE_LINKS, ovl.sign_opts(**bad), 1)
+    ctx.fermion_update(so, 3)
+    t = ctx.timing()
+    assert t.guard_level >= 1 and t.guard_checks == 0
+    o = orc.StepOpts(orc.OVERLAP_CHEB, 0.02, 1, 8, 1.0, deg, cut, 0.0, 0.0, 8)
+    _, p, pm, _ = orc.leapfrog(L, links, e, psi[:4], psi[:4], o, 3)
+    assert rel(ctx.get_modes(ovl.PSI, 0, 4), p) < 1e-10 and rel(ctx.get_modes(ovl.PSIMID, 0, 4), pm) < 1e-10
+    so.sign.guard = 2
+    ctx.fermion_update(so, 1)
+    t = ctx.timing()
+    assert t.guard_checks == 1 and t.guard_level >= 1 and t.guard_err <= 1e-10
+    # new links: the guard starts again from the requested ladder
+    ctx.set_links(links)
+    ctx.apply_overlap(ovl.PSI, ovl.SCRATCH, ovl.sign_opts(degree=deg, cut=cut, precision=ovl.PREC_MIXED, tol=1e-10))
+    t = ctx.timing()
+    assert t.guard_level == 0 and t.guard_checks == 1 and t.guard_err <= 1e-10
