@@ -532,15 +532,17 @@ std::vector<double> clenshaw_sup(const std::vector<double> &c, double t0)
 }
 
 // Precision ladder of the mixed recurrence: the storage of b_k (k >= first) in a format with unit
-// roundoff u perturbs the result by about kappa * u * B_first.  kappa is set per format from the
+// roundoff u perturbs the result by about kappa * u * B_first.  kappa is set per format above the
 // largest error measured for that format alone (storage roundings of the whole tail from `first`
 // on, host model and GPU, random SU(2)/SU(3) fields and a 6^3 SU(3) lattice with 8 flux quanta, the
-// worst case): FP32 0.1-0.7 u32 B and FP16 up to 0.58 u16 B -> kappa 2; 48-bit storage 0.013-0.052
-// u48 B (its tier sits at low k) -> kappa 0.25.  The coherent worst case (errors aligned with the
-// extreme eigenvector and summed linearly over steps) would ask for 1e4x more and is not what
-// rounding produces.  Returns the smallest `first` >= 1 such that every step k >= first satisfies
-// the bound (n + 1: none).
-int tier_start(const std::vector<double> &B, int n, double u, double tol, double kappa = 2.0)
+// worst case): FP32 0.1-0.7 u32 B and FP16 up to 0.58 u16 B -> kappa 1; 48-bit storage 0.013-0.052
+// u48 B (its tier sits at low k) -> kappa 0.1.  The mixed sign's accuracy guard checks the result
+// against FP64 on the lattice at hand and tightens the ladder where this model falls short (it does
+// when the spectrum of h_W reaches below the cut: B_k is then larger than on [-1, 1]).  The coherent
+// worst case (errors aligned with the extreme eigenvector and summed linearly over steps) would ask
+// for 1e4x more and is not what rounding produces.  Returns the smallest `first` >= 1 such that
+// every step k >= first satisfies the bound (n + 1: none).
+int tier_start(const std::vector<double> &B, int n, double u, double tol, double kappa)
 {
     int first = n + 1;
     for (int k = n; k >= 1; --k) {
@@ -597,6 +599,9 @@ struct SignPlan {
     double sx = 1.0;
 };
 
+// error-model factors of the storage tiers (tier_start)
+constexpr double KAPPA32 = 1.0, KAPPA16 = 1.0, KAPPA48 = 0.1;
+
 // the precision ladder: k < head48 FP64, head48 <= k < head 48-bit storage, head <= k < head16
 // FP32, k >= head16 FP16 storage
 // B1: clenshaw_sup(coef, -1) when the caller holds it (null: computed here for the mixed ladder)
@@ -619,16 +624,16 @@ void plan_tiers(const ovl_sign_opts *o, const std::vector<double> &coef, int *he
         // error budget: 1/4 to each of the FP16 and 48-bit tiers that are on, the rest to FP32
         const double s16 = fp16 ? 0.25 : 0.0, s48 = fp48 ? 0.25 : 0.0, s32 = 1.0 - s16 - s48;
         if (o->head >= 0) *head = std::min(o->head, n + 1);
-        else *head = tier_start(B, n, u32, s32 * tol);
+        else *head = tier_start(B, n, u32, s32 * tol, KAPPA32);
         if (fp16) {
             if (o->head16 > 0) *head16 = std::min(o->head16, n + 1);
-            else *head16 = tier_start(B, n, u16, s16 * tol);
+            else *head16 = tier_start(B, n, u16, s16 * tol, KAPPA16);
             // at least one FP32 step between the FP16 tail and the FP64 head
             *head16 = std::min(std::max(*head16, *head + 1), n + 1);
         }
         *head48 = *head;
         if (fp48 && *head > 1) {
-            const int h = (o->head48 > 0) ? o->head48 : tier_start(B, n, u48, s48 * tol, 0.25);
+            const int h = (o->head48 > 0) ? o->head48 : tier_start(B, n, u48, s48 * tol, KAPPA48);
             // the last step (k = 0) stays FP64: its output feeds the final FP64 stencil
             *head48 = std::min(std::max(h, 1), *head);
         }
@@ -1029,6 +1034,8 @@ int guard_check(ovl_ctx *c, const ovl_sign_opts *o, const void *in64, double xma
         c->guard_err = ok ? worst : INFINITY;
         if (ok && worst <= tol) break;
         c->guard_level++;
+        // level 1 only drops head overrides: without any it is level 0 again
+        if (c->guard_level == 1 && o->head < 0 && o->head16 <= 0 && o->head48 <= 0) c->guard_level = 2;
     }
     c->profiling = prof;
     if (r == OVL_OK) {

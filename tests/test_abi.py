@@ -88,7 +88,7 @@ def _tier_starts_numpy(deg, cut, tol, fp16, fp48=True):
     # recurrence of the Chebyshev coefficients (as orc.cheb_coeffs computes them) on t in [-1, 1];
     # a tier with unit roundoff u starts at the smallest k whose whole tail has kappa u B_k <= its
     # share of tol: 1/4 each to the FP16 and 48-bit tiers when they are on, the rest to FP32;
-    # kappa = 2 for FP32 and FP16, 0.25 for the 48-bit layout
+    # kappa = 1 for FP32 and FP16, 0.1 for the 48-bit layout
     import orc
     c = orc.cheb_coeffs(deg, cut)
     t = np.cos(np.pi * np.arange(2049) / 2048)
@@ -100,7 +100,7 @@ def _tier_starts_numpy(deg, cut, tol, fp16, fp48=True):
         b2, b1 = b1, bk
         B[k] = np.abs(bk).max()
 
-    def start(u, share, kappa=2.0):
+    def start(u, share, kappa=1.0):
         first = deg + 1
         for k in range(deg, 0, -1):
             if kappa * u * B[k] > share:
@@ -110,18 +110,18 @@ def _tier_starts_numpy(deg, cut, tol, fp16, fp48=True):
     s16, s48 = (0.25 if fp16 else 0.0), (0.25 if fp48 else 0.0)
     head = start(2.0 ** -24, (1 - s16 - s48) * tol)
     head16 = min(max(start(2.0 ** -11, s16 * tol), head + 1), deg + 1) if fp16 else deg + 1
-    head48 = min(max(start(2.0 ** -37, s48 * tol, 0.25), 1), head) if fp48 else head
+    head48 = min(max(start(2.0 ** -37, s48 * tol, 0.1), 1), head) if fp48 else head
     return head48, head, head16
 
 
 def test_sign_plan_host_only():
     o = ovl.sign_opts(degree=384, cut=0.05, precision=ovl.PREC_MIXED, tol=1e-10)
-    assert ovl.sign_plan(o) == (19, 117, 211)  # the ladder of the 24^3 SU(3) bench runs
+    assert ovl.sign_plan(o) == (12, 110, 204)  # the ladder of the 24^3 SU(3) bench runs
     assert ovl.sign_plan(o) == _tier_starts_numpy(384, 0.05, 1e-10, True)
     o.fp48 = 0
-    assert ovl.sign_plan(o) == (113, 113, 211) == _tier_starts_numpy(384, 0.05, 1e-10, True, False)
+    assert ovl.sign_plan(o) == (106, 106, 204) == _tier_starts_numpy(384, 0.05, 1e-10, True, False)
     o.fp16_tail = 0
-    assert ovl.sign_plan(o) == (110, 110, 385) == _tier_starts_numpy(384, 0.05, 1e-10, False, False)
+    assert ovl.sign_plan(o) == (103, 103, 385) == _tier_starts_numpy(384, 0.05, 1e-10, False, False)
     o.fp48 = 1
     assert ovl.sign_plan(o) == _tier_starts_numpy(384, 0.05, 1e-10, False, True)
     o = ovl.sign_opts(degree=256, cut=0.05, precision=ovl.PREC_MIXED, tol=1e-11)
