@@ -54,7 +54,10 @@ typedef struct {
     double r_wilson;     /* rWilson   (Simulation.cpp:194) */
     double m_fermion;    /* mFermion  (Simulation.cpp:194) */
     double m_dw;         /* MDWHeight (Simulation.cpp:200), the overlap rho */
-    int mode_block;      /* modes per SoA block: 16, 32 or 64; 0 = choose */
+    int mode_block;      /* modes per SoA block: 16, 32 or 64; 0 = by the mode count; -1 = timed: a
+                            Clenshaw step of every storage precision is timed for each block size
+                            on this lattice (unit links, zero fields) and the fastest kept; the
+                            choice leads ovl_autotune's report */
     int reunitarize;     /* 1: project links back onto SU(Nc) after every link update */
     int traversal;       /* order in which the stencil launches walk (mode, site) lanes over the 8 XCDs:
                             0 = automatic (ovl_autotune picks among the others),
@@ -217,7 +220,7 @@ typedef struct {
     const int *dims;     /* 3 * n_systems lattice sizes Nx, Ny, Nz (each >= 3), system-major */
     int n_modes;         /* mode functions per system and field */
     double r_wilson, m_fermion, m_dw;
-    int mode_block;      /* 16, 32 or 64; 0 = choose */
+    int mode_block;      /* 16, 32 or 64; 0 = by the mode count; -1 = timed (as ovl_params) */
     int reunitarize;
 } ovl_ensemble_params;
 int ovl_create_ensemble(const ovl_ensemble_params *p, ovl_ctx **out);
@@ -300,7 +303,9 @@ int ovl_set_profiling(ovl_ctx *ctx, int on);
 int ovl_autotune(ovl_ctx *ctx, char *report, int len);
 /* the stencil kernel's tuning per storage precision (0 = FP64, 1 = FP32, 2 = FP16, 3 = 48-bit), as
  * ovl_autotune leaves it: lane traversal (1 lexicographic, 2 XCD boxes, 3 XCD mode split; see
- * ovl_params.traversal), y slabs of the traversal, streaming output stores (0/1), occupancy
+ * ovl_params.traversal; for an ensemble 1 = the static work table, 4 = the persistent work-queue
+ * kernel, whose `slabs` is the number of tiles a workgroup claims at a time), y slabs of the
+ * traversal, streaming output stores (0/1), occupancy
  * variant (minimum waves per SIMD, 1-4; 5 = one workgroup per CU through an LDS reservation, where
  * the device allows 96 KiB per workgroup).  Setting one replays a saved tuning without the sweep. */
 typedef struct {

@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <functional>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -51,6 +52,7 @@ struct StepKey {
 
 struct ovl_ctx {
     ovl_params prm{};
+    std::string mb_report;  // the timed mode-block choice (mode_block -1), if one was made
     Geom g{};
     int ncomp = 0;
     size_t field64_bytes = 0, field32_bytes = 0;  // full field (all blocks)
@@ -158,6 +160,9 @@ struct ovl_ctx {
     SysDesc *d_sys = nullptr;           // device copy of sys (ensemble stencil)
     int2 *d_work = nullptr;             // ensemble stencil work table
     int ens_grid = 0;
+    int2 *d_qwork = nullptr;            // the same tiles queue by queue (XCD k's systems together)
+    int *d_qoff = nullptr;              // queue k: d_qwork[d_qoff[k] .. d_qoff[k + 1])
+    int *d_qctr = nullptr;              // claim counters [8] and done counters [8] of the queues
     double ens_imbalance = 1.0;
 };
 
@@ -341,11 +346,17 @@ HopArgs hop_args(const ovl_ctx *c, int prec)
     a.nt_out = c->nt_out[prec];
     a.order = c->order[prec];
     if (c->ensemble) {
-        a.trav = ovl::TRAV_LEX;
+        a.trav = c->trav[prec] == ovl::TRAV_QUEUE ? ovl::TRAV_QUEUE : ovl::TRAV_LEX;
         a.order = nullptr;
         a.ens_sys = c->d_sys;
         a.ens_work = c->d_work;
         a.ens_grid = c->ens_grid;
+        if (a.trav == ovl::TRAV_QUEUE) {
+            a.ens_qwork = c->d_qwork;
+            a.ens_qoff = c->d_qoff;
+            a.ens_qctr = c->d_qctr;
+            a.ens_per = c->slabs[prec];
+        }
     }
     return a;
 }
@@ -421,7 +432,13 @@ int default_slabs(const ovl_ctx *c, int prec)
 // select the traversal of precision `prec`; uploads the slab order when it changes
 int set_traversal(ovl_ctx *c, int prec, int trav, int slabs)
 {
-    if (c->ensemble) return OVL_OK;
+    if (c->ensemble) {
+        // ensembles: the static table (TRAV_LEX) or the work queues with `slabs` tiles per claim
+        c->trav[prec] = trav == ovl::TRAV_QUEUE ? ovl::TRAV_QUEUE : ovl::TRAV_LEX;
+        c->slabs[prec] = c->trav[prec] == ovl::TRAV_QUEUE ? std::max(1, std::min(slabs, 64)) : 1;
+        return OVL_OK;
+    }
+    if (trav == ovl::TRAV_QUEUE) trav = ovl::TRAV_LEX;
     if (trav == ovl::TRAV_BOXES && !c->boxes_ok) trav = ovl::TRAV_LEX;
     slabs = std::max(1, std::min(slabs, c->g.n1));
     if (trav == ovl::TRAV_LEX) slabs = 1;
@@ -1321,6 +1338,20 @@ static int ctx_init(ovl_ctx *c)
             d.dv_n0 = ovl::make_fastdiv((unsigned)d.n0);
         }
         HIPCHK(c, hipMemcpy(c->d_work, tab.data(), sizeof(int2) * tab.size(), hipMemcpyHostToDevice));
+        // the work queues: the same tiles, XCD k's in one run
+        std::vector<int2> qw;
+        std::vector<int> qoff(ovl::NUM_XCD + 1, 0);
+        for (int k = 0; k < ovl::NUM_XCD; ++k) {
+            qoff[k] = (int)qw.size();
+            qw.insert(qw.end(), bins[k].begin(), bins[k].end());
+        }
+        qoff[ovl::NUM_XCD] = (int)qw.size();
+        RETCHK(dalloc(c, (void **)&c->d_qwork, sizeof(int2) * std::max<size_t>(1, qw.size())));
+        RETCHK(dalloc(c, (void **)&c->d_qoff, sizeof(int) * qoff.size()));
+        RETCHK(dalloc(c, (void **)&c->d_qctr, sizeof(int) * 2 * ovl::NUM_XCD));
+        HIPCHK(c, hipMemcpy(c->d_qwork, qw.data(), sizeof(int2) * qw.size(), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(c->d_qoff, qoff.data(), sizeof(int) * qoff.size(), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemset(c->d_qctr, 0, sizeof(int) * 2 * ovl::NUM_XCD));
         HIPCHK(c, hipMemcpy(c->d_sys, c->sys.data(), sizeof(SysDesc) * c->sys.size(), hipMemcpyHostToDevice));
     }
     HIPCHK(c, hipStreamSynchronize(c->st));
@@ -1329,8 +1360,72 @@ static int ctx_init(ovl_ctx *c)
 
 static int pick_mode_block(int mode_block, int n_modes)
 {
-    if (mode_block != 0) return mode_block;
+    if (mode_block > 0) return mode_block;
     return n_modes >= 64 ? 64 : (n_modes > 16 ? 32 : 16);
+}
+
+// mode_block = -1: each block size (16, 32, 64 modes; not one whose padding would double the lanes)
+// gets a context of its own (`make`) with unit links and zero fields; one Clenshaw step of every
+// storage precision is timed at the default traversal, two waves per SIMD and streaming stores (the
+// common choice of the autotuner), and the size with the smallest sum is returned.  The fields are
+// memory-bound and their contents do not change the time.
+static int time_clenshaw_step(ovl_ctx *c, int prec, void *x, void *b1, void *b2, void *w, double *ms_per_step);
+
+static int timed_mode_block(int n_modes, const std::function<int(int, ovl_ctx **)> &make, int *mb_out,
+                            std::string *report)
+{
+    double best = 1e30;
+    int best_mb = pick_mode_block(0, n_modes);
+    std::string rep = "mode block:";
+    for (int mb : {16, 32, 64}) {
+        const int nblk = (n_modes + mb - 1) / mb;
+        if (mb > 16 && (long)nblk * mb >= 2L * n_modes) continue;
+        ovl_ctx *c = nullptr;
+        int rc = make(mb, &c);
+        if (rc != OVL_OK) {
+            if (c) ovl_destroy(c);
+            continue;  // e.g. too large for one launch: not a candidate
+        }
+        double sum = 0;
+        rc = ensure_stencil(c);
+        if (rc == OVL_OK) rc = ensure_work(c, c->g.nblk, true, true, true);
+        for (int prec = 0; prec < NUM_PREC && rc == OVL_OK; ++prec) {
+            c->wpe[prec] = 2;
+            c->nt_out[prec] = 1;
+            auto pick = [prec](void *p64, void *p32, void *p16, void *p48) {
+                return prec == P64 ? p64 : (prec == P32 ? p32 : (prec == P16 ? p16 : p48));
+            };
+            double ms = 0;
+            rc = time_clenshaw_step(c, prec, pick(c->field[OVL_PSI], c->x32, c->x16, c->x48),
+                                    pick(c->a64, c->a32, c->a16, c->a48), pick(c->b64, c->b32, c->b16, c->b48),
+                                    pick(c->w64, c->w32, c->w16, c->w48), &ms);
+            sum += ms;
+        }
+        ovl_destroy(c);
+        if (rc != OVL_OK) continue;
+        char buf[64];
+        std::snprintf(buf, sizeof buf, " %d %.1f us", mb, 1e3 * sum);
+        rep += buf;
+        if (sum < best) {
+            best = sum;
+            best_mb = mb;
+        }
+    }
+    rep += ", chosen " + std::to_string(best_mb);
+    *mb_out = best_mb;
+    if (report) *report = rep;
+    return OVL_OK;
+}
+
+// unit links in the reference format for n links
+static std::vector<double> unit_links(int nc, long n)
+{
+    std::vector<double> u((size_t)n * (nc == 2 ? 4 : 18), 0.0);
+    for (long i = 0; i < n; ++i) {
+        if (nc == 2) u[4 * i + 3] = 1.0;
+        else u[18 * i] = u[18 * i + 8] = u[18 * i + 16] = 1.0;
+    }
+    return u;
 }
 
 int ovl_create(const ovl_params *p, ovl_ctx **out)
@@ -1340,11 +1435,24 @@ int ovl_create(const ovl_params *p, ovl_ctx **out)
     if (p->nc != 2 && p->nc != 3) return OVL_E_ARG;
     if (p->n[0] < 3 || p->n[1] < 3 || p->n[2] < 3) return OVL_E_ARG;
     if (p->n_modes < 1) return OVL_E_ARG;
-    if (p->mode_block != 0 && p->mode_block != 16 && p->mode_block != 32 && p->mode_block != 64) return OVL_E_ARG;
+    if (p->mode_block != -1 && p->mode_block != 0 && p->mode_block != 16 && p->mode_block != 32 &&
+        p->mode_block != 64)
+        return OVL_E_ARG;
     if (p->traversal < 0 || p->traversal > 3) return OVL_E_ARG;
     const long V = (long)p->n[0] * p->n[1] * p->n[2];
     if (V > (1L << 26)) return OVL_E_ARG;
-    const int mb = pick_mode_block(p->mode_block, p->n_modes);
+    std::string mb_report;
+    int mb = pick_mode_block(p->mode_block, p->n_modes);
+    if (p->mode_block == -1) {
+        const std::vector<double> ul = unit_links(p->nc, 3 * V);
+        auto make = [&](int m, ovl_ctx **cc) {
+            ovl_params q = *p;
+            q.mode_block = m;
+            RETCHK(ovl_create(&q, cc));
+            return ovl_set_links(*cc, ul.data());
+        };
+        RETCHK(timed_mode_block(p->n_modes, make, &mb, &mb_report));
+    }
     const int nblk = (p->n_modes + mb - 1) / mb;
     if ((long)nblk * V * mb > (1L << 31) - 1) return OVL_E_ARG;
     int ndev = 0;
@@ -1352,6 +1460,7 @@ int ovl_create(const ovl_params *p, ovl_ctx **out)
     ovl_ctx *c = new ovl_ctx();
     c->prm = *p;
     c->prm.mode_block = mb;
+    c->mb_report = mb_report;
     c->g.nc = p->nc;
     c->g.n0 = p->n[0];
     c->g.n1 = p->n[1];
@@ -1376,8 +1485,27 @@ int ovl_create_ensemble(const ovl_ensemble_params *p, ovl_ctx **out)
     *out = nullptr;
     if (p->nc != 2 && p->nc != 3) return OVL_E_ARG;
     if (p->n_systems < 1 || p->n_systems > 65536 || p->n_modes < 1) return OVL_E_ARG;
-    if (p->mode_block != 0 && p->mode_block != 16 && p->mode_block != 32 && p->mode_block != 64) return OVL_E_ARG;
-    const int mb = pick_mode_block(p->mode_block, p->n_modes);
+    if (p->mode_block != -1 && p->mode_block != 0 && p->mode_block != 16 && p->mode_block != 32 &&
+        p->mode_block != 64)
+        return OVL_E_ARG;
+    std::string mb_report;
+    int mb = pick_mode_block(p->mode_block, p->n_modes);
+    if (p->mode_block == -1) {
+        long sites = 0;
+        for (int s = 0; s < p->n_systems; ++s) {
+            const int *n = p->dims + 3 * s;
+            sites += (long)n[0] * n[1] * n[2];
+        }
+        if (sites > (1L << 29)) return OVL_E_ARG;
+        const std::vector<double> ul = unit_links(p->nc, 3 * sites);
+        auto make = [&](int m, ovl_ctx **cc) {
+            ovl_ensemble_params q = *p;
+            q.mode_block = m;
+            RETCHK(ovl_create_ensemble(&q, cc));
+            return ovl_set_links(*cc, ul.data());
+        };
+        RETCHK(timed_mode_block(p->n_modes, make, &mb, &mb_report));
+    }
     const int nblk = (p->n_modes + mb - 1) / mb;
     std::vector<SysDesc> sys;
     long q0 = 0, sites = 0;
@@ -1395,6 +1523,7 @@ int ovl_create_ensemble(const ovl_ensemble_params *p, ovl_ctx **out)
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return OVL_E_HIP;
     ovl_ctx *c = new ovl_ctx();
     c->ensemble = true;
+    c->mb_report = mb_report;
     c->prm.nc = p->nc;
     c->prm.n[0] = (int)q0;  // the concatenation seen as one flat run of site-blocks
     c->prm.n[1] = 1;
@@ -1432,7 +1561,8 @@ int ovl_destroy(ovl_ctx *c)
     void *ps[] = {c->links, c->efield, c->ul64, c->ul32, c->phase, c->field[0], c->field[1], c->field[2], c->w64, c->a64,
                   c->b64, c->x32, c->w32, c->a32, c->b32, c->partial, c->dout, c->gam, c->jv, c->ja, c->staging,
                   c->ms_ps, c->ms_r, c->ms_q, c->ms_w, c->ms_chi, c->ms_coef, c->ms_active, c->rz[0], c->rz[1],
-                  c->rz[2], c->rz[3], c->rz[4], c->rz[5], c->d_sys, c->d_work, c->order[0], c->order[1],
+                  c->rz[2], c->rz[3], c->rz[4], c->rz[5], c->d_sys, c->d_work, c->d_qwork, c->d_qoff, c->d_qctr,
+                  c->order[0], c->order[1],
                   c->order[2], c->order[3], c->x16, c->w16, c->a16, c->b16, c->x48, c->w48, c->a48, c->b48,
                   c->f32[0], c->f32[1], c->guard_buf[0], c->guard_buf[1]};
     if (c->step_exec) hipGraphExecDestroy(c->step_exec);
@@ -2086,7 +2216,8 @@ static StepKey make_step_key(const ovl_ctx *c, const ovl_step_opts *o, const Sig
     }
     k.ptrs = {c->links, c->efield, c->ul64, c->ul32, c->phase, c->field[0], c->field[1], c->field[2],
               c->w64, c->a64, c->b64, c->x32, c->w32, c->a32, c->b32, c->x16, c->w16, c->a16, c->b16,
-              c->x48, c->w48, c->a48, c->b48, c->partial, c->d_sys, c->d_work, c->order[0], c->order[1],
+              c->x48, c->w48, c->a48, c->b48, c->partial, c->d_sys, c->d_work, c->d_qwork, c->d_qoff, c->d_qctr,
+              c->order[0], c->order[1],
               c->order[2], c->order[3], c->f32[0], c->f32[1]};
     k.ints = {c->work_blocks, c->ens_grid};
     for (int nb : c->sys_nb) k.ints.push_back(nb != 0);  // systems whose links take the phases
@@ -2936,7 +3067,10 @@ int ovl_set_profiling(ovl_ctx *c, int on)
 }
 
 // public traversal numbers (ovl_params.traversal) <-> TRAV_*
-static int trav_public(int t) { return t == ovl::TRAV_BOXES ? 2 : (t == ovl::TRAV_MODES ? 3 : 1); }
+static int trav_public(int t)
+{
+    return t == ovl::TRAV_BOXES ? 2 : (t == ovl::TRAV_MODES ? 3 : (t == ovl::TRAV_QUEUE ? 4 : 1));
+}
 
 int ovl_get_tuning(ovl_ctx *c, int precision, ovl_tuning *t)
 {
@@ -2951,14 +3085,51 @@ int ovl_get_tuning(ovl_ctx *c, int precision, ovl_tuning *t)
 int ovl_set_tuning(ovl_ctx *c, int precision, const ovl_tuning *t)
 {
     if (!c || !t || precision < 0 || precision >= NUM_PREC) return OVL_E_ARG;
-    if (t->traversal < 1 || t->traversal > 3 || t->slabs < 1 || t->waves_per_eu < 1 || t->waves_per_eu > 5 ||
+    if (t->traversal < 1 || t->traversal > 4 || t->slabs < 1 || t->waves_per_eu < 1 || t->waves_per_eu > 5 ||
         (t->nt_store != 0 && t->nt_store != 1))
         return fail(c, OVL_E_ARG, "tuning out of range");
-    const int trav = t->traversal == 2 ? ovl::TRAV_BOXES : (t->traversal == 3 ? ovl::TRAV_MODES : ovl::TRAV_LEX);
+    if ((t->traversal == 4) != c->ensemble && t->traversal != 1)
+        return fail(c, OVL_E_ARG, "traversals 2 and 3 are for single lattices, 4 (work queues) for ensembles");
+    const int trav = t->traversal == 2 ? ovl::TRAV_BOXES
+                   : (t->traversal == 3 ? ovl::TRAV_MODES : (t->traversal == 4 ? ovl::TRAV_QUEUE : ovl::TRAV_LEX));
     RETCHK(set_traversal(c, precision, trav, t->slabs));
     c->nt_out[precision] = t->nt_store;
     c->wpe[precision] = t->waves_per_eu;
     c->tuned = true;
+    return OVL_OK;
+}
+
+// Time of one Clenshaw step of storage precision prec (the pure launch w = h b1, then the epilogue
+// b2 = 0.1 h w + 0.01 x + 0.5 b1 - 0.5 b2) with the context's current tuning: three steps after a
+// warm-up pass, in ms per step.
+static int time_clenshaw_step(ovl_ctx *c, int prec, void *x, void *b1, void *b2, void *w, double *ms_per_step)
+{
+    float ms = 0;
+    for (int rep_i = 0; rep_i < 2; ++rep_i) {  // first pass warms up
+        HIPCHK(c, hipEventRecord(c->t0, c->st));
+        for (int k = 0; k < 3; ++k) {
+            HopArgs a = hop_args(c, prec);
+            a.in = b1;
+            a.out = w;
+            a.sh_r = 1.0;
+            RETCHK(hop(c, c->g, prec, prec, a));
+            HopArgs q = hop_args(c, prec);
+            q.in = w;
+            q.out = b2;
+            q.sh_r = 0.1;
+            q.x = x;
+            q.sx_r = 0.01;
+            q.b1 = b1;
+            q.s1_r = 0.5;
+            q.b2 = b2;
+            q.s2_r = -0.5;
+            RETCHK(hop(c, c->g, prec, prec, q));
+        }
+        HIPCHK(c, hipEventRecord(c->t1, c->st));
+        HIPCHK(c, hipEventSynchronize(c->t1));
+        HIPCHK(c, hipEventElapsedTime(&ms, c->t0, c->t1));
+    }
+    *ms_per_step = ms / 3.0;
     return OVL_OK;
 }
 
@@ -2993,6 +3164,7 @@ int ovl_autotune(ovl_ctx *c, char *report, int len)
     HIPCHK(c, hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
     const bool one_wg = max_lds >= 96 * 1024;
     auto trav_name = [](const ovl_ctx *cc, int prec) {
+        if (cc->trav[prec] == ovl::TRAV_QUEUE) return std::string("queue/") + std::to_string(cc->slabs[prec]);
         if (cc->trav[prec] == ovl::TRAV_MODES) return std::string("modes/") + std::to_string(cc->slabs[prec]);
         if (cc->trav[prec] == ovl::TRAV_BOXES) return std::string("boxes/") + std::to_string(cc->slabs[prec]);
         return std::string("lex");
@@ -3000,7 +3172,11 @@ int ovl_autotune(ovl_ctx *c, char *report, int len)
     for (int prec = 0; prec < NUM_PREC; ++prec) {
         // candidate traversals (trav, slabs)
         std::vector<std::pair<int, int>> cand;
-        if (c->ensemble || c->prm.traversal != 0) {
+        if (c->ensemble) {
+            // the static table, and the work queues claiming 1, 4 or 16 tiles at a time
+            cand.push_back({ovl::TRAV_LEX, 1});
+            for (int t : {1, 4, 16}) cand.push_back({ovl::TRAV_QUEUE, t});
+        } else if (c->prm.traversal != 0) {
             cand.push_back({c->trav[prec], c->slabs[prec]});
         } else {
             if (c->boxes_ok)
@@ -3025,34 +3201,11 @@ int ovl_autotune(ovl_ctx *c, char *report, int len)
                 for (int v = 1; v <= (one_wg ? 5 : 4); ++v) {
                     c->wpe[prec] = v;
                     c->nt_out[prec] = nt;
-                    float ms = 0;
-                    for (int rep_i = 0; rep_i < 2; ++rep_i) {  // first pass warms up
-                        HIPCHK(c, hipEventRecord(c->t0, c->st));
-                        for (int k = 0; k < 3; ++k) {
-                            HopArgs a = hop_args(c, prec);
-                            a.in = b1;
-                            a.out = w;
-                            a.sh_r = 1.0;
-                            RETCHK(hop(c, c->g, prec, prec, a));
-                            HopArgs q = hop_args(c, prec);
-                            q.in = w;
-                            q.out = b2;
-                            q.sh_r = 0.1;
-                            q.x = x;
-                            q.sx_r = 0.01;
-                            q.b1 = b1;
-                            q.s1_r = 0.5;
-                            q.b2 = b2;
-                            q.s2_r = -0.5;
-                            RETCHK(hop(c, c->g, prec, prec, q));
-                        }
-                        HIPCHK(c, hipEventRecord(c->t1, c->st));
-                        HIPCHK(c, hipEventSynchronize(c->t1));
-                        HIPCHK(c, hipEventElapsedTime(&ms, c->t0, c->t1));
-                    }
+                    double ms = 0;
+                    RETCHK(time_clenshaw_step(c, prec, x, b1, b2, w, &ms));
                     char buf[96];
                     std::snprintf(buf, sizeof buf, "%s %s%s wpe=%d %.1f us; ", pname[prec],
-                                  trav_name(c, prec).c_str(), nt ? " nt" : "", v, 1e3 * ms / 3);
+                                  trav_name(c, prec).c_str(), nt ? " nt" : "", v, 1e3 * ms);
                     rep += buf;
                     if (ms < best) {
                         best = ms;
@@ -3068,6 +3221,7 @@ int ovl_autotune(ovl_ctx *c, char *report, int len)
     }
     c->profiling = prof;
     c->tuned = true;
+    if (!c->mb_report.empty()) rep = c->mb_report + "; " + rep;
     rep += "chosen";
     for (int prec = 0; prec < NUM_PREC; ++prec) {
         char buf[96];

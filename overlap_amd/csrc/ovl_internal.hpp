@@ -83,13 +83,22 @@ struct HopArgs {
     const SysDesc *ens_sys;
     const int2 *ens_work;
     int ens_grid;
+    // ensemble work queues (trav == TRAV_QUEUE): the tiles {system, workgroup within the system} of
+    // XCD k's systems are ens_qwork[ens_qoff[k] .. ens_qoff[k + 1]); a persistent grid of workgroups
+    // serves them, workgroup b the queue of XCD b % 8, claiming ens_per tiles at a time from the
+    // counter ens_qctr[k] (ens_qctr[8 + k] counts the workgroups done; the last one of a queue
+    // resets both for the next launch)
+    const int2 *ens_qwork;
+    const int *ens_qoff;
+    int *ens_qctr;
+    int ens_per;
     // lane coordinates of a single-lattice launch (set by the launcher): V, nblk V / 8, V / 8,
     // n0 n1, n0; lanes are counted in 32 bits (nblk V MB < 2^31)
     FastDiv dv_V, dv_per, dv_v8, dv_n01, dv_n0;
 };
 
 // lane traversals of the stencil launch (see lane_coords)
-enum { TRAV_LEX = 0, TRAV_BOXES = 1, TRAV_MODES = 2 };
+enum { TRAV_LEX = 0, TRAV_BOXES = 1, TRAV_MODES = 2, TRAV_QUEUE = 3 };
 
 // number of XCDs the hardware round-robins workgroups over (MI355X: 8)
 constexpr int NUM_XCD = 8;

@@ -565,15 +565,12 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(WPE))) void
     hop_site<NC, TI, TO, MB, EPI>(a, n0, n1, n2, s, x, y, z, blk * V, 0, m);
 }
 
-// Ensemble launch: many independent lattices of different sizes in one grid.  Workgroup b works on
-// workgroup ens_work[b].y of system ens_work[b].x; the host lays the table out so that the
-// workgroups of one system all run on one XCD (whole systems per XCD L2, balanced by size).
-template <int NC, typename TI, typename TO, int MB, int WPE>
-__global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(WPE))) void k_hop_ens(
-    HopArgs a)
+// Ensemble launch: many independent lattices of different sizes in one grid.  One tile = TPB lanes
+// of one system: workgroup w.y of system w.x (lanes (site, mode) in block-major order, as k_hop's
+// lexicographic traversal).
+template <int NC, typename TI, typename TO, int MB, bool EPI>
+__device__ __forceinline__ void hop_ens_tile(const HopArgs &a, int2 w)
 {
-    const int2 w = a.ens_work[blockIdx.x];
-    if (w.x < 0) return;
     const SysDesc d = a.ens_sys[w.x];
     const unsigned gl = (unsigned)w.y * TPB + threadIdx.x;  // lane within the system
     if (gl >= (unsigned)(d.V * d.nblk * MB)) return;
@@ -585,7 +582,42 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(WPE))) void
     const int z = (int)fdiv((unsigned)s, d.dv_n01);
     const int rem = s - z * d.n0 * d.n1;
     const int y = (int)fdiv((unsigned)rem, d.dv_n0);
-    hop_site<NC, TI, TO, MB>(a, d.n0, d.n1, d.n2, s, rem - y * d.n0, y, z, d.q0 + blk * d.V, d.l0, m);
+    hop_site<NC, TI, TO, MB, EPI>(a, d.n0, d.n1, d.n2, s, rem - y * d.n0, y, z, d.q0 + blk * d.V, d.l0, m);
+}
+
+// Static table (ens_qctr null): workgroup b runs tile ens_work[b]; the host lays the table out so
+// that the tiles of one system all run on one XCD (whole systems per XCD L2, balanced by size).
+// Work queues (ens_qctr set): a persistent grid; workgroup b claims ens_per tiles at a time from the
+// queue of XCD b % 8 until it is empty, so the tiles of an XCD's systems go to whichever of its
+// workgroups is free.  Every workgroup leaves when its queue is exhausted (no workgroup waits on
+// another); the last one of each queue to leave resets the queue's counters for the next launch.
+template <int NC, typename TI, typename TO, int MB, int WPE, bool EPI>
+__global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(WPE))) void k_hop_ens(HopArgs a)
+{
+    if (!a.ens_qctr) {
+        const int2 w = a.ens_work[blockIdx.x];
+        if (w.x >= 0) hop_ens_tile<NC, TI, TO, MB, EPI>(a, w);
+        return;
+    }
+    const int q = (int)(blockIdx.x % NUM_XCD);
+    const int beg = a.ens_qoff[q], len = a.ens_qoff[q + 1] - beg;
+    __shared__ int claim;
+    for (;;) {
+        if (threadIdx.x == 0) claim = atomicAdd(&a.ens_qctr[q], 1);
+        __syncthreads();
+        const int t0 = claim * a.ens_per;
+        __syncthreads();  // every lane has read `claim` before the next claim overwrites it
+        if (t0 >= len) break;
+        const int t1 = min(len, t0 + a.ens_per);
+        for (int t = t0; t < t1; ++t) hop_ens_tile<NC, TI, TO, MB, EPI>(a, a.ens_qwork[beg + t]);
+    }
+    if (threadIdx.x == 0) {
+        const int wgs = (int)(gridDim.x / NUM_XCD);
+        if (atomicAdd(&a.ens_qctr[NUM_XCD + q], 1) == wgs - 1) {
+            atomicExch(&a.ens_qctr[q], 0);
+            atomicExch(&a.ens_qctr[NUM_XCD + q], 0);
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1599,20 +1631,41 @@ void hop_wpe(const HopArgs &b, unsigned nb, int wpe, const Geom &g, hipStream_t 
     else hop_wpe_epi<NC, TI, TO, MB, false>(b, nb, wpe, g, st);
 }
 
+template <int NC, typename TI, typename TO, int MB, bool EPI>
+void hop_ens_wpe_epi(const HopArgs &b, int wpe, hipStream_t st)
+{
+    unsigned nb = (unsigned)b.ens_grid;
+    if (b.ens_qctr) {
+        // persistent grid: the resident workgroups of the variant on every CU, a multiple of the XCDs
+        int dev = 0, cus = 0, per_cu = 0;
+        (void)hipGetDevice(&dev);
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
+        const void *f = wpe == 2 ? reinterpret_cast<const void *>(k_hop_ens<NC, TI, TO, MB, 2, EPI>)
+                      : wpe == 3 ? reinterpret_cast<const void *>(k_hop_ens<NC, TI, TO, MB, 3, EPI>)
+                      : wpe == 4 ? reinterpret_cast<const void *>(k_hop_ens<NC, TI, TO, MB, 4, EPI>)
+                                 : reinterpret_cast<const void *>(k_hop_ens<NC, TI, TO, MB, 1, EPI>);
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, TPB, wpe == 5 ? LDS_ONE_WG : 0) != hipSuccess ||
+            per_cu < 1)
+            per_cu = 1;
+        nb = (unsigned)std::max(NUM_XCD, (cus * per_cu) / NUM_XCD * NUM_XCD);
+    }
+    switch (wpe) {
+    case 2: k_hop_ens<NC, TI, TO, MB, 2, EPI><<<nb, TPB, 0, st>>>(b); break;
+    case 3: k_hop_ens<NC, TI, TO, MB, 3, EPI><<<nb, TPB, 0, st>>>(b); break;
+    case 4: k_hop_ens<NC, TI, TO, MB, 4, EPI><<<nb, TPB, 0, st>>>(b); break;
+    case 5:
+        (void)reserve_lds(reinterpret_cast<const void *>(k_hop_ens<NC, TI, TO, MB, 1, EPI>));
+        k_hop_ens<NC, TI, TO, MB, 1, EPI><<<nb, TPB, LDS_ONE_WG, st>>>(b);
+        break;
+    default: k_hop_ens<NC, TI, TO, MB, 1, EPI><<<nb, TPB, 0, st>>>(b); break;
+    }
+}
 template <int NC, typename TI, typename TO, int MB>
 void hop_ens_wpe(const HopArgs &b, int wpe, hipStream_t st)
 {
-    const unsigned nb = (unsigned)b.ens_grid;
-    switch (wpe) {
-    case 2: k_hop_ens<NC, TI, TO, MB, 2><<<nb, TPB, 0, st>>>(b); break;
-    case 3: k_hop_ens<NC, TI, TO, MB, 3><<<nb, TPB, 0, st>>>(b); break;
-    case 4: k_hop_ens<NC, TI, TO, MB, 4><<<nb, TPB, 0, st>>>(b); break;
-    case 5:
-        (void)reserve_lds(reinterpret_cast<const void *>(k_hop_ens<NC, TI, TO, MB, 1>));
-        k_hop_ens<NC, TI, TO, MB, 1><<<nb, TPB, LDS_ONE_WG, st>>>(b);
-        break;
-    default: k_hop_ens<NC, TI, TO, MB, 1><<<nb, TPB, 0, st>>>(b); break;
-    }
+    // the pure launch of the Clenshaw step (no pointwise operand) has a variant of its own
+    if (b.x || b.b1 || b.b2 || b.tgt || b.out2) hop_ens_wpe_epi<NC, TI, TO, MB, true>(b, wpe, st);
+    else hop_ens_wpe_epi<NC, TI, TO, MB, false>(b, wpe, st);
 }
 
 template <int NC, typename TI, typename TO>

@@ -182,3 +182,36 @@ def test_ensemble_rejects_single_lattice_calls(gpu):
         with pytest.raises(ovl.OvlError) as ex:
             ctx.get_system_modes(1, ovl.PSI)
         assert ex.value.code == ovl.E_ARG
+
+
+@pytest.mark.parametrize("nc,mb,nm", [(2, 32, 32), (3, 64, 64), (2, 16, 20)])
+def test_ensemble_work_queues_are_permutations(gpu, nc, mb, nm):
+    # the persistent work-queue kernel (tuning traversal 4, 1 / 4 / 16 tiles per claim) runs the same
+    # tiles as the static table in another order: a Wilson apply, a mixed sign (pure and epilogue
+    # launches of every storage tier) and a graph-replayed leapfrog are bit-identical, and repeated
+    # launches find the queues reset
+    print(f"[case] ensemble queues nc={nc} mb={mb} modes={nm}", flush=True)
+    dims = [(12, 12, 12), (6, 5, 4), (8, 8, 8), (3, 4, 5), (10, 10, 10)]
+    sy = systems(nc, nm, 700 + nc, dims=dims)
+    so = ovl.sign_opts(degree=40, cut=0.1, precision=ovl.PREC_MIXED, head48=3, head=10, head16=24, guard=0)
+    step = ovl.StepOpts(ovl.OVERLAP, 0.02, ovl.GAUGE_LINKS, so, 1)
+    outs = []
+    for trav, per in ((1, 1), (4, 1), (4, 4), (4, 16)):
+        ens = make_ensemble(nc, nm, sy, mb=mb, M=1.2, flux=[1, 0, 2, 0, 3])
+        for p in range(4):
+            t = ens.get_tuning(p)
+            ens.set_tuning(p, trav, per, t.nt_store, t.waves_per_eu)
+        ens.apply_wilson(ovl.PSI, ovl.PSIMID)
+        r = [ens.get_system_modes(k, ovl.PSIMID) for k in range(len(dims))]
+        ens.apply_sign(ovl.PSI, ovl.SCRATCH, so)
+        r += [ens.get_system_modes(k, ovl.SCRATCH) for k in range(len(dims))]
+        for k, s_ in enumerate(sy):
+            ens.set_system_modes(k, ovl.PSIMID, s_[3])
+        ens.fermion_update(step, 3)
+        r += [ens.get_system_modes(k, ovl.PSI) for k in range(len(dims))]
+        assert ens.get_tuning(0).traversal == trav
+        outs.append(r)
+        ens.close()
+    for other in outs[1:]:
+        for a, b in zip(outs[0], other):
+            assert np.isfinite(a).all() and np.array_equal(a, b)

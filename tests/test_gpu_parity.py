@@ -761,3 +761,37 @@ def test_guard_falls_back_to_fp64_below_the_cut(gpu):
     print(f"  checks {t.guard_checks}, level {t.guard_level}, last check {t.guard_err:.2e}", flush=True)
     assert t.guard_level == 6 and t.guard_checks == 5 and t.guard_err > 1e-10
     assert np.array_equal(ctx.get_modes(ovl.SCRATCH), ref)
+
+
+def test_timed_mode_block(gpu):
+    # mode_block = -1: every block size is timed on the lattice and the fastest kept; the context
+    # then computes exactly what a context created with that block size computes
+    import re
+    print("[case] timed mode block", flush=True)
+    n, nc, nm = (8, 8, 6), 3, 96
+    g = cf.rng(880)
+    links = cf.su3_links(n, 0.3, g)
+    psi = cf.spinors(n, nc, nm, g)
+    with ovl.Context(nc, n, nm, m_dw=1.2, mode_block=-1) as ctx:
+        ctx.set_links(links)
+        ctx.set_modes(ovl.PSI, psi)
+        rep = ctx.autotune()
+        m = re.search(r"mode block:((?: \d+ [0-9.]+ us)+), chosen (\d+)", rep)
+        assert m, rep[:200]
+        sizes = [int(v) for v in m.group(1).split()[0::3]]
+        chosen = int(m.group(2))
+        print(f"  {m.group(0)}", flush=True)
+        assert sizes == [16, 32, 64] and chosen in sizes
+        ctx.apply_wilson(ovl.PSI, ovl.PSIMID)
+        a = ctx.get_modes(ovl.PSIMID)
+    with ovl.Context(nc, n, nm, m_dw=1.2, mode_block=chosen) as ref:
+        ref.set_links(links)
+        ref.set_modes(ovl.PSI, psi)
+        ref.apply_wilson(ovl.PSI, ovl.PSIMID)
+        assert np.array_equal(a, ref.get_modes(ovl.PSIMID))
+    # 20 modes: a 64-mode block would more than double the lanes and is not a candidate
+    with ovl.Ensemble(2, [(4, 4, 4), (5, 4, 3)], 20, mode_block=-1) as ens:
+        for k, d in enumerate([(4, 4, 4), (5, 4, 3)]):
+            ens.set_system_links(k, cf.su2_links(d, 0.3, g))
+        m = re.search(r"mode block:((?: \d+ [0-9.]+ us)+), chosen (\d+)", ens.autotune())
+        assert m and [int(v) for v in m.group(1).split()[0::3]] == [16, 32]
