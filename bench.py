@@ -589,6 +589,28 @@ def main():
             if alg_step_bytes:
                 roofline["step_alg_frac"] = round(alg_step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
 
+    # --- cost of one accuracy-guard check of the mixed ladder (due every 100 steps and whenever new
+    # links are set): the links are set again to their current values, which makes a check due, and
+    # a one-step graph call with the check is timed against the next one without it
+    guard = None
+    if c.kind == "overlap" and not args.quick:
+        sog1 = step_opts(c, prec_mixed, True, args.tol, args.chunk)
+        ctx.set_links(ctx.get_links())
+        t0 = time.perf_counter()
+        ctx.fermion_update(sog1, 1)
+        w_check = time.perf_counter() - t0
+        tg = ctx.timing()
+        t0 = time.perf_counter()
+        ctx.fermion_update(sog1, 1)
+        w_plain = time.perf_counter() - t0
+        check_ms = 1e3 * (w_check - w_plain)
+        every = int(sog1.sign.guard)
+        guard = {"checks_in_call": int(tg.guard_checks), "level_after": int(tg.guard_level),
+                 "max_rel_err_of_check": float(tg.guard_err), "check_ms": round(check_ms, 2),
+                 "every_steps": every, "amortised_ms_per_step": round(check_ms / max(every, 1), 3),
+                 "ms_per_step_with_guard": round(ms_per_step + check_ms / max(every, 1), 2)}
+        log(f"guard: {json.dumps(guard)}")
+
     # --- Chern-Simons rate and axial-charge history (currents from the leapfrog's own overlap apply)
     observables = None
     if c.kind == "overlap" and not args.quick:
