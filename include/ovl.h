@@ -4,7 +4,8 @@
  * The reference exposes this path as free C++ functions on GaugeLinks / ElectricFields /
  * FermionField objects (no plugin or FFI layer).  Each entry point below names the reference
  * interface it replaces.  All functions return an int status (OVL_OK = 0, negative = error);
- * ovl_error() gives the message of the last failure on a context.
+ * ovl_error() gives the message of the last failure on a context (ovl_error(NULL): of the last
+ * failed context-free call on the calling thread).
  *
  * Ownership: device memory is owned by the context; host pointers are borrowed for the
  * duration of the call only.  One context per host thread; every call is ordered on the
@@ -237,6 +238,20 @@ int ovl_get_system_modes(ovl_ctx *ctx, int sys, int field, double *aos, int firs
 enum { OVL_OBS_E_EL = 0, OVL_OBS_E_MAG, OVL_OBS_GAUSS_MAX, OVL_OBS_GAUSS_RMS, OVL_OBS_UNIT_MAX,
        OVL_OBS_UNIT_RMS, OVL_OBS_NCS_DOT, OVL_OBS_COUNT };
 int ovl_system_observables(ovl_ctx *ctx, int sys, double *obs);
+/* Allocation-free variant of ovl_system_observables.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under hipGraph capture, serving loops).  It
+ * takes no context: `links` and `efield` are DEVICE arrays in the host layouts above (nc colours,
+ * lattice n[0] x n[1] x n[2]), `obs` a DEVICE array of OVL_OBS_COUNT doubles that receives the
+ * values ovl_system_observables returns for the same fields, bit for bit.  The kernels are queued on
+ * `stream` (a hipStream_t; NULL = the default stream) and the call returns without waiting.  It
+ * calls no allocator: the reductions' partial results live in a static 32 KiB array per device,
+ * shared by every call on that device, so calls that overlap on one device (other streams, other
+ * threads) race and must be ordered by the caller.  At most OVL_NOALLOC_MAX_SITES sites; larger
+ * lattices, nc other than 2/3 or an extent below 3 return OVL_E_ARG, and ovl_error(NULL) then gives
+ * the message (per thread). */
+#define OVL_NOALLOC_MAX_SITES (1L << 26)
+int ovl_observables_noalloc(int nc, const int *n, const double *links, const double *efield, double *obs,
+                            void *stream);
 /* stencil work balance of an ensemble: max over XCDs of the workgroups assigned / mean (1 = even) */
 int ovl_ensemble_balance(ovl_ctx *ctx, double *imbalance);
 

@@ -15,6 +15,7 @@
 #include <cstring>
 #include <fstream>
 #include <functional>
+#include <mutex>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -1576,7 +1577,12 @@ int ovl_destroy(ovl_ctx *c)
     return OVL_OK;
 }
 
-const char *ovl_error(const ovl_ctx *c) { return c ? c->err.c_str() : "null context"; }
+static thread_local std::string g_free_err;  // last failure of a context-free call on this thread
+const char *ovl_error(const ovl_ctx *c)
+{
+    if (c) return c->err.c_str();
+    return g_free_err.empty() ? "null context" : g_free_err.c_str();
+}
 
 static int stage(ovl_ctx *c, size_t bytes)
 {
@@ -2490,6 +2496,92 @@ int ovl_system_observables(ovl_ctx *c, int sys, double *obs)
     RETCHK(obs_gauss(c, sys, &obs[OVL_OBS_GAUSS_MAX], &obs[OVL_OBS_GAUSS_RMS]));
     RETCHK(obs_unitarity(c, sys, &obs[OVL_OBS_UNIT_MAX], &obs[OVL_OBS_UNIT_RMS]));
     return obs_ncs_dot(c, sys, &obs[OVL_OBS_NCS_DOT]);
+}
+
+// ---- allocation-free observables ----
+// The reductions' per-workgroup partial results (ovl::reduce_partial_len doubles) and the four raw
+// results live in this file-scope device array instead of a context's buffers.  Each device holds
+// its own copy; host code uses only the address hipGetSymbolAddress returns for the current device.
+constexpr size_t NOALLOC_SCRATCH_DOUBLES = 4096;  // 32 KiB per device
+constexpr size_t NOALLOC_RAW = NOALLOC_SCRATCH_DOUBLES - 8;  // raw results: scratch[NOALLOC_RAW .. +7)
+static __device__ __align__(256) double scratch[NOALLOC_SCRATCH_DOUBLES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[NOALLOC_MAX_DEVICES];  // device address of `scratch`, per device
+
+static int free_fail(int code, const std::string &msg)
+{
+    g_free_err = msg;
+    return code;
+}
+
+// the device address of `scratch` on the current device, looked up once per device
+static int noalloc_scratch(double **out)
+{
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0) return free_fail(OVL_E_HIP, "no current HIP device");
+    if (dev >= NOALLOC_MAX_DEVICES) return free_fail(OVL_E_HIP, "device ordinal beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return free_fail(OVL_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = static_cast<double *>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return OVL_OK;
+}
+
+// raw reduction results -> the quantities of ovl_system_observables, in the same arithmetic
+__global__ void k_obs_noalloc_finish(const double *raw, double *obs, double n_adj_v, double three_v)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    obs[OVL_OBS_E_EL] = raw[0];
+    obs[OVL_OBS_E_MAG] = raw[1];
+    obs[OVL_OBS_GAUSS_MAX] = raw[2];
+    obs[OVL_OBS_GAUSS_RMS] = sqrt(raw[3] / n_adj_v);
+    obs[OVL_OBS_UNIT_MAX] = raw[4];
+    obs[OVL_OBS_UNIT_RMS] = sqrt(raw[5] / three_v);
+    obs[OVL_OBS_NCS_DOT] = 0.125 * raw[6] / (M_PI * M_PI);  // ChernSimonsDerivative.cpp:47
+}
+
+int ovl_observables_noalloc(int nc, const int *n, const double *links, const double *efield, double *obs,
+                            void *stream)
+{
+    if ((nc != 2 && nc != 3) || !n || !links || !efield || !obs)
+        return free_fail(OVL_E_ARG, "ovl_observables_noalloc: nc must be 2 or 3 and every pointer set");
+    if (n[0] < 3 || n[1] < 3 || n[2] < 3)
+        return free_fail(OVL_E_ARG, "ovl_observables_noalloc: every lattice extent must be >= 3");
+    const long V = (long)n[0] * n[1] * n[2];
+    if (V > OVL_NOALLOC_MAX_SITES)
+        return free_fail(OVL_E_ARG, "ovl_observables_noalloc: " + std::to_string(V) + " sites exceed the maximum of " +
+                                        std::to_string((long)OVL_NOALLOC_MAX_SITES));
+    Geom g{};
+    g.nc = nc;
+    g.n0 = n[0];
+    g.n1 = n[1];
+    g.n2 = n[2];
+    g.V = (int)V;
+    g.mb = 16;
+    g.nblk = 1;
+    if (ovl::reduce_partial_len(g) > NOALLOC_RAW)
+        return free_fail(OVL_E_ARG, "ovl_observables_noalloc: reduction partials exceed the static scratch");
+    double *s = nullptr;
+    RETCHK(noalloc_scratch(&s));
+    double *raw = s + NOALLOC_RAW;
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    hipError_t e = ovl::launch_energy(g, links, efield, s, raw + 0, st);
+    if (e == hipSuccess) e = ovl::launch_gauss(g, links, efield, s, raw + 2, nullptr, st);
+    if (e == hipSuccess) e = ovl::launch_unitarity(g, links, s, raw + 4, st);
+    if (e == hipSuccess) e = ovl::launch_edotb(g, links, efield, s, raw + 6, st);
+    if (e == hipSuccess) {
+        k_obs_noalloc_finish<<<1, 64, 0, st>>>(raw, obs, (double)(nc * nc - 1) * (double)V, 3.0 * (double)V);
+        e = hipGetLastError();
+    }
+    if (e != hipSuccess) return free_fail(OVL_E_HIP, std::string("ovl_observables_noalloc: ") + hipGetErrorString(e));
+    return OVL_OK;
 }
 
 // GaussLawRestoration::Restore (GaussRestore.cpp:129-160): relaxation sweeps of E with step

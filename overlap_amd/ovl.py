@@ -31,7 +31,7 @@ SYMBOLS = [
     "ovl_fermion_update_observed", "ovl_mode_norms",
     "ovl_create_ensemble", "ovl_set_system_links", "ovl_get_system_links", "ovl_set_system_efield",
     "ovl_get_system_efield", "ovl_set_system_u1_flux", "ovl_set_system_modes", "ovl_get_system_modes",
-    "ovl_system_observables", "ovl_ensemble_balance", "ovl_gauss_restore", "ovl_qed_eigenbasis",
+    "ovl_system_observables", "ovl_observables_noalloc", "ovl_ensemble_balance", "ovl_gauss_restore", "ovl_qed_eigenbasis",
     "ovl_save_configuration", "ovl_load_configuration",
     "ovl_get_timing", "ovl_set_profiling", "ovl_autotune", "ovl_get_tuning", "ovl_set_tuning",
 ]
@@ -98,6 +98,8 @@ def lib():
             if name not in ("ovl_error", "ovl_zolotarev_coefficients", "ovl_sign_defaults"):
                 f.restype = C.c_int
         _lib.ovl_sign_defaults.restype = None
+        _lib.ovl_observables_noalloc.argtypes = [C.c_int, C.POINTER(C.c_int), C.c_void_p, C.c_void_p, C.c_void_p,
+                                                 C.c_void_p]
     return _lib
 
 
@@ -134,6 +136,16 @@ def zolotarev_coefficients(amin, amax, errtol=1e-12, no=25):
     eps = lib().ovl_zolotarev_coefficients(C.c_double(amin), C.c_double(amax), C.c_double(errtol), C.c_int(no),
                                            _d(bz), _d(dz))
     return eps, bz, dz
+
+
+def observables_noalloc(nc, n, links_dev: int, efield_dev: int, obs_dev: int, stream: int = 0) -> None:
+    """ovl_observables_noalloc: device addresses of the links and E (reference layouts) and of
+    OVL_OBS_COUNT output doubles; queued on `stream` (0 = default stream), no device allocation."""
+    dims = (C.c_int * 3)(*n)
+    rc = lib().ovl_observables_noalloc(int(nc), dims, C.c_void_p(links_dev), C.c_void_p(efield_dev),
+                                       C.c_void_p(obs_dev), C.c_void_p(stream))
+    if rc != OK:
+        raise OvlError(rc, lib().ovl_error(None).decode())
 
 
 def device_name() -> str:

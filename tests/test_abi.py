@@ -75,6 +75,24 @@ def test_create_ensemble_rejects_bad_params_without_touching_device():
     assert ovl.lib().ovl_create(C.byref(p), C.byref(C.c_void_p())) == ovl.E_ARG
 
 
+def test_noalloc_observables_reject_bad_shapes_without_touching_device():
+    # the argument checks come before any HIP call; the (never dereferenced) addresses are dummies
+    dummy = 256
+
+    def call(nc, n, ptr=dummy):
+        return ovl.lib().ovl_observables_noalloc(nc, (C.c_int * 3)(*n), C.c_void_p(ptr), C.c_void_p(ptr),
+                                                 C.c_void_p(ptr), None)
+    assert call(4, (4, 4, 4)) == ovl.E_ARG
+    assert b"nc must be 2 or 3" in ovl.lib().ovl_error(None)
+    assert call(2, (4, 4, 4), ptr=0) == ovl.E_ARG
+    assert call(3, (4, 2, 4)) == ovl.E_ARG
+    assert b">= 3" in ovl.lib().ovl_error(None)
+    assert call(2, (1024, 1024, 128)) == ovl.E_ARG  # 2^27 sites, beyond the 2^26 maximum
+    assert b"exceed the maximum of 67108864" in ovl.lib().ovl_error(None)
+    with pytest.raises(ovl.OvlError, match="maximum"):
+        ovl.observables_noalloc(3, (512, 512, 512), dummy, dummy, dummy)
+
+
 @pytest.mark.skipif(bool(os.environ.get("HIP_VISIBLE_DEVICES")) or os.path.exists("/dev/kfd"),
                     reason="a GPU may be present")
 def test_create_reports_missing_device():
