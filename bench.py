@@ -611,15 +611,30 @@ def main():
                  "ms_per_step_with_guard": round(ms_per_step + check_ms / max(every, 1), 2)}
         log(f"guard: {json.dumps(guard)}")
 
-    # --- Chern-Simons rate and axial-charge history (currents from the leapfrog's own overlap apply)
+    # --- the workload's full run from its initial state (config steps, mixed form, graph replay):
+    # dN_CS/dt after every step, integrated by the trapezoid rule into the Chern-Simons number
+    # history N_CS(t) - N_CS(0), and the axial charge every 5 steps (currents from the leapfrog's own
+    # overlap apply)
     observables = None
     if c.kind == "overlap" and not args.quick:
+        ctx.set_links(links)
+        ctx.set_modes(ovl.PSI, psi)
+        ctx.set_modes(ovl.PSIMID, psi)
         so_obs = step_opts(c, prec_mixed, True, args.tol, args.chunk)
+        rate0 = ctx.ncs_dot()
         t0 = time.perf_counter()
-        q5, ncs, _, _ = ctx.fermion_update_observed(so_obs, 10, every=5)
-        observables = {"steps": 10, "every": 5, "axial_charge": [float(v) for v in q5],
-                       "ncs_dot": [float(v) for v in ncs], "seconds": round(time.perf_counter() - t0, 3)}
-        log(f"observed run: axial charge {observables['axial_charge']}, dN_CS/dt {observables['ncs_dot']}")
+        q5, rate, _, _ = ctx.fermion_update_observed(so_obs, c.steps, every=1)
+        secs = time.perf_counter() - t0
+        rates = np.concatenate([[rate0], rate])
+        ncs = np.concatenate([[0.0], np.cumsum(0.5 * c.dt * (rates[1:] + rates[:-1]))])
+        every = 5
+        observables = {"steps": c.steps, "dt": c.dt, "every": every, "t": [round(k * c.dt, 6) for k in range(0, c.steps + 1, every)],
+                       "ncs": [float(v) for v in ncs[::every]], "ncs_dot": [float(v) for v in rates[::every]],
+                       "axial_charge": [float(v) for v in q5[every - 1::every]],
+                       "axial_charge_t": [round(k * c.dt, 6) for k in range(every, c.steps + 1, every)],
+                       "seconds": round(secs, 3), "ms_per_step": round(1e3 * secs / c.steps, 2)}
+        log(f"observed run ({c.steps} steps, {observables['ms_per_step']} ms/step): N_CS(t) - N_CS(0) "
+            f"{observables['ncs'][-1]:.4e}, axial charge {observables['axial_charge'][-1]:.4e}")
 
     # --- plain FP64 form
     fp64 = None

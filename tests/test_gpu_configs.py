@@ -212,23 +212,30 @@ def test_config2_su3_wilson_32_all_200_steps(gpu):
     assert ctx.unitarity_violation()[0] < 1e-12
 
 
-def test_config3_su3_overlap_24_ten_steps(gpu):
-    # ten leapfrog steps of the bench workload (256 modes, degree 384, 8 flux quanta): mixed
-    # against FP64 for every mode, and the leapfrog invariant
+def test_config3_su3_overlap_24_all_100_steps(gpu):
+    # the bench workload's full run (256 modes, degree 384, 8 flux quanta, 100 steps at dt 0.02) with
+    # its observables after every step: mixed against FP64 for every mode at the end, the leapfrog
+    # invariant, the dN_CS/dt history (gauge fields only: the same bits in both runs) and the axial
+    # charge history every 5 steps
     c = cf.CONFIGS["su3_overlap_24"]
-    steps = 10
-    print(f"[case] {c.name} {steps} steps", flush=True)
+    print(f"[case] {c.name} {c.steps} steps", flush=True)
     runs = []
     for prec in (ovl.PREC_FP64, ovl.PREC_MIXED):
         ctx, links, e, psi = setup(c)
-        ctx.fermion_update(step_opts(c, prec), steps)
-        runs.append((ctx.get_modes(ovl.PSI), ctx.get_modes(ovl.PSIMID), stagger_invariant(ctx)))
+        q5, rate, _, _ = ctx.fermion_update_observed(step_opts(c, prec), c.steps, every=1)
+        runs.append((ctx.get_modes(ovl.PSI), ctx.get_modes(ovl.PSIMID), stagger_invariant(ctx), q5, rate))
         ctx.close()
-    (p64, m64, i64), (pmx, mmx, imx) = runs
+    (p64, m64, i64, q64, r64), (pmx, mmx, imx, qmx, rmx) = runs
     err = max(max(np.linalg.norm(a - b) / np.linalg.norm(b) for a, b in zip(x, y)) for x, y in ((pmx, p64), (mmx, m64)))
-    print(f"  mixed vs fp64 after {steps} steps over 256 modes: {err:.2e}", flush=True)
+    dq = np.abs(qmx - q64).max()
+    print(f"  mixed vs fp64 after {c.steps} steps over 256 modes: {err:.2e}; axial charge {q64[4::5]} "
+          f"(largest mixed-fp64 difference {dq:.2e}); dN_CS/dt {r64[4::5]}", flush=True)
     assert err < 1e-10
     assert np.abs(i64 - 1).max() < 1e-12 and np.abs(imx - 1).max() < 1e-10
+    assert len(r64) == c.steps and np.array_equal(r64, rmx) and np.isfinite(r64).all()
+    # each mode adds a bilinear of norm <= (1 + |h_ov| / 2) |psi|^2 / 2V <= 4 / 2V to the charge (unit
+    # modes, |h_ov| <= 2 rho): a relative error of 1e-10 per mode moves it by at most 4e-10 M / 2V
+    assert dq <= 1e-10 * 4 * c.nmodes / (2 * c.volume)
 
 
 def test_config2_su3_wilson_32_fp32_all_200_steps(gpu):
