@@ -36,7 +36,9 @@ enum {
     OVL_E_STATE = -3,    /* call order (e.g. no links uploaded) */
     OVL_E_CONVERGE = -4, /* iterative solver hit its iteration cap */
     OVL_E_NOMEM = -5,    /* device allocation failed */
-    OVL_E_SPECTRUM = -6  /* invalid spectral interval (lmin <= 0 or lmin >= lmax) */
+    OVL_E_SPECTRUM = -6, /* invalid spectral interval (lmin <= 0 or lmin >= lmax) */
+    OVL_E_NONFINITE = -7 /* a leapfrog call left Inf/NaN in Psi or PsiMid (a step far beyond the
+                            leapfrog's stability bound dt |omega| < 2) */
 };
 
 /* fields held on the device: the reference's Fermions::Psi and Fermions::PsiMid

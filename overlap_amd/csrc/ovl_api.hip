@@ -2267,6 +2267,34 @@ int ovl_fermion_update(ovl_ctx *c, const ovl_step_opts *o, int nsteps)
     return update_impl(c, o, nsteps, false);
 }
 
+// Psi and PsiMid after a leapfrog call: a call whose fields turned non-finite (a step far beyond the
+// leapfrog's stability bound) fails instead of handing Inf/NaN back silently.  One reduction per
+// field and call.
+static int check_fields_finite(ovl_ctx *c)
+{
+    const long n = (long)(c->field64_bytes / sizeof(double));
+    const int nb = ovl::eig_partial_blocks();
+    std::vector<double> h((size_t)nb);
+    for (int f = 0; f < 2; ++f) {
+        HIPCHK(c, ovl::launch_nonfinite(c->field[f], n, c->partial, c->st));
+        HIPCHK(c, hipMemcpyAsync(h.data(), c->partial, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->st));
+        HIPCHK(c, hipStreamSynchronize(c->st));
+        for (double v : h)
+            if (v != 0.0)
+                return fail(c, OVL_E_NONFINITE, std::string(f == 0 ? "Psi" : "PsiMid") +
+                                                    " holds Inf/NaN after the leapfrog (dt beyond its stability bound?)");
+    }
+    return OVL_OK;
+}
+
+// ||h_ov|| <= M (|1 + m/2| + |1 - m/2|) for |eps| <= 1 (OverlapU.cpp:188-201); the staggered leapfrog
+// keeps the mode norms bounded only while dt ||h|| < 2
+static double overlap_norm_bound(const ovl_ctx *c)
+{
+    const double m = c->prm.m_fermion, M = std::fabs(c->prm.m_dw);
+    return M * (std::fabs(1.0 + 0.5 * m) + std::fabs(1.0 - 0.5 * m));
+}
+
 // nsteps leapfrog steps; with observe_last the last step runs with stream launches and leaves
 // -i h_ov Psi in the scratch field
 static int update_impl(ovl_ctx *c, const ovl_step_opts *o, int nsteps, bool observe_last)
@@ -2297,6 +2325,9 @@ static int update_impl(ovl_ctx *c, const ovl_step_opts *o, int nsteps, bool obse
         // run with the ladder as tightened
         oe.sign = guarded_opts(c, &o->sign);
         if (guard_due(c, &o->sign)) RETCHK(guard_check(c, &o->sign, c->field[OVL_PSI], c->step_xmax, &oe.sign));
+        // FP16 storage is scaled for mode norms within twice their start: beyond the leapfrog's
+        // stability bound (dt ||h_ov|| >= 2, norms growing) the ladder stops at FP32
+        if (std::fabs(o->dt) * overlap_norm_bound(c) >= 2.0) oe.sign.fp16_tail = 0;
         o = &oe;
         RETCHK(make_plan(c, &o->sign, pl, c->step_xmax));
     }
@@ -2311,7 +2342,8 @@ static int update_impl(ovl_ctx *c, const ovl_step_opts *o, int nsteps, bool obse
         begin_timing(c);
         RETCHK(one_step(c, o, true));
         c->guard_since += 1;
-        return end_timing(c);
+        RETCHK(end_timing(c));
+        return check_fields_finite(c);
     }
     // FP32 Wilson: the call steps the FP32 copies of Psi and PsiMid and writes them back at the end
     if (w32)
@@ -2322,7 +2354,8 @@ static int update_impl(ovl_ctx *c, const ovl_step_opts *o, int nsteps, bool obse
         if (w32)
             for (int f = 0; f < 2; ++f)
                 HIPCHK(c, ovl::launch_convert(c->g, c->g.nblk, c->f32[f], P32, c->field[f], P64, c->st));
-        return end_timing(c);
+        RETCHK(end_timing(c));
+        return nsteps > 0 ? check_fields_finite(c) : OVL_OK;
     };
     const bool graph = o->use_graph && !(o->kind == OVL_OVERLAP && o->sign.method == OVL_SIGN_ZOLOTAREV) && nsteps > 0;
     if (!graph) {

@@ -12,7 +12,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libovl.so")
 
-OK, E_ARG, E_HIP, E_STATE, E_CONVERGE, E_NOMEM, E_SPECTRUM = 0, -1, -2, -3, -4, -5, -6
+OK, E_ARG, E_HIP, E_STATE, E_CONVERGE, E_NOMEM, E_SPECTRUM, E_NONFINITE = 0, -1, -2, -3, -4, -5, -6, -7
 PSI, PSIMID, SCRATCH = 0, 1, 2
 SIGN_CHEBYSHEV, SIGN_ZOLOTAREV = 0, 1
 PREC_FP64, PREC_FP32, PREC_MIXED = 0, 1, 2

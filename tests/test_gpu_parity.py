@@ -460,6 +460,33 @@ def test_error_paths(gpu):
     assert ex.value.code == ovl.E_ARG
 
 
+def test_leapfrog_beyond_its_stability_bound(gpu):
+    # dt |omega| >> 2: the staggered leapfrog amplifies a mode by about dt |omega| per half step, so
+    # after 40 steps at dt = 1e4 (|omega| up to 6) the norms pass the double range; the call reports
+    # it instead of handing Inf back (every kernel still runs to its end: the host chose the inputs)
+    print("[case] leapfrog beyond the stability bound", flush=True)
+    ctx, L, ucc, links, e, psi = make(2, (4, 4, 4), 16, 808, M=0.0)
+    with ctx:
+        so = ovl.StepOpts(ovl.WILSON, 1e4, ovl.GAUGE_FROZEN, ovl.sign_opts(precision=ovl.PREC_FP64), 0)
+        with pytest.raises(ovl.OvlError) as ex:
+            ctx.fermion_update(so, 40)
+        assert ex.value.code == ovl.E_NONFINITE
+        ctx.set_modes(ovl.PSI, psi)
+        ctx.set_modes(ovl.PSIMID, psi)
+        so.dt = 0.05
+        ctx.fermion_update(so, 2)  # a stable step afterwards works as before
+    # the mixed overlap ladder stores its tail in FP16 scaled for norms within twice their start:
+    # beyond dt ||h_ov|| = 2 (||h_ov|| <= 2 rho at m = 0) it stops at FP32
+    ctx, *_ = make(3, (4, 4, 4), 16, 809, M=1.2, nb=1)
+    with ctx:
+        o = ovl.sign_opts(degree=64, cut=0.1, precision=ovl.PREC_MIXED, tol=1e-8)
+        ctx.fermion_update(ovl.StepOpts(ovl.OVERLAP, 0.02, ovl.GAUGE_FROZEN, o, 0), 1)
+        assert ctx.timing().fp16_steps > 0
+        ctx.fermion_update(ovl.StepOpts(ovl.OVERLAP, 1.0, ovl.GAUGE_FROZEN, o, 0), 1)
+        t = ctx.timing()
+        assert t.fp16_steps == 0 and t.fp32_steps > 0
+
+
 @pytest.mark.parametrize("nc,n,nm,mb", [(3, (4, 4, 4), 64, 64), (2, (8, 4, 6), 32, 16), (3, (6, 4, 4), 64, 32),
                                         (3, (6, 4, 4), 128, 16), (2, (8, 4, 6), 128, 16), (3, (4, 4, 8), 512, 64)])
 def test_traversals_are_permutations(gpu, nc, n, nm, mb):
