@@ -11,6 +11,7 @@ FP64 form and the CPU oracle are timed beside it.  Prints one JSON line on stdou
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import json
 import os
 import csv
@@ -319,35 +320,70 @@ def bench_other_workloads(tol: float):
                 f"{VARIANT_NAMES[var]} {gbps:.0f} GB/s")
         ctx.close()
         if c.nc == 2:
-            out[name]["cpu_reference"] = reference_cpu(c, links, psi0)
+            out[name]["cpu_reference"] = reference_cpu(c, links, psi0, default_cpu_threads())
     return out
 
 
-def reference_cpu(c: cf.Config, links, psi):
-    """The reference's own operator on the host (oracle/_ref, built from the reference's SU(2)
-    sources), one core: WilsonEvoOperator::Compute (WilsonOperator.cpp:30) or OverlapU with its
-    Zolotarev coefficients (OverlapOperator.cpp:25-64 without the unused random vector) on the
-    interval the Chebyshev sign uses, timed per mode.  A leapfrog step applies it twice per mode
-    (OverlapUpdate.cpp:9-58); the reference's four Ritz solves per step are not counted."""
+def reference_cpu(c: cf.Config, links, psi, threads: int):
+    """The reference's own code on the host (oracle/_ref, built from the reference's SU(2) sources):
+    WilsonEvoOperator::Compute (WilsonOperator.cpp:30) or OverlapU with its Zolotarev coefficients
+    (OverlapOperator.cpp:25-64 without the unused random vector) on the interval the Chebyshev sign
+    uses.  Modes are independent (the reference's MPI decomposition, ModePartition.cpp:12-25), so
+    the sample of modes runs on `threads` host threads at once (the functions keep no state between
+    calls); a leapfrog step applies the operator twice per mode (OverlapUpdate.cpp:9-58).  The
+    overlap step also makes four Ritz solves (OverlapUpdate.cpp:61-97, 99-135), serial, on the whole
+    lattice: each iteration one WilsonUU; the MaxEValue solve (sign -1) runs all 2000 iterations, its
+    stopping test never firing (RitzSolver.cpp:250), the MinEValue one at least one.  Their cost is
+    the measured time of 20 iterations scaled to 2 x 2000 per step (a lower bound of the four)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import orc
+    from concurrent.futures import ThreadPoolExecutor
     if not orc.ref_available():
         return None
     L = orc.lat(c.nc, c.n, c.r_wilson, c.m_fermion, c.m_dw)
     ucc = orc.links_to_ucc(L, links, orc.u1_phases(c.n, c.nb, c.q) if c.nb else None)
-    k, t0 = 0, time.perf_counter()
-    while k < len(psi) and (k == 0 or time.perf_counter() - t0 < 5.0):
+    lam = orc.norm_bound(L)
+    orc.ref_set(L)  # the reference's globals (lattice size, masses), set once before the threads start
+    R, uc = orc.ref(), orc.cz(ucc)
+    ins = [orc.cz(p) for p in psi]
+    outs = [np.zeros_like(ins[0]) for _ in range(2 * threads)]  # distinct per concurrent call
+
+    def apply(k):
+        i = k % len(ins)
+        o = outs[k % len(outs)]
         if c.kind == "overlap":
-            lam = orc.norm_bound(L)
-            orc.ref_overlap_evo(L, ucc, psi[k], (c.cheb_cut * lam) ** 2, lam ** 2)
+            R.ref_overlap_evo(orc.dp(uc), orc.dp(ins[i]), orc.dp(o), C.c_double((c.cheb_cut * lam) ** 2),
+                              C.c_double(lam ** 2))
         else:
-            orc.ref_wilson_evo(L, ucc, psi[k])
-        k += 1
-    per_mode = (time.perf_counter() - t0) / k
-    fn = "OverlapU (Zolotarev, 25 poles, multi-shift CG)" if c.kind == "overlap" else "WilsonEvoOperator::Compute"
-    return {"kind": "reference", "function": fn, "cores": 1, "modes_timed": k,
-            "seconds_per_mode_apply": round(per_mode, 5), "unit": "site*mode updates/s",
-            "value": c.volume / (2 * per_mode)}
+            R.ref_wilson_evo(orc.dp(uc), orc.dp(ins[i]), orc.dp(o))
+
+    apply(0)  # first call outside the clock
+    done, t0 = 0, time.perf_counter()
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        while done == 0 or (time.perf_counter() - t0 < 6.0 and done < 64 * threads):
+            list(ex.map(apply, range(done, done + threads)))
+            done += threads
+    wall = time.perf_counter() - t0
+    apply_wall = wall / done  # wall seconds per mode apply with `threads` running at once
+    out = {"kind": "reference", "cores": threads, "modes_timed": done, "unit": "site*mode updates/s",
+           "function": "OverlapU (Zolotarev, 25 poles, multi-shift CG)" if c.kind == "overlap"
+           else "WilsonEvoOperator::Compute",
+           "wall_seconds_per_mode_apply": round(apply_wall, 6)}
+    step = 2 * c.nmodes * apply_wall
+    if c.kind == "overlap":
+        g = cf.rng(7)
+        v = (g.normal(size=psi[0].size) + 1j * g.normal(size=psi[0].size))
+        v = v / np.linalg.norm(v)
+        t1 = time.perf_counter()
+        orc.ref_ritz(L, ucc, v.reshape(-1), delitr=1e-16, nitlz=20, sign=-1.0, nrestart=100)
+        per_it = (time.perf_counter() - t1) / 20
+        out["ritz_seconds_per_iteration"] = round(per_it, 6)
+        out["ritz_seconds_per_step_lower_bound"] = round(2 * 2000 * per_it, 3)
+        out["value_without_ritz"] = c.volume * c.nmodes / step
+        step += 2 * 2000 * per_it
+    out["seconds_per_step"] = round(step, 3)
+    out["value"] = c.volume * c.nmodes / step
+    return out
 
 
 def bench_next_rows(c: cf.Config, links, e):
