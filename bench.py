@@ -453,7 +453,8 @@ def main():
     ap.add_argument("--modes", type=int, default=0, help="override the mode count (test runs only)")
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--chunk", type=int, default=0, help="mode blocks per recurrence pass (0 = all)")
-    ap.add_argument("--mb", type=int, default=0, help="modes per SoA block (16/32/64, 0 = choose)")
+    ap.add_argument("--mb", type=int, default=-1,
+                    help="modes per SoA block (16/32/64; 0 = by the mode count; -1 = timed on this lattice)")
     ap.add_argument("--traversal", type=int, default=0, help="0 = XCD boxes, 1 = lexicographic")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
@@ -538,6 +539,10 @@ def main():
     tuning = tuning_string(ctx)
     res["autotune"] = tune.split(";")[-1].strip()  # the choice; every candidate's time is in the log
     res["tuning"] = tuning
+    mbm = re.search(r"mode block:((?: \d+ [0-9.]+ us)+), chosen (\d+)", tune)
+    if mbm:  # the timed mode block (--mb -1): each size's Clenshaw step time, and the choice
+        res["mode_block_timing"] = mbm.group(0)
+        args.mb = int(mbm.group(2))  # the counter passes below run the chosen block size
 
     # --- warmup + timed region: mixed form, per-launch HIP events on the library stream
     so = step_opts(c, prec_mixed, False, args.tol, args.chunk)
