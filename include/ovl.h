@@ -245,7 +245,8 @@ int ovl_system_observables(ovl_ctx *ctx, int sys, double *obs);
  * takes no context: `links` and `efield` are DEVICE arrays in the host layouts above (nc colours,
  * lattice n[0] x n[1] x n[2]), `obs` a DEVICE array of OVL_OBS_COUNT doubles that receives the
  * values ovl_system_observables returns for the same fields, bit for bit.  The kernels are queued on
- * `stream` (a hipStream_t; NULL = the default stream) and the call returns without waiting.  It
+ * `stream` (a hipStream_t of the current device; NULL = its default stream) and the call returns
+ * without waiting.  It
  * calls no allocator: the reductions' partial results live in a static 32 KiB array per device,
  * shared by every call on that device, so calls that overlap on one device (other streams, other
  * threads) race and must be ordered by the caller.  At most OVL_NOALLOC_MAX_SITES sites; larger
