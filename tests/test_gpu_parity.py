@@ -239,6 +239,27 @@ def test_sign_residual_and_mode_chunks(gpu):
     assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("nc,nb", [(2, 0), (3, 2)])
+def test_sign_residual_vs_oracle(gpu, nc, nb):
+    # OverlapSignTest (OverlapSignTest.cpp:155-165): max over modes of ||eps(eps x) - x|| / ||x||, on the
+    # GPU and from the oracle's sign applied twice to the same inputs.  A low degree makes the
+    # residual the polynomial's own error (1e-5 .. 1e-3 here), far above the rounding of either side
+    print(f"[case] sign residual nc={nc} nb={nb}", flush=True)
+    n = (4, 4, 3)
+    ctx, L, ucc, *_, psi = make(nc, n, 8, 61 + nc, M=1.2, nb=nb)
+    deg, cut = 40, 0.1
+    r = ctx.sign_residual(ovl.PSI, ovl.sign_opts(degree=deg, cut=cut, precision=ovl.PREC_FP64))
+    coef, lam = orc.cheb_coeffs(deg, cut), orc.norm_bound(L)
+    want = 0.0
+    for k in range(8):
+        e1 = orc.sign_cheb(L, ucc, lam, cut, deg, coef, psi[k])
+        e2 = orc.sign_cheb(L, ucc, lam, cut, deg, coef, e1)
+        want = max(want, np.linalg.norm(e2 - psi[k]) / np.linalg.norm(psi[k]))
+    print(f"  gpu {r:.6e} oracle {want:.6e}", flush=True)
+    assert 1e-7 < want < 1e-1
+    assert abs(r - want) <= 1e-9 * want
+
+
 def test_zolotarev_overlap_batched_vs_oracle(gpu):
     n = (4, 4, 4)
     ctx, L, ucc, *_, psi = make(2, n, 20, 91, m=0.1, M=1.0, mb=16)
