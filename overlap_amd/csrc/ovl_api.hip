@@ -2535,6 +2535,8 @@ int ovl_system_observables(ovl_ctx *c, int sys, double *obs)
 // The reductions' per-workgroup partial results (ovl::reduce_partial_len doubles) and the four raw
 // results live in this file-scope device array instead of a context's buffers.  Each device holds
 // its own copy; host code uses only the address hipGetSymbolAddress returns for the current device.
+// Every reduction writes all the partial results its finishing pass reads (a fixed grid), so the
+// array needs no clearing between calls; calls that overlap on one device share it and race.
 constexpr size_t NOALLOC_SCRATCH_DOUBLES = 4096;  // 32 KiB per device
 constexpr size_t NOALLOC_RAW = NOALLOC_SCRATCH_DOUBLES - 8;  // raw results: scratch[NOALLOC_RAW .. +7)
 static __device__ __align__(256) double scratch[NOALLOC_SCRATCH_DOUBLES];
