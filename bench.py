@@ -751,6 +751,9 @@ def main():
                       "traversal": args.traversal, "parallelism": "single GPU"},
            "roofline": roofline, "cpu_baseline": cpu, "fp64_gpu": fp64,
            "speedup_vs_fp64": (value / fp64["value"]) if fp64 else None,
+           # a leapfrog step applies the overlap operator twice to every mode (each a full-lattice apply)
+           "overlap_applies_per_s": (2.0 * value / c.volume) if c.kind == "overlap" else None,
+           "fp64_overlap_applies_per_s": (2.0 * fp64["value"] / c.volume) if (fp64 and c.kind == "overlap") else None,
            "graph_ms_per_step": graph_ms, "eager": eager, "timing": timing, "accuracy": res, "guard": guard,
            "observables": observables, "ensemble": ens,
            "other_workloads": other, "next_rows": nxt}
