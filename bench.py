@@ -512,18 +512,23 @@ def main():
     if args.quick:
         args.no_fp64 = args.no_cpu = args.no_ensemble = args.no_traffic = args.no_other = args.no_next = True
     if c.kind == "overlap" and not args.quick:
+        # every mode of one overlap apply, mixed against FP64 (the FP64 result held on the host)
         ctx.apply_overlap(ovl.PSI, ovl.SCRATCH, so64)
-        ref = ctx.get_modes(ovl.SCRATCH, 0, min(16, c.nmodes))
+        ref = ctx.get_modes(ovl.SCRATCH)
         ctx.apply_overlap(ovl.PSI, ovl.SCRATCH, som)
-        mix = ctx.get_modes(ovl.SCRATCH, 0, min(16, c.nmodes))
         tm = ctx.timing()
-        res["rel_err_mixed_vs_fp64"] = float(max(np.linalg.norm(a - b) / np.linalg.norm(b) for a, b in zip(mix, ref)))
+        worst = 0.0
+        for f in range(0, c.nmodes, 32):
+            mix = ctx.get_modes(ovl.SCRATCH, f, min(32, c.nmodes - f))
+            worst = max(worst, max(np.linalg.norm(a - b) / np.linalg.norm(b) for a, b in zip(mix, ref[f:f + 32])))
+        res["rel_err_mixed_vs_fp64"] = float(worst)
+        res["rel_err_modes"] = c.nmodes
         res["fp64_steps"] = int(tm.fp64_steps)
         res["fp48_steps"] = int(tm.fp48_steps)
         res["fp32_steps"] = int(tm.fp32_steps)
         res["fp16_steps"] = int(tm.fp16_steps)
         del ref, mix
-        log(f"mixed vs fp64 rel err {res['rel_err_mixed_vs_fp64']:.3e}  steps fp64 {tm.fp64_steps} "
+        log(f"mixed vs fp64 rel err (all {c.nmodes} modes) {res['rel_err_mixed_vs_fp64']:.3e}  steps fp64 {tm.fp64_steps} "
             f"48-bit {tm.fp48_steps} fp32 {tm.fp32_steps} fp16 {tm.fp16_steps}")
 
     # --- stencil autotune for this shape (or a fixed tuning)
