@@ -97,6 +97,11 @@ typedef struct {
                             the last check, or when at least `guard` leapfrog steps (applies) went
                             by.  The tightening is kept by the context until new links are set.
                             Default 100. */
+    int relax;           /* MIXED with guard: levels (0..8) by which a check may loosen the ladder on
+                            this lattice, each planned for 4x the tol of the last: a level whose
+                            check stays within tol/2 with 4x its error to spare is tried one looser
+                            on the same block and kept while that check stays within tol/2; later
+                            checks tighten again as above.  Default 0 (never loosen). */
 } ovl_sign_opts;
 
 typedef struct {

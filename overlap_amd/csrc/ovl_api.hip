@@ -978,15 +978,18 @@ bool guard_due(const ovl_ctx *c, const ovl_sign_opts *o)
     const ovl_sign_opts &g = c->guard_opts;
     const bool same = g.degree == o->degree && g.cut == o->cut && g.tol == o->tol && g.head == o->head &&
                       g.fp16_tail == o->fp16_tail && g.head16 == o->head16 && g.fp48 == o->fp48 &&
-                      g.head48 == o->head48 && g.precision == o->precision && g.method == o->method;
+                      g.head48 == o->head48 && g.precision == o->precision && g.method == o->method &&
+                      g.relax == o->relax;
     return c->guard_fresh || !same || c->guard_since >= o->guard;
 }
 
-// the options a guarded mixed sign runs with at the context's tightening level: level 0 as
-// requested; level >= 1 without head overrides (the ladder follows tol); level l >= 2 planned for
-// tol / 16^(l - 1) (each tier starts about 26 steps earlier at the geometric decay of the Clenshaw
-// intermediates); the last level is plain FP64
+// the options a guarded mixed sign runs with at the context's level: level 0 as requested; level >= 1
+// without head overrides (the ladder follows tol); level l >= 2 planned for tol / 16^(l - 1) (each
+// tier starts about 26 steps earlier at the geometric decay of the Clenshaw intermediates); the
+// last level is plain FP64.  Negative levels (only with o->relax) loosen: level -l is planned for
+// tol 4^l (each tier about 13 steps later).
 constexpr int GUARD_LEVELS = 6;
+constexpr double RELAX_KEEP = 0.5;  // a loosened ladder is kept while the check stays within tol / 2
 ovl_sign_opts guarded_opts(const ovl_ctx *c, const ovl_sign_opts *o)
 {
     ovl_sign_opts e = *o;
@@ -995,7 +998,8 @@ ovl_sign_opts guarded_opts(const ovl_ctx *c, const ovl_sign_opts *o)
     e.head16 = 0;
     e.head48 = 0;
     const double tol = o->tol > 0 ? o->tol : 1e-10;
-    e.tol = tol * std::pow(16.0, -(double)(c->guard_level - 1));
+    if (c->guard_level < 0) e.tol = tol * std::pow(4.0, -(double)c->guard_level);
+    else e.tol = tol * std::pow(16.0, -(double)(c->guard_level - 1));
     if (c->guard_level >= GUARD_LEVELS) e.precision = OVL_PREC_FP64;
     return e;
 }
@@ -1024,6 +1028,12 @@ int guard_check(ovl_ctx *c, const ovl_sign_opts *o, const void *in64, double xma
     c->profiling = false;
     const double tol = o->tol > 0 ? o->tol : 1e-10;
     int r = OVL_OK;
+    // loosening (o->relax > 0): a level that passes with 4x its error still within tol / 2 is tried
+    // one level looser, on the same block; the looser level is kept if its check stays within
+    // tol / 2, else the context returns to the level that passed
+    bool trying = false;
+    int passed_level = c->guard_level;
+    double passed_err = 0.0;
     for (;;) {
         *eff = guarded_opts(c, o);
         if (eff->precision != OVL_PREC_MIXED) break;  // plain FP64: nothing left to compare
@@ -1050,7 +1060,22 @@ int guard_check(ovl_ctx *c, const ovl_sign_opts *o, const void *in64, double xma
         if (r != OVL_OK) break;
         c->guard_checks++;
         c->guard_err = ok ? worst : INFINITY;
-        if (ok && worst <= tol) break;
+        if (trying && !(ok && worst <= RELAX_KEEP * tol)) {  // the looser ladder misses: back
+            c->guard_level = passed_level;
+            c->guard_err = passed_err;
+            *eff = guarded_opts(c, o);
+            break;
+        }
+        if (ok && worst <= (trying ? RELAX_KEEP : 1.0) * tol) {
+            if (o->relax > 0 && c->guard_level <= 0 && c->guard_level > -o->relax && 4.0 * worst <= RELAX_KEEP * tol) {
+                trying = true;
+                passed_level = c->guard_level;
+                passed_err = worst;
+                c->guard_level--;
+                continue;
+            }
+            break;
+        }
         c->guard_level++;
         // level 1 only drops head overrides: without any it is level 0 again
         if (c->guard_level == 1 && o->head < 0 && o->head16 <= 0 && o->head48 <= 0) c->guard_level = 2;
@@ -1831,6 +1856,7 @@ static int check_sign_opts(ovl_ctx *c, const ovl_sign_opts *o)
     if (o->method == OVL_SIGN_ZOLOTAREV && o->precision != OVL_PREC_FP64)
         return fail(c, OVL_E_ARG, "the Zolotarev sign runs in FP64 only");
     if (o->method == OVL_SIGN_ZOLOTAREV) RETCHK(single_lattice(c, "the Zolotarev sign"));
+    if (o->relax < 0 || o->relax > 8) return fail(c, OVL_E_ARG, "relax must be 0..8 levels");
     return OVL_OK;
 }
 
@@ -2197,7 +2223,7 @@ static int fermion_half(ovl_ctx *c, const ovl_step_opts *o, int src, int tgt, vo
 }
 
 // every field of the step options; a new option must be added here (the sizes below catch it)
-static_assert(sizeof(ovl_sign_opts) == 80 && sizeof(ovl_step_opts) == 112, "compare every option in same_step_opts");
+static_assert(sizeof(ovl_sign_opts) == 88 && sizeof(ovl_step_opts) == 120, "compare every option in same_step_opts");
 static bool same_step_opts(const ovl_step_opts &a, const ovl_step_opts &b)
 {
     const ovl_sign_opts &x = a.sign, &y = b.sign;
@@ -2205,7 +2231,7 @@ static bool same_step_opts(const ovl_step_opts &a, const ovl_step_opts &b)
            x.method == y.method && x.precision == y.precision && x.degree == y.degree && x.cut == y.cut &&
            x.tol == y.tol && x.head == y.head && x.lmin == y.lmin && x.lmax == y.lmax &&
            x.mode_chunk == y.mode_chunk && x.fp16_tail == y.fp16_tail && x.head16 == y.head16 &&
-           x.fp48 == y.fp48 && x.head48 == y.head48 && x.guard == y.guard;
+           x.fp48 == y.fp48 && x.head48 == y.head48 && x.guard == y.guard && x.relax == y.relax;
 }
 
 static StepKey make_step_key(const ovl_ctx *c, const ovl_step_opts *o, const SignPlan *pl)

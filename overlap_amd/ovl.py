@@ -54,7 +54,7 @@ class SignOpts(C.Structure):
     _fields_ = [("method", C.c_int), ("precision", C.c_int), ("degree", C.c_int), ("cut", C.c_double),
                 ("tol", C.c_double), ("head", C.c_int), ("lmin", C.c_double), ("lmax", C.c_double),
                 ("mode_chunk", C.c_int), ("fp16_tail", C.c_int), ("head16", C.c_int),
-                ("fp48", C.c_int), ("head48", C.c_int), ("guard", C.c_int)]
+                ("fp48", C.c_int), ("head48", C.c_int), ("guard", C.c_int), ("relax", C.c_int)]
 
 
 class StepOpts(C.Structure):

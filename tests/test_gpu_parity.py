@@ -809,6 +809,57 @@ def test_guard_falls_back_to_fp64_below_the_cut(gpu):
     print(f"  checks {t.guard_checks}, level {t.guard_level}, last check {t.guard_err:.2e}", flush=True)
     assert t.guard_level == 6 and t.guard_checks == 5 and t.guard_err > 1e-10
     assert np.array_equal(ctx.get_modes(ovl.SCRATCH), ref)
+    # allowed to loosen, the guard still ends in FP64 here: loosening starts only from a level that
+    # passes
+    ctx.set_links(links)
+    ctx.apply_overlap(ovl.PSI, ovl.SCRATCH,
+                      ovl.sign_opts(degree=deg, cut=cut, precision=ovl.PREC_MIXED, tol=1e-10, relax=2))
+    t = ctx.timing()
+    assert t.guard_level == 6
+    assert np.array_equal(ctx.get_modes(ovl.SCRATCH), ref)
+
+
+def test_guard_loosens_within_half_tol(gpu):
+    # relax > 0: a check whose error leaves 4x to spare below tol / 2 tries the ladder of 4 tol on the
+    # same block and keeps it while that check stays within tol / 2.  On 6^3 SU(2) at degree 256
+    # the tol ladder's error is 4e-12, the looser one's 2e-11 (a level further would need 8e-11):
+    # one level.  Every mode of the apply, and of a 3-step leapfrog after it, stays within tol of
+    # the FP64 form; without relax the ladder stays at the tol level.
+    print("[case] guard loosening", flush=True)
+    n, nc, nm, deg, tol = (6, 6, 6), 2, 64, 256, 1e-10
+    ctx, L, ucc, links, e, psi = make(nc, n, nm, 32, sigma=0.25, M=1.2)
+    with ctx:
+        ctx.apply_sign(ovl.PSI, ovl.PSIMID, ovl.sign_opts(degree=deg, precision=ovl.PREC_FP64))
+        ref = ctx.get_modes(ovl.PSIMID)
+        o0 = ovl.sign_opts(degree=deg, precision=ovl.PREC_MIXED, tol=tol)
+        ctx.apply_sign(ovl.PSI, ovl.SCRATCH, o0)
+        t0 = ctx.timing()
+        assert t0.guard_level == 0 and t0.guard_checks == 1
+        o2 = ovl.sign_opts(degree=deg, precision=ovl.PREC_MIXED, tol=tol, relax=2)
+        ctx.apply_sign(ovl.PSI, ovl.SCRATCH, o2)  # new options: the check is due again
+        t = ctx.timing()
+        mix = ctx.get_modes(ovl.SCRATCH)
+        err = max(np.linalg.norm(a - b) / np.linalg.norm(b) for a, b in zip(mix, ref))
+        print(f"  level {t.guard_level}, checks {t.guard_checks}, last check {t.guard_err:.2e}, all modes "
+              f"{err:.2e}, fp16 steps {t0.fp16_steps} -> {t.fp16_steps}", flush=True)
+        assert -2 <= t.guard_level <= -1 and t.guard_err <= 0.5 * tol and err <= 0.5 * tol
+        assert t.fp16_steps > t0.fp16_steps and t.fp64_steps < t0.fp64_steps
+        # the leapfrog with the loosened ladder against the FP64 leapfrog from the same state
+        runs = []
+        for prec, relax in ((ovl.PREC_FP64, 0), (ovl.PREC_MIXED, 2)):
+            ctx.set_links(links)
+            ctx.set_modes(ovl.PSI, psi)
+            ctx.set_modes(ovl.PSIMID, psi)
+            so = ovl.sign_opts(degree=deg, precision=prec, tol=tol, relax=relax)
+            ctx.fermion_update(ovl.StepOpts(ovl.OVERLAP, 0.02, ovl.GAUGE_LINKS, so, 1), 3)
+            runs.append((ctx.get_modes(ovl.PSI), ctx.timing().guard_level))
+        err = max(np.linalg.norm(a - b) / np.linalg.norm(b) for a, b in zip(runs[1][0], runs[0][0]))
+        print(f"  leapfrog: level {runs[1][1]}, mixed vs fp64 after 3 steps {err:.2e}", flush=True)
+        assert runs[1][1] < 0 and err <= tol
+    with pytest.raises(ovl.OvlError):
+        with ovl.Context(nc, n, nm) as c2:
+            c2.set_links(links)
+            c2.apply_sign(ovl.PSI, ovl.PSIMID, ovl.sign_opts(precision=ovl.PREC_MIXED, relax=9))
 
 
 def test_timed_mode_block(gpu):
