@@ -479,6 +479,24 @@ def test_error_paths(gpu):
     with pytest.raises(ovl.OvlError) as ex:
         ctx.fermion_update_observed(so, 2, 1)
     assert ex.value.code == ovl.E_ARG
+    # the single-mode drop-ins check their host inputs for Inf/NaN (on the device, after the
+    # upload): every such input is refused, and the next call is unaffected
+    g = cf.rng(3)
+    V = 64
+    ucc = (g.standard_normal((3 * V, 2, 2)) + 1j * g.standard_normal((3 * V, 2, 2))) / 2
+    phi = g.standard_normal(V * 8) + 1j * g.standard_normal(V * 8)
+    good = ctx.wilson_evo_compute(ucc, phi)
+    for which, idx in (("ucc", (3 * V - 1, 1, 0)), ("ucc", (0, 0, 1)), ("phi", (17,)), ("phi", (V * 8 - 1,))):
+        for v in (np.nan, np.inf, complex(0.0, -np.inf), complex(0.0, np.nan)):
+            u, p = ucc.copy(), phi.copy()
+            (u if which == "ucc" else p)[idx] = v
+            with pytest.raises(ovl.OvlError, match=f"non-finite input \\({'u_cc' if which == 'ucc' else 'phi'}\\)") as ex:
+                ctx.wilson_evo_compute(u, p)
+            assert ex.value.code == ovl.E_ARG
+    with pytest.raises(ovl.OvlError) as ex:
+        ctx.overlap_operator_compute(ucc, np.where(np.arange(V * 8) == 3, np.nan, phi), 0.01, 4.0)
+    assert ex.value.code == ovl.E_ARG
+    assert np.array_equal(ctx.wilson_evo_compute(ucc, phi), good)
 
 
 def test_leapfrog_beyond_its_stability_bound(gpu):
