@@ -172,23 +172,32 @@ def test_config4_ensemble_full_size(gpu):
 
 
 def test_config1_su2_overlap_16_all_50_steps(gpu):
-    # the workload's full run: 50 leapfrog steps of 64 modes, mixed against FP64 for every mode
-    # (the links move with E fixed and do not depend on the modes, so the two runs see the same
-    # operators and differ only by the mixed form's rounding, accumulated over 100 overlap applies)
+    # the workload's full run: 50 leapfrog steps of 64 modes, the mixed form (the guard only
+    # tightening, and allowed to loosen by two levels) against FP64 for every mode (the links move
+    # with E fixed and do not depend on the modes, so the runs see the same operators and differ
+    # only by the mixed form's rounding, accumulated over 100 overlap applies); the sign residual
+    # ||eps(eps psi) - psi|| of the final modes
     c = cf.CONFIGS["su2_overlap_16"]
     print(f"[case] {c.name} {c.steps} steps", flush=True)
     runs = []
-    for prec in (ovl.PREC_FP64, ovl.PREC_MIXED):
+    for prec, relax in ((ovl.PREC_FP64, 0), (ovl.PREC_MIXED, 0), (ovl.PREC_MIXED, 2)):
         ctx, links, e, psi = setup(c)
-        ctx.fermion_update(step_opts(c, prec), c.steps)
-        runs.append((ctx, ctx.get_modes(ovl.PSI), ctx.get_modes(ovl.PSIMID)))
-    (c64, p64, m64), (cmx, pmx, mmx) = runs
-    err = max(max(np.linalg.norm(a - b) / np.linalg.norm(b) for a, b in zip(x, y)) for x, y in ((pmx, p64), (mmx, m64)))
-    print(f"  mixed vs fp64 after {c.steps} steps: {err:.2e}", flush=True)
-    assert err < 1e-10
-    assert np.array_equal(cmx.get_links(), c64.get_links())
-    assert np.abs(stagger_invariant(cmx) - 1).max() < 1e-10
+        ctx.fermion_update(step_opts(c, prec, relax=relax), c.steps)
+        runs.append((ctx, ctx.get_modes(ovl.PSI), ctx.get_modes(ovl.PSIMID), int(ctx.timing().guard_level)))
+    c64, p64, m64, _ = runs[0]
     assert np.abs(stagger_invariant(c64) - 1).max() < 1e-12
+    for relax, (cmx, pmx, mmx, level) in zip((0, 2), runs[1:]):
+        err = max(max(np.linalg.norm(a - b) / np.linalg.norm(b) for a, b in zip(x, y))
+                  for x, y in ((pmx, p64), (mmx, m64)))
+        so = step_opts(c, ovl.PREC_MIXED, relax=relax).sign
+        r = cmx.sign_residual(ovl.PSI, so)
+        print(f"  relax {relax} (guard level {level}): mixed vs fp64 after {c.steps} steps: {err:.2e}; "
+              f"sign residual {r:.2e}", flush=True)
+        assert err < 1e-10
+        assert r < 1e-9
+        assert np.array_equal(cmx.get_links(), c64.get_links())
+        assert np.abs(stagger_invariant(cmx) - 1).max() < 1e-10
+        assert level <= 0 if relax == 0 else level >= -relax
 
 
 def test_config2_su3_wilson_32_all_200_steps(gpu):
