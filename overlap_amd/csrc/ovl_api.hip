@@ -288,11 +288,12 @@ void *blk_ptr(void *base, int prec, size_t blk, const ovl_ctx *c)
     return (char *)base + blk * c->blk32_elems * 16;
 }
 
+// the next event of the pool (nullptr when no event can be created: that launch goes untimed)
 hipEvent_t next_event(ovl_ctx *c)
 {
     if (c->evused == c->evpool.size()) {
-        hipEvent_t e;
-        hipEventCreate(&e);
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
         c->evpool.push_back(e);
     }
     return c->evpool[c->evused++];
@@ -322,13 +323,17 @@ int hop(ovl_ctx *c, const Geom &g, int pin, int pout, const HopArgs &a)
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (c->profiling && !c->capturing) {
         e0 = next_event(c);
-        e1 = next_event(c);
-        c->evvar.resize(c->evused / 2);
-        c->evvar[c->evused / 2 - 1] = var;
-        hipEventRecord(e0, c->st);
+        e1 = e0 ? next_event(c) : nullptr;
+        if (e1) {
+            c->evvar.resize(c->evused / 2);
+            c->evvar[c->evused / 2 - 1] = var;
+            HIPCHK(c, hipEventRecord(e0, c->st));
+        } else if (e0) {
+            c->evused--;  // an unpaired event stays unused
+        }
     }
     HIPCHK(c, ovl::launch_hop(g, pin, pout, a, c->wpe[pin], c->st));
-    if (e1) hipEventRecord(e1, c->st);
+    if (e1) HIPCHK(c, hipEventRecord(e1, c->st));
     const double b = hop_alg_bytes(c, a, pin, pout);
     c->hop_launches++;
     c->hop_bytes += b;
@@ -1290,8 +1295,7 @@ static int ctx_init(ovl_ctx *c)
     c->sys_nb.assign(c->sys.size(), 0);
     c->sys_links_set.assign(c->sys.size(), 0);
     if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) return OVL_E_HIP;
-    hipEventCreate(&c->t0);
-    hipEventCreate(&c->t1);
+    if (hipEventCreate(&c->t0) != hipSuccess || hipEventCreate(&c->t1) != hipSuccess) return OVL_E_HIP;
     const size_t L = (size_t)c->link_sites;
     const size_t esz = sizeof(double) * n_adj(c) * 3 * L;
     RETCHK(dalloc(c, (void **)&c->links, sizeof(double) * link_reals(c) * 3 * L));
@@ -1932,8 +1936,6 @@ int ovl_sign_residual(ovl_ctx *c, int in, const ovl_sign_opts *o, double *rel)
     if (!c || !rel) return OVL_E_ARG;
     RETCHK(check_field(c, in));
     RETCHK(check_sign_opts(c, o));
-    const int f1 = (in == OVL_SCRATCH) ? OVL_PSI : OVL_SCRATCH;
-    (void)f1;
     void *t1 = nullptr, *t2 = nullptr;
     RETCHK(dalloc(c, &t1, c->field64_bytes));
     if (dalloc(c, &t2, c->field64_bytes) != OVL_OK) {
@@ -1947,7 +1949,10 @@ int ovl_sign_residual(ovl_ctx *c, int in, const ovl_sign_opts *o, double *rel)
     if (r == OVL_OK) r = sign_driver(c, o, t1, t2, FIN_SIGN, 0.0, 2.0 * xmax);
     std::vector<cplx> dx, dd;
     if (r == OVL_OK) {
-        ovl::launch_lincomb(c->g, c->g.nblk, P64, t2, t2, 1, 0, c->field[in], -1, 0, c->st);
+        const hipError_t e = ovl::launch_lincomb(c->g, c->g.nblk, P64, t2, t2, 1, 0, c->field[in], -1, 0, c->st);
+        if (e != hipSuccess) r = fail(c, OVL_E_HIP, std::string("sign residual: ") + hipGetErrorString(e));
+    }
+    if (r == OVL_OK) {
         r = mode_dot_all(c, t2, t2, dd);
         if (r == OVL_OK) r = mode_dot_all(c, c->field[in], c->field[in], dx);
     }
@@ -3185,7 +3190,7 @@ int ovl_qed_eigenbasis(ovl_ctx *c, const ovl_sign_opts *o, int nb, double q, dou
     std::vector<ovl::EigMode> em((size_t)nm);
     const double norm = std::sqrt((double)V);
     for (int sm = 0; sm < nm; ++sm) {
-        const long g = (long)first_mode + sm, particle = g % 2, level = (g / 2) % (2L * V * nc);
+        const long gm = (long)first_mode + sm, particle = gm % 2, level = (gm / 2) % (2L * V * nc);
         const long b = particle == 0 ? D / 2 - 1 - level : D / 2 + level;
         const Ent &e = ent[b];
         const cplx rot = std::exp(cplx(0, -0.5 * e.w * dt));

@@ -812,7 +812,6 @@ __global__ __launch_bounds__(TPB) void k_soa_to_aos(const double2 *__restrict__ 
 // ---------------------------------------------------------------------------------------------
 struct cd { double r, i; };
 __device__ __forceinline__ cd cmul(cd a, cd b) { return {a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r}; }
-__device__ __forceinline__ cd cmulc(cd a, cd b) { return {a.r * b.r + a.i * b.i, a.i * b.r - a.r * b.i}; }  // a*conj(b)
 __device__ __forceinline__ cd cadd(cd a, cd b) { return {a.r + b.r, a.i + b.i}; }
 
 template <int NC> struct Mat { cd e[NC * NC]; };
