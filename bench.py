@@ -262,7 +262,7 @@ def wilson_step_alg_bytes(c: cf.Config, real_bytes: int) -> float:
     return 2 * (3 * S + Lb / c.nmodes) * c.volume * c.nmodes
 
 
-def bench_other_workloads(tol: float, relax: int = 0):
+def bench_other_workloads(tol: float, relax: int = 0, cpu: bool = True):
     """The other three single-lattice workloads, timed the same way as the main one (inputs
     resident, autotuned stencil, HIP events around every stencil launch): the SU(2) Wilson
     leapfrog in FP64, the SU(2) overlap in the mixed form, the SU(3) 32^3 Wilson leapfrog with FP32
@@ -320,7 +320,7 @@ def bench_other_workloads(tol: float, relax: int = 0):
             log(f"{key}: {r['ms_per_step']:.2f} ms/step {r['value']:.4e} site*mode/s, "
                 f"{VARIANT_NAMES[var]} {gbps:.0f} GB/s")
         ctx.close()
-        if c.nc == 2:
+        if c.nc == 2 and cpu:
             out[name]["cpu_reference"] = reference_cpu(c, links, psi0, default_cpu_threads())
     return out
 
@@ -474,6 +474,7 @@ def main():
     ap.add_argument("--no-other", action="store_true", help="skip the other three single-lattice workloads")
     ap.add_argument("--no-next", action="store_true", help="skip the Gauss restore / files / eigenbasis timings")
     ap.add_argument("--next-only", action="store_true", help="only the Gauss restore / files / eigenbasis timings")
+    ap.add_argument("--other-only", action="store_true", help="only the other three single-lattice workloads")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -482,6 +483,9 @@ def main():
     if args.ensemble_only:
         r = bench_ensemble(cf.ENSEMBLES["su2_overlap_ensemble"], args.steps, args.warmup, args.tol, args.relax)
         print(json.dumps(r))
+        return
+    if args.other_only:
+        print(json.dumps({"other_workloads": bench_other_workloads(args.tol, args.relax, not args.no_cpu)}))
         return
 
     c = cf.CONFIGS[args.workload]
@@ -732,7 +736,7 @@ def main():
 
     other = None
     if not args.no_other:
-        other = bench_other_workloads(args.tol, args.relax)
+        other = bench_other_workloads(args.tol, args.relax, not args.no_cpu)
 
     nxt = None
     if not args.no_next:

@@ -1631,11 +1631,12 @@ static bool all_finite(const double *p, size_t n)
     return true;
 }
 
-// *ok = no Inf/NaN among the n doubles at device address x (one reduction, then a synchronisation)
-static int device_finite(ovl_ctx *c, const void *x, long n, bool *ok)
+// *ok = no Inf/NaN among the n doubles at device address x, nor at y when given (one streaming
+// reduction over both, then a synchronisation)
+static int device_finite(ovl_ctx *c, const void *x, const void *y, long n, bool *ok)
 {
     std::vector<double> h((size_t)ovl::eig_partial_blocks());
-    HIPCHK(c, ovl::launch_nonfinite(x, n, c->partial, c->st));
+    HIPCHK(c, ovl::launch_nonfinite(x, y, n, c->partial, c->st));
     HIPCHK(c, hipMemcpyAsync(h.data(), c->partial, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     *ok = true;
@@ -1978,7 +1979,7 @@ static int dropin_setup(ovl_ctx *c, const double *ucc, const double *phi)
     bool ok = true;
     RETCHK(stage(c, std::max(nu, np) * sizeof(double)));
     HIPCHK(c, hipMemcpyAsync(c->staging, ucc, nu * sizeof(double), hipMemcpyHostToDevice, c->st));
-    RETCHK(device_finite(c, c->staging, (long)nu, &ok));
+    RETCHK(device_finite(c, c->staging, nullptr, (long)nu, &ok));
     if (!ok) return fail(c, OVL_E_ARG, "non-finite input (u_cc)");
     HIPCHK(c, ovl::launch_ucc_to_links(c->g, c->staging, c->ul64, c->ul32, c->st));
     c->stencil_dirty = true;  // the next batched call re-derives the stencil from the master links
@@ -1987,7 +1988,7 @@ static int dropin_setup(ovl_ctx *c, const double *ucc, const double *phi)
     HIPCHK(c, hipMemsetAsync(c->field[OVL_SCRATCH], 0, c->blk64_elems * 16, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     HIPCHK(c, hipMemcpyAsync(c->staging, phi, np * sizeof(double), hipMemcpyHostToDevice, c->st));
-    RETCHK(device_finite(c, c->staging, (long)np, &ok));
+    RETCHK(device_finite(c, c->staging, nullptr, (long)np, &ok));
     if (!ok) return fail(c, OVL_E_ARG, "non-finite input (phi)");
     HIPCHK(c, ovl::launch_aos_to_soa(g1, c->staging, 0, 1, c->field[OVL_SCRATCH], c->st));
     return OVL_OK;
@@ -2322,13 +2323,10 @@ int ovl_fermion_update(ovl_ctx *c, const ovl_step_opts *o, int nsteps)
 static int check_fields_finite(ovl_ctx *c)
 {
     const long n = (long)(c->field64_bytes / sizeof(double));
-    for (int f = 0; f < 2; ++f) {
-        bool ok = true;
-        RETCHK(device_finite(c, c->field[f], n, &ok));
-        if (!ok)
-            return fail(c, OVL_E_NONFINITE, std::string(f == 0 ? "Psi" : "PsiMid") +
-                                                " holds Inf/NaN after the leapfrog (dt beyond its stability bound?)");
-    }
+    bool ok = true;
+    RETCHK(device_finite(c, c->field[OVL_PSI], c->field[OVL_PSIMID], n, &ok));
+    if (!ok)
+        return fail(c, OVL_E_NONFINITE, "Psi or PsiMid holds Inf/NaN after the leapfrog (dt beyond its stability bound?)");
     return OVL_OK;
 }
 
@@ -3159,7 +3157,7 @@ int ovl_qed_eigenbasis(ovl_ctx *c, const ovl_sign_opts *o, int nb, double q, dou
             if (!std::isfinite(w[i])) why = "zheevd: non-finite eigenvalue";
     }
     if (!why.empty()) return fail(c, OVL_E_CONVERGE, "eigensolver failed: " + why);
-    HIPCHK(c, ovl::launch_nonfinite(dA, (long)n2 * Dz * Dz * 2, c->partial, c->st));
+    HIPCHK(c, ovl::launch_nonfinite(dA, nullptr, (long)n2 * Dz * Dz * 2, c->partial, c->st));
     double bad = 0;
     RETCHK(partial_max(1, 0, &bad));
     if (bad != 0) return fail(c, OVL_E_CONVERGE, "eigensolver failed: non-finite eigenvector entries");

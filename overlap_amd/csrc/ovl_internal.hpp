@@ -170,7 +170,9 @@ hipError_t launch_eig_sources(const Geom &g, void *field, long k0, int cnt, long
 hipError_t launch_eig_gather(const Geom &g, const void *out, void *A, long k0, int nsb, hipStream_t st);
 hipError_t launch_eig_leak(const Geom &g, const void *field, int nvalid, double *partial, hipStream_t st);
 hipError_t launch_eig_hermitize(void *A, long dz, int n2, double *partial, hipStream_t st);
-hipError_t launch_nonfinite(const void *x, long n, double *partial, hipStream_t st);
+// partial[0..eig_partial_blocks()) = 1 in blocks that saw an Inf/NaN among the n doubles of x, or of
+// y when y is not null (the two arrays are read in one pass); x and y 16-byte aligned
+hipError_t launch_nonfinite(const void *x, const void *y, long n, double *partial, hipStream_t st);
 // one requested mode of the eigenbasis: blk >= 0: f exp(i k_blk z) chi(x, y, spin) on colour 0, chi
 // the eigenvector in column idx of z-momentum block blk; blk = -1: the four spin components v at
 // site idx, colour col >= 1.  The second field holds the mode times rot.

@@ -1844,12 +1844,44 @@ __global__ __launch_bounds__(TPB) void k_eig_modes(const EigMode *__restrict__ m
     }
 }
 
-// 1 in partial[blockIdx.x] when one of the n doubles is not finite
-__global__ __launch_bounds__(TPB) void k_nonfinite(const double *__restrict__ x, long n, double *partial)
+// Inf or NaN: the 11 exponent bits all set (read from the high word of each double)
+__device__ __forceinline__ unsigned nonfinite_bits(d2v v)
 {
-    double v[1] = {0.0};
-    for (long i = (long)blockIdx.x * TPB + threadIdx.x; i < n; i += (long)gridDim.x * TPB)
-        if (!(fabs(x[i]) <= 1.7976931348623157e308)) v[0] = 1.0;
+    const unsigned hx = (unsigned)((unsigned long long)__double_as_longlong(v.x) >> 32);
+    const unsigned hy = (unsigned)((unsigned long long)__double_as_longlong(v.y) >> 32);
+    return (unsigned)((hx & 0x7ff00000u) == 0x7ff00000u) | (unsigned)((hy & 0x7ff00000u) == 0x7ff00000u);
+}
+
+// 1 in partial[blockIdx.x] when one of the n doubles of a, or of b (when given), is not finite.  A
+// streaming pass over one index range (b's 16-byte units follow a's): four 16-byte loads in flight
+// per lane, read once (non-temporal).
+__global__ __launch_bounds__(TPB) void k_nonfinite(const double *__restrict__ a, const double *__restrict__ b, long n,
+                                                   double *partial)
+{
+    constexpr int U = 4;
+    const long n2 = n / 2;                // whole double2 units per array
+    const long total = b ? 2 * n2 : n2;   // units of a, then of b
+    const d2v *__restrict__ a2 = reinterpret_cast<const d2v *>(a);
+    const d2v *__restrict__ b2 = reinterpret_cast<const d2v *>(b);
+    const long stride = (long)gridDim.x * TPB;
+    long i = (long)blockIdx.x * TPB + threadIdx.x;
+    unsigned bad = 0;
+    for (; i + (U - 1) * stride < total; i += U * stride) {
+        d2v v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const long j = i + u * stride;
+            v[u] = __builtin_nontemporal_load(j < n2 ? a2 + j : b2 + (j - n2));
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) bad |= nonfinite_bits(v[u]);
+    }
+    for (; i < total; i += stride) bad |= nonfinite_bits(i < n2 ? a2[i] : b2[i - n2]);
+    if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) {  // an odd last double of each array
+        bad |= nonfinite_bits(d2v{a[n - 1], 0.0});
+        if (b) bad |= nonfinite_bits(d2v{b[n - 1], 0.0});
+    }
+    double v[1] = {bad ? 1.0 : 0.0};
     const int ism[1] = {1};
     block_reduce<1>(v, ism);
     if (threadIdx.x == 0) partial[blockIdx.x] = v[0];
@@ -2116,9 +2148,10 @@ hipError_t launch_eig_hermitize(void *A, long dz, int n2, double *partial, hipSt
     return hipGetLastError();
 }
 
-hipError_t launch_nonfinite(const void *x, long n, double *partial, hipStream_t st)
+hipError_t launch_nonfinite(const void *x, const void *y, long n, double *partial, hipStream_t st)
 {
-    k_nonfinite<<<RED_BLOCKS, TPB, 0, st>>>((const double *)x, n, partial);
+    if (!x || n < 0) return hipErrorInvalidValue;
+    k_nonfinite<<<RED_BLOCKS, TPB, 0, st>>>((const double *)x, (const double *)y, n, partial);
     return hipGetLastError();
 }
 

@@ -499,6 +499,43 @@ def test_error_paths(gpu):
     assert np.array_equal(ctx.wilson_evo_compute(ucc, phi), good)
 
 
+def test_nonfinite_check_finds_every_position(gpu):
+    # the device check streams 512 x 256 lanes over 16-byte units, four units per lane in flight, then
+    # a remainder loop: on 28^3 SU(3) the u_cc input of the drop-in (592 704 units) runs both loops, so
+    # one bad double is planted at the ends of each (and in both halves of a unit)
+    print("[case] non-finite check positions", flush=True)
+    n = (28, 28, 28)
+    V = n[0] * n[1] * n[2]
+    ctx = ovl.Context(3, n, 1)
+    g = cf.rng(4)
+    ucc = (g.standard_normal((3 * V, 3, 3)) + 1j * g.standard_normal((3 * V, 3, 3))) / 3
+    phi = g.standard_normal(V * 12) + 1j * g.standard_normal(V * 12)
+    good = ctx.wilson_evo_compute(ucc, phi)
+    flat = ucc.reshape(-1)
+    lanes = 512 * 256
+    units = flat.size  # one complex = one 16-byte unit
+    assert units > 4 * lanes
+    for k in (0, 1, lanes - 1, lanes, 4 * lanes - 1, 4 * lanes, units - lanes, units - 1):
+        for v in (complex(np.nan, 0.0), complex(0.0, np.inf)):
+            u = flat.copy()
+            u[k] = v
+            with pytest.raises(ovl.OvlError, match="non-finite input") as ex:
+                ctx.wilson_evo_compute(u.reshape(ucc.shape), phi)
+            assert ex.value.code == ovl.E_ARG
+    for k in (0, phi.size // 2, phi.size - 1):
+        p = phi.copy()
+        p[k] = complex(-np.inf, 1.0)
+        with pytest.raises(ovl.OvlError, match="non-finite input") as ex:
+            ctx.wilson_evo_compute(ucc, p)
+        assert ex.value.code == ovl.E_ARG
+    # the largest finite doubles are finite
+    big = ucc.copy()
+    big.reshape(-1)[7] = complex(np.finfo(float).max, -np.finfo(float).max)
+    ctx.wilson_evo_compute(big, phi)
+    assert np.array_equal(ctx.wilson_evo_compute(ucc, phi), good)
+    ctx.close()
+
+
 def test_leapfrog_beyond_its_stability_bound(gpu):
     # dt |omega| >> 2: the staggered leapfrog amplifies a mode by about dt |omega| per half step, so
     # after 40 steps at dt = 1e4 (|omega| up to 6) the norms pass the double range; the call reports
