@@ -175,7 +175,7 @@ def measure_traffic(args, c: cf.Config, prefix: str, n_last: int, tuning: str):
     return res
 
 
-def bench_ensemble(ec: cf.EnsembleConfig, steps: int, warmup: int, tol: float, relax: int = 0):
+def bench_ensemble(ec: cf.EnsembleConfig, steps: int, warmup: int, tol: float, relax: int = 0, graph: bool = True):
     """Many SU(2) overlap systems of uneven sizes in one context (one stencil launch per Clenshaw
     step over all systems): mixed and FP64 timing from the same start (hipGraph replay of the
     step, captured in an untimed warmup step), per-system error of the mixed form against the FP64
@@ -218,9 +218,9 @@ def bench_ensemble(ec: cf.EnsembleConfig, steps: int, warmup: int, tol: float, r
         # new links make the mixed sign's accuracy guard due: its check runs here, untimed (in a
         # run it recurs every 100 steps)
         ens.apply_overlap(ovl.PSI, ovl.SCRATCH, opts(prec).sign)
-        ens.fermion_update(opts(prec, True), 1)  # untimed warmup step: captures the step graph
+        ens.fermion_update(opts(prec, graph), 1)  # untimed warmup step: captures the step graph
         t0 = time.perf_counter()
-        ens.fermion_update(opts(prec, True), steps)
+        ens.fermion_update(opts(prec, graph), steps)
         wall = time.perf_counter() - t0
         psi = [ens.get_system_modes(k, ovl.PSI) for k in range(ec.n_systems)]
         # the stencil's rate: one eager step with HIP events around every stencil launch
@@ -491,14 +491,15 @@ def main():
     ap.add_argument("--next-only", action="store_true", help="only the Gauss restore / files / eigenbasis timings")
     ap.add_argument("--other-only", action="store_true", help="only the other three single-lattice workloads")
     ap.add_argument("--no-graph", action="store_true",
-                    help="the other workloads' runs as eager stream launches (for kernel-trace profiling)")
+                    help="the ensemble's and the other workloads' runs as eager stream launches (for kernel traces)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
 
     if args.ensemble_only:
-        r = bench_ensemble(cf.ENSEMBLES["su2_overlap_ensemble"], args.steps, args.warmup, args.tol, args.relax)
+        r = bench_ensemble(cf.ENSEMBLES["su2_overlap_ensemble"], args.steps, args.warmup, args.tol, args.relax,
+                           not args.no_graph)
         print(json.dumps(r))
         return
     if args.other_only:
@@ -750,7 +751,7 @@ def main():
 
     ens = None
     if not args.no_ensemble:
-        ens = bench_ensemble(cf.ENSEMBLES["su2_overlap_ensemble"], 10, 1, args.tol, args.relax)
+        ens = bench_ensemble(cf.ENSEMBLES["su2_overlap_ensemble"], 10, 1, args.tol, args.relax, not args.no_graph)
 
     other = None
     if not args.no_other:
