@@ -88,9 +88,10 @@ typedef struct {
                             top 48 bits of the double, rounded; FP64 arithmetic), 0 = plain FP64 */
     int head48;          /* MIXED with fp48: first 48-bit step (> 0 overrides tol; 0 = from tol) */
     int guard;           /* MIXED: a-posteriori accuracy guard (0 = off).  A check takes one mode
-                            block of the input (Psi for the leapfrog; a different block each time)
-                            through the mixed sign and through the FP64 sign; while a mode's relative
-                            difference exceeds tol, the ladder is tightened a level (head overrides
+                            block of the input (Psi for the leapfrog; a different block each time;
+                            in an ensemble the one flat block of every system) through the mixed
+                            sign and through the FP64 sign; while a mode's relative difference (per
+                            system in an ensemble) exceeds tol, the ladder is tightened a level (head overrides
                             dropped, then tol divided by 16 per level, at last plain FP64) and
                             checked again.  A check runs before the first step of an update call (or
                             before an apply) when the links or these ladder options changed since
@@ -100,7 +101,7 @@ typedef struct {
     int relax;           /* MIXED with guard: levels (0..8) by which a check may loosen the ladder on
                             this lattice, each planned for 4x the tol of the last: a level whose
                             check stays within tol/2 with 4x its error to spare is tried one looser
-                            on the same block and kept while that check stays within tol/2; later
+                            on every mode block and kept while that check stays within tol/2; later
                             checks tighten again as above.  Default 0 (never loosen). */
 } ovl_sign_opts;
 
@@ -311,9 +312,11 @@ typedef struct {
     double var_bytes[5];
     int graph_captures;       /* hipGraph captures the call made (a graph-mode ovl_fermion_update
                                  whose step is unchanged since the previous one replays that graph: 0) */
-    int guard_checks;         /* accuracy-guard checks the call made (each: one mode block, mixed and
-                                 FP64 sign; their kernels are not counted above) */
-    int guard_level;          /* tightening of the mixed ladder after the call (0 = as requested) */
+    int guard_checks;         /* accuracy-guard checks the call made (each: one mode block, or all of
+                                 them for a loosening try, mixed and FP64 sign; their kernels are
+                                 not counted above) */
+    int guard_level;          /* tightening of the mixed ladder after the call (0 = as requested;
+                                 negative: loosened by that many levels) */
     double guard_err;         /* largest relative mixed-vs-FP64 difference of the last check */
 } ovl_timing;
 int ovl_get_timing(ovl_ctx *ctx, ovl_timing *t);

@@ -86,6 +86,7 @@ struct ovl_ctx {
     ovl_sign_opts guard_opts{};
     double guard_err = 0.0;
     void *guard_buf[2] = {nullptr, nullptr};
+    size_t guard_bytes = 0;  // size of each guard buffer
 
     // Clenshaw work buffers (sized for `work_blocks` mode blocks)
     int work_blocks = 0;
@@ -1019,10 +1020,21 @@ int guard_check(ovl_ctx *c, const ovl_sign_opts *o, const void *in64, double xma
     *eff = guarded_opts(c, o);
     if (!guarded(o)) return OVL_OK;
     const int blk = c->guard_next++ % c->g.nblk;
-    Geom g1 = c->g;
-    g1.nblk = 1;
-    for (int f = 0; f < 2; ++f)
-        if (!c->guard_buf[f]) RETCHK(dalloc(c, &c->guard_buf[f], c->blk64_elems * 16));
+    const bool multi = nsys(c) > 1;  // an ensemble: its one flat block holds every system
+    // the check's two buffers, grown on demand (one mode block, or all of them for a loosening try)
+    auto ensure_buffers = [&](size_t bytes) -> int {
+        if (c->guard_bytes >= bytes) return OVL_OK;
+        HIPCHK(c, hipStreamSynchronize(c->st));
+        for (int f = 0; f < 2; ++f) {
+            if (c->guard_buf[f]) HIPCHK(c, hipFree(c->guard_buf[f]));
+            c->guard_buf[f] = nullptr;
+        }
+        c->guard_bytes = 0;
+        for (int f = 0; f < 2; ++f) RETCHK(dalloc(c, &c->guard_buf[f], bytes));
+        c->guard_bytes = bytes;
+        return OVL_OK;
+    };
+    RETCHK(ensure_buffers(c->blk64_elems * 16));
     const long long hl = c->hop_launches;
     const double hb = c->hop_bytes;
     long long vl[NVAR];
@@ -1034,8 +1046,9 @@ int guard_check(ovl_ctx *c, const ovl_sign_opts *o, const void *in64, double xma
     const double tol = o->tol > 0 ? o->tol : 1e-10;
     int r = OVL_OK;
     // loosening (o->relax > 0): a level that passes with 4x its error still within tol / 2 is tried
-    // one level looser, on the same block; the looser level is kept if its check stays within
-    // tol / 2, else the context returns to the level that passed
+    // one level looser, on every mode block (a looser ladder must hold for all modes, not only for
+    // the block in turn); it is kept if that check stays within tol / 2, else the context returns to
+    // the level that passed.  Errors are per mode, and in an ensemble per mode of each system.
     bool trying = false;
     int passed_level = c->guard_level;
     double passed_err = 0.0;
@@ -1044,16 +1057,27 @@ int guard_check(ovl_ctx *c, const ovl_sign_opts *o, const void *in64, double xma
         if (eff->precision != OVL_PREC_MIXED) break;  // plain FP64: nothing left to compare
         ovl_sign_opts o64 = *eff;
         o64.precision = OVL_PREC_FP64;
-        r = sign_driver(c, eff, in64, c->guard_buf[0], FIN_SIGN, 0.0, xmax, nullptr, blk, 1, true);
-        if (r == OVL_OK) r = sign_driver(c, &o64, in64, c->guard_buf[1], FIN_SIGN, 0.0, xmax, nullptr, blk, 1, true);
+        const bool all = trying && c->g.nblk > 1;
+        const int b0 = all ? 0 : blk, nb = all ? c->g.nblk : 1;
+        Geom gk = c->g;
+        gk.nblk = nb;
+        r = ensure_buffers(c->blk64_elems * 16 * (size_t)nb);
+        if (r == OVL_OK)
+            r = sign_driver(c, eff, in64, c->guard_buf[0], FIN_SIGN, 0.0, xmax, nullptr, b0, nb, true);
+        if (r == OVL_OK) r = sign_driver(c, &o64, in64, c->guard_buf[1], FIN_SIGN, 0.0, xmax, nullptr, b0, nb, true);
         if (r == OVL_OK) {
-            const hipError_t e = ovl::launch_lincomb(g1, 1, P64, c->guard_buf[0], c->guard_buf[0], 1.0, 0.0,
+            const hipError_t e = ovl::launch_lincomb(gk, nb, P64, c->guard_buf[0], c->guard_buf[0], 1.0, 0.0,
                                                      c->guard_buf[1], -1.0, 0.0, c->st);
             if (e != hipSuccess) r = fail(c, OVL_E_HIP, std::string("guard: ") + hipGetErrorString(e));
         }
         std::vector<cplx> dd, yy;
-        if (r == OVL_OK) r = mode_dot(c, g1, c->guard_buf[0], c->guard_buf[0], dd);
-        if (r == OVL_OK) r = mode_dot(c, g1, c->guard_buf[1], c->guard_buf[1], yy);
+        if (multi) {
+            if (r == OVL_OK) r = mode_dot_all(c, c->guard_buf[0], c->guard_buf[0], dd);
+            if (r == OVL_OK) r = mode_dot_all(c, c->guard_buf[1], c->guard_buf[1], yy);
+        } else {
+            if (r == OVL_OK) r = mode_dot(c, gk, c->guard_buf[0], c->guard_buf[0], dd);
+            if (r == OVL_OK) r = mode_dot(c, gk, c->guard_buf[1], c->guard_buf[1], yy);
+        }
         if (r != OVL_OK) break;
         double worst = 0.0;
         bool ok = true;

@@ -94,6 +94,14 @@ def test_ensemble_sign_fp64_and_mixed(gpu, nc):
         emix = max(np.linalg.norm(mix[j] - f64[j]) / np.linalg.norm(psi[j]) for j in range(nm))
         print(f"  system {k} {n}: mixed vs fp64 {emix:.2e}", flush=True)
         assert emix < 1e-10
+    # the accuracy guard's check ran on every mode of every system, each mode against its own FP64
+    # norm within its own system: its error is the apply's largest per-system, per-mode error
+    worst = max(max(np.linalg.norm(a - b) / np.linalg.norm(b)
+                    for a, b in zip(ens.get_system_modes(k, ovl.SCRATCH), ens.get_system_modes(k, ovl.PSIMID)))
+                for k in range(len(sy)))
+    print(f"  guard: checks {t.guard_checks}, level {t.guard_level}, error {t.guard_err:.4e} (apply {worst:.4e})",
+          flush=True)
+    assert t.guard_checks >= 1 and abs(t.guard_err - worst) <= 1e-3 * worst
     r = ens.sign_residual(ovl.PSI, ovl.sign_opts(degree=256, cut=0.05, precision=ovl.PREC_FP64))
     assert r < 1e-8
 

@@ -875,8 +875,8 @@ def test_guard_falls_back_to_fp64_below_the_cut(gpu):
 
 
 def test_guard_loosens_within_half_tol(gpu):
-    # relax > 0: a check whose error leaves 4x to spare below tol / 2 tries the ladder of 4 tol on the
-    # same block and keeps it while that check stays within tol / 2.  On 6^3 SU(2) at degree 256
+    # relax > 0: a check whose error leaves 4x to spare below tol / 2 tries the ladder of 4 tol on
+    # every mode block and keeps it while that check stays within tol / 2.  On 6^3 SU(2) at degree 256
     # the tol ladder's error is 4e-12, the looser one's 2e-11 (a level further would need 8e-11):
     # one level.  Every mode of the apply, and of a 3-step leapfrog after it, stays within tol of
     # the FP64 form; without relax the ladder stays at the tol level.
@@ -911,6 +911,20 @@ def test_guard_loosens_within_half_tol(gpu):
         err = max(np.linalg.norm(a - b) / np.linalg.norm(b) for a, b in zip(runs[1][0], runs[0][0]))
         print(f"  leapfrog: level {runs[1][1]}, mixed vs fp64 after 3 steps {err:.2e}", flush=True)
         assert runs[1][1] < 0 and err <= tol
+    # four mode blocks: the try of a looser ladder is checked on all of them, so the check's error is
+    # the apply's error over every mode (the periodic checks take one block in turn)
+    ctx, L, ucc, links, e, psi = make(nc, n, 128, 33, sigma=0.25, M=1.2, mb=32)
+    with ctx:
+        ctx.apply_sign(ovl.PSI, ovl.PSIMID, ovl.sign_opts(degree=deg, precision=ovl.PREC_FP64))
+        ref = ctx.get_modes(ovl.PSIMID)
+        ctx.apply_sign(ovl.PSI, ovl.SCRATCH, ovl.sign_opts(degree=deg, precision=ovl.PREC_MIXED, tol=tol, relax=2))
+        t = ctx.timing()
+        mix = ctx.get_modes(ovl.SCRATCH)
+        err = max(np.linalg.norm(a - b) / np.linalg.norm(b) for a, b in zip(mix, ref))
+        print(f"  4 blocks: level {t.guard_level}, checks {t.guard_checks}, check {t.guard_err:.4e}, "
+              f"all modes {err:.4e}", flush=True)
+        assert t.guard_level < 0 and err <= 0.5 * tol
+        assert abs(t.guard_err - err) <= 1e-3 * err
     with pytest.raises(ovl.OvlError):
         with ovl.Context(nc, n, nm) as c2:
             c2.set_links(links)
