@@ -5,8 +5,13 @@ quanta, 256 mode functions, Chebyshev sign of degree 384 (cut 0.05), rho = 1.2, 
 A step is one full leapfrog step of all modes: two overlap half steps (each an overlap
 apply of every mode plus the axpy into the other field) and two gauge-link half steps.
 
-value = site*mode updates/s of the mixed FP32/FP64 form (inputs resident in HBM); the plain
-FP64 form and the CPU oracle are timed beside it.  Prints one JSON line on stdout.
+value = site*mode updates/s of the mixed form (the Clenshaw recurrence stored in FP64, 48-bit,
+FP32 and FP16 tiers, its accuracy guard allowed to loosen the ladder by --relax levels), inputs
+resident in HBM, timed as hipGraph replay of the captured step; the plain FP64 form and the CPU
+oracle are timed beside it, with the roofline of the dominant stencil variant (HIP events around
+every launch of an eager step) and its HBM traffic (two rocprofv3 counter passes run as child
+processes).  Also timed: the 32-system ensemble, the three smaller workloads over their own run
+lengths, and the calls either side of the leapfrog.  Prints one JSON line on stdout.
 """
 from __future__ import annotations
 
