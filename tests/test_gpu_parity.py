@@ -316,6 +316,56 @@ def test_overlap_leapfrog_full_gauge_vs_oracle(gpu, prec):
     assert err < tol
 
 
+def _random_shape(case, seed=2026):
+    # (nc, lattice, modes, mode block, flux quanta, field seed) of case number `case`, drawn from one
+    # fixed stream: odd and uneven lattice sides, mode counts that leave the last block ragged
+    g = np.random.default_rng(seed)
+    for _ in range(case + 1):
+        nc = int(g.choice([2, 3]))
+        n = tuple(int(v) for v in g.integers(3, 8, size=3))
+        nm = int(g.integers(1, 71))
+        mb = int(g.choice([0, 16, 32, 64]))
+        nb = int(g.integers(0, 4))
+        fs = int(g.integers(1, 1 << 20))
+    return nc, n, nm, mb, nb, fs
+
+
+@pytest.mark.parametrize("case", range(8))
+def test_random_shapes_vs_oracle(gpu, case):
+    # the overlap leapfrog (links moving) on drawn shapes: FP64 against the oracle for the first and
+    # last mode; the mixed ladder (every tier at degree 256, the guard allowed to loosen) against
+    # FP64 for every mode; hipGraph replay bit-identical to stream launches
+    nc, n, nm, mb, nb, fs = _random_shape(case)
+    print(f"[case] random shape {case}: nc={nc} n={n} modes={nm} mb={mb} flux={nb}", flush=True)
+    deg, cut, dt, steps = 256, 0.05, 0.02, 2
+    ctx, L, ucc, links, e, psi = make(nc, n, nm, fs, sigma=0.3, mb=mb, M=1.2, nb=nb)
+    with ctx:
+        ctx.set_modes(ovl.PSIMID, psi)
+        so = ovl.sign_opts(degree=deg, cut=cut, precision=ovl.PREC_FP64)
+        ctx.fermion_update(ovl.StepOpts(ovl.OVERLAP, dt, ovl.GAUGE_LINKS, so, 0), steps)
+        p64, m64 = ctx.get_modes(ovl.PSI), ctx.get_modes(ovl.PSIMID)
+        sample = sorted({0, nm - 1})
+        oo = orc.StepOpts(orc.OVERLAP_CHEB, dt, 1, nb, 1.0, deg, cut, 0.0, 0.0, 8)
+        lk, p, pm, _ = orc.leapfrog(L, links, e, psi[sample], psi[sample], oo, steps)
+        assert rel(ctx.get_links(), lk) < 1e-13
+        for i, s in enumerate(sample):
+            assert rel(p64[s], p[i]) < 1e-12 and rel(m64[s], pm[i]) < 1e-12, (s, rel(p64[s], p[i]))
+        runs = []
+        for graph in (0, 1):
+            ctx.set_links(links)
+            ctx.set_modes(ovl.PSI, psi)
+            ctx.set_modes(ovl.PSIMID, psi)
+            so = ovl.sign_opts(degree=deg, cut=cut, precision=ovl.PREC_MIXED, tol=1e-10, relax=2)
+            ctx.fermion_update(ovl.StepOpts(ovl.OVERLAP, dt, ovl.GAUGE_LINKS, so, graph), steps)
+            t = ctx.timing()
+            runs.append((ctx.get_modes(ovl.PSI), ctx.get_modes(ovl.PSIMID), t.guard_level))
+        assert np.array_equal(runs[0][0], runs[1][0]) and np.array_equal(runs[0][1], runs[1][1])
+        err = max(max(np.linalg.norm(a - b) / np.linalg.norm(b) for a, b in zip(x, y))
+                  for x, y in ((runs[0][0], p64), (runs[0][1], m64)))
+        print(f"  guard level {runs[0][2]}, mixed vs fp64 {err:.2e}", flush=True)
+        assert err <= 1e-10
+
+
 def test_graph_replay_matches_stream(gpu):
     n = (6, 4, 4)
     outs = []
