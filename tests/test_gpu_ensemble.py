@@ -223,3 +223,62 @@ def test_ensemble_work_queues_are_permutations(gpu, nc, mb, nm):
     for other in outs[1:]:
         for a, b in zip(outs[0], other):
             assert np.isfinite(a).all() and np.array_equal(a, b)
+
+
+def _random_ensemble(case, seed=4051):
+    # (nc, system sides, flux per system, modes, mode block) of case number `case`, from one fixed
+    # stream: 2-6 systems of uneven, odd sides, ragged mode blocks
+    g = np.random.default_rng(seed)
+    for _ in range(case + 1):
+        nc = int(g.choice([2, 3]))
+        ns = int(g.integers(2, 7))
+        dims = [tuple(int(v) for v in g.integers(3, 8, size=3)) for _ in range(ns)]
+        flux = [int(v) for v in g.integers(0, 4, size=ns)]
+        nm = int(g.integers(1, 41))
+        mb = int(g.choice([0, 16, 32, 64]))
+    return nc, dims, flux, nm, mb
+
+
+@pytest.mark.parametrize("case", range(4))
+def test_ensemble_random_shapes(gpu, case):
+    # drawn ensembles through the overlap leapfrog with moving links: each system in FP64 against an
+    # ordinary context holding that system alone, the mixed ladder (every tier, the guard allowed
+    # to loosen) against FP64 per system, and hipGraph replay bit-identical to stream launches
+    nc, dims, flux, nm, mb = _random_ensemble(case)
+    print(f"[case] random ensemble {case}: nc={nc} systems={dims} flux={flux} modes={nm} mb={mb}", flush=True)
+    deg, cut, dt, steps, M = 256, 0.05, 0.02, 2, 1.2
+    sy = systems(nc, nm, 700 + 10 * case, dims=dims, sigma=0.3)
+
+    def opts(prec, graph):
+        so = ovl.sign_opts(degree=deg, cut=cut, precision=prec, tol=1e-10, relax=2)
+        return ovl.StepOpts(ovl.OVERLAP, dt, ovl.GAUGE_LINKS, so, graph)
+
+    ens = make_ensemble(nc, nm, sy, mb=mb, M=M, flux=flux)
+    runs = {}
+    for key, prec, graph in (("fp64", ovl.PREC_FP64, 0), ("mixed", ovl.PREC_MIXED, 0), ("mixed_graph", ovl.PREC_MIXED, 1)):
+        for k, (n, links, e, psi) in enumerate(sy):
+            ens.set_system_links(k, links)
+            ens.set_system_modes(k, ovl.PSI, psi)
+            ens.set_system_modes(k, ovl.PSIMID, psi)
+        ens.fermion_update(opts(prec, graph), steps)
+        runs[key] = [(ens.get_system_modes(k, ovl.PSI), ens.get_system_modes(k, ovl.PSIMID)) for k in range(len(sy))]
+    level = ens.timing().guard_level
+    ens.close()
+    worst = 0.0
+    for k, (n, links, e, psi) in enumerate(sy):
+        with ovl.Context(nc, n, nm, m_dw=M, mode_block=mb) as ctx:
+            ctx.set_links(links)
+            ctx.set_efield(e)
+            if flux[k]:
+                ctx.set_u1_flux(flux[k], 1.0)
+            ctx.set_modes(ovl.PSI, psi)
+            ctx.set_modes(ovl.PSIMID, psi)
+            ctx.fermion_update(opts(ovl.PREC_FP64, 0), steps)
+            single = (ctx.get_modes(ovl.PSI), ctx.get_modes(ovl.PSIMID))
+        for f in range(2):
+            assert rel(runs["fp64"][k][f], single[f]) < 1e-13, (k, f)
+            assert np.array_equal(runs["mixed"][k][f], runs["mixed_graph"][k][f]), (k, f)
+            worst = max(worst, max(np.linalg.norm(a - b) / np.linalg.norm(b)
+                                   for a, b in zip(runs["mixed"][k][f], runs["fp64"][k][f])))
+    print(f"  guard level {level}, mixed vs fp64 per system: max {worst:.2e}", flush=True)
+    assert worst <= 1e-10
