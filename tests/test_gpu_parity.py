@@ -366,6 +366,45 @@ def test_random_shapes_vs_oracle(gpu, case):
         assert err <= 1e-10
 
 
+@pytest.mark.parametrize("case", range(4))
+def test_random_shapes_wilson_and_observables(gpu, case):
+    # drawn shapes (another fixed stream) through the Wilson leapfrog with the full E update in FP64
+    # and, links moving only, with FP32 spinor arithmetic, against the oracle for the first and last
+    # mode; the FP64 observables of the evolved fields against the oracle's
+    nc, n, nm, mb, nb, fs = _random_shape(case, seed=77)
+    print(f"[case] random wilson shape {case}: nc={nc} n={n} modes={nm} mb={mb} flux={nb}", flush=True)
+    dt, steps = 0.05, 4
+    ctx, L, ucc, links, e, psi = make(nc, n, nm, fs, sigma=0.4, mb=mb, m=0.1, nb=nb, esig=0.3)
+    sample = sorted({0, nm - 1})
+    with ctx:
+        ctx.set_modes(ovl.PSIMID, psi)
+        so = ovl.StepOpts(ovl.WILSON, dt, ovl.GAUGE_FULL, ovl.sign_opts(precision=ovl.PREC_FP64), 1)
+        ctx.fermion_update(so, steps)
+        oo = orc.StepOpts(orc.WILSON, dt, 2, nb, 1.0, 0, 0.0, 0.0, 0.0, 8)
+        lk, p, pm, ee = orc.leapfrog(L, links, e, psi[sample], psi[sample], oo, steps)
+        assert rel(ctx.get_links(), lk) < 1e-13 and rel(ctx.get_efield(), ee) < 1e-12
+        for i, s in enumerate(sample):
+            assert rel(ctx.get_modes(ovl.PSI, s, 1)[0], p[i]) < 1e-12
+            assert rel(ctx.get_modes(ovl.PSIMID, s, 1)[0], pm[i]) < 1e-12
+        L2 = orc.lat(nc, n, 1.0, 0.1, 0.0)
+        assert np.allclose(ctx.energy(), orc.plaquette_energy(L2, lk, ee), rtol=1e-12, atol=0)
+        assert np.allclose(ctx.gauss_violation(), orc.gauss_violation(L2, lk, ee), rtol=1e-11, atol=1e-15)
+        ref = orc.edotb(L2, lk, ee) / (8 * np.pi ** 2)
+        assert abs(ctx.ncs_dot() - ref) <= 1e-12 * max(abs(ref), 1e-12)
+        # FP32 spinor arithmetic (links moving, E fixed) from the same start
+        ctx.set_links(links)
+        ctx.set_efield(e)
+        ctx.set_modes(ovl.PSI, psi)
+        ctx.set_modes(ovl.PSIMID, psi)
+        so = ovl.StepOpts(ovl.WILSON, dt, ovl.GAUGE_LINKS, ovl.sign_opts(precision=ovl.PREC_FP32), 1)
+        ctx.fermion_update(so, steps)
+        oo = orc.StepOpts(orc.WILSON, dt, 1, nb, 1.0, 0, 0.0, 0.0, 0.0, 8)
+        _, p, pm, _ = orc.leapfrog(L, links, e, psi[sample], psi[sample], oo, steps)
+        for i, s in enumerate(sample):
+            assert rel(ctx.get_modes(ovl.PSI, s, 1)[0], p[i]) < 2e-5
+            assert rel(ctx.get_modes(ovl.PSIMID, s, 1)[0], pm[i]) < 2e-5
+
+
 def test_graph_replay_matches_stream(gpu):
     n = (6, 4, 4)
     outs = []
