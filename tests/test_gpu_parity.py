@@ -260,15 +260,18 @@ def test_sign_residual_vs_oracle(gpu, nc, nb):
     assert abs(r - want) <= 1e-9 * want
 
 
-def test_zolotarev_overlap_batched_vs_oracle(gpu):
-    n = (4, 4, 4)
-    ctx, L, ucc, *_, psi = make(2, n, 20, 91, m=0.1, M=1.0, mb=16)
+@pytest.mark.parametrize("nc,n,nm,mb,nb", [(2, (4, 4, 4), 20, 16, 0), (3, (5, 3, 4), 13, 32, 2)])
+def test_zolotarev_overlap_batched_vs_oracle(gpu, nc, n, nm, mb, nb):
+    # the reference's rational sign (25-pole Zolotarev, multi-shift CG) batched over every mode,
+    # including a ragged SU(3) block with flux, against the oracle's restatement mode by mode
+    print(f"[case] zolotarev nc={nc} n={n} modes={nm} mb={mb} flux={nb}", flush=True)
+    ctx, L, ucc, *_, psi = make(nc, n, nm, 91 + nc, m=0.1, M=1.0, mb=mb, nb=nb)
     o = ovl.sign_opts(method=ovl.SIGN_ZOLOTAREV, lmin=0.2, lmax=30.0)
     ctx.apply_overlap(ovl.PSI, ovl.PSIMID, o)
     out = ctx.get_modes(ovl.PSIMID)
-    for k in range(20):
+    for k in range(nm):
         err = rel(out[k], orc.overlap_evo_zolo(L, ucc, psi[k], 0.2, 30.0)[0])
-        assert err < 1e-9
+        assert err < 1e-9, (k, err)
 
 
 def test_wilson_leapfrog_config0(gpu):
