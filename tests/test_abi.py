@@ -60,6 +60,16 @@ def test_create_rejects_bad_params_without_touching_device():
     assert ovl.lib().ovl_create(C.byref(p), C.byref(h)) == ovl.E_ARG
 
 
+def test_create_rejects_sizes_beyond_the_lane_limits():
+    # at most 2^26 sites, and (mode blocks) x sites x (block size) lanes below 2^31, both checked
+    # before any device call
+    h = C.c_void_p()
+    p = ovl.Params(2, (C.c_int * 3)(512, 512, 512), 16, 1.0, 0.0, 0.0, 16, 0, 0)  # 2^27 sites
+    assert ovl.lib().ovl_create(C.byref(p), C.byref(h)) == ovl.E_ARG and not h.value
+    p = ovl.Params(3, (C.c_int * 3)(256, 256, 256), 256, 1.0, 0.0, 0.0, 64, 0, 0)  # 2^24 sites x 256 modes
+    assert ovl.lib().ovl_create(C.byref(p), C.byref(h)) == ovl.E_ARG and not h.value
+
+
 def test_create_ensemble_rejects_bad_params_without_touching_device():
     def create(nc, dims, nmodes=8, mb=0):
         flat = (C.c_int * max(1, 3 * len(dims)))(*[v for d in dims for v in d])
