@@ -144,8 +144,12 @@ def _tier_starts_numpy(deg, cut, tol, fp16, fp48=True):
 
 def test_sign_plan_host_only():
     o = ovl.sign_opts(degree=384, cut=0.05, precision=ovl.PREC_MIXED, tol=1e-10)
-    assert ovl.sign_plan(o) == (12, 110, 204)  # the ladder of the 24^3 SU(3) bench runs
+    assert ovl.sign_plan(o) == (12, 110, 204)  # the ladder of the 24^3 SU(3) bench workload at tol
     assert ovl.sign_plan(o) == _tier_starts_numpy(384, 0.05, 1e-10, True)
+    # the accuracy guard's loosened level -1 plans for 4 tol with the head overrides dropped: the
+    # ladder the bench runs (FP64 1 step, 48-bit 96, FP32 94, FP16 194 of 385)
+    o4 = ovl.sign_opts(degree=384, cut=0.05, precision=ovl.PREC_MIXED, tol=4e-10)
+    assert ovl.sign_plan(o4) == (1, 97, 191) == _tier_starts_numpy(384, 0.05, 4e-10, True)
     o.fp48 = 0
     assert ovl.sign_plan(o) == (106, 106, 204) == _tier_starts_numpy(384, 0.05, 1e-10, True, False)
     o.fp16_tail = 0
